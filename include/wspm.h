@@ -1,0 +1,164 @@
+/*
+ * wspm.h -- C ABI of the MI355X (gfx950) implicit P1-FEM Richards time step.
+ *
+ * Plain C linkage, plain pointers and sizes.  "dev" pointers are HIP device
+ * pointers owned by the caller (e.g. torch tensor data_ptr()); "host" pointers
+ * are ordinary host memory.  Every call returns 0 on success or a negative
+ * WSPM_E* code; wspm_last_error() gives the message.  Numerical
+ * non-convergence is not an error: it is reported in the stats structs, as the
+ * reference reports nothing and accepts the iterate.
+ *
+ * One context = one mesh + one HIP stream; a context is not thread-safe,
+ * distinct contexts are independent.  All work is enqueued on the context's
+ * stream; entry points that return host-visible results synchronise it.
+ *
+ * Reference interfaces replaced (paths relative to the JAX-WSPM checkout):
+ *   wspm_ctx_create        src/mesh/loader.py:11-108 (mesh arrays) + the BCOO
+ *                          index construction of
+ *                          src/numerics/assembly_re_v2.py:114-152
+ *   wspm_bc_set            src/models/boundary.py:38-53, 603-686 (ordered
+ *                          Dirichlet lists, later list wins)
+ *   wspm_soil              src/models/van_genuchten.py:9-47,
+ *                          src/models/exponential.py:10-42 (vmapped)
+ *   wspm_assemble_richards src/numerics/assembly_re_v2.py:62-166 and
+ *                          src/numerics/assembly_re_3D.py:72-170, fused with
+ *                          the Dirichlet rows and the Jacobi diagonal of
+ *                          src/solvers/jax_linear_solver.py:12-36
+ *   wspm_spmv              the BCOO matvec inside jax.scipy.sparse.linalg
+ *   wspm_krylov            src/solvers/solver_types.py:7-68 ->
+ *                          jax_linear_solver.py:262-330 (cg/bicgstab/gmres)
+ *   wspm_richards_*        src/solvers/richards_solver_v2.py:107-168 and
+ *                          src/solvers/richards_solver_3D_v2.py:177-222
+ *                          (Picard loop of one implicit time step)
+ */
+#ifndef WSPM_H
+#define WSPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wspm_ctx wspm_ctx;
+
+enum {
+    WSPM_OK = 0,
+    WSPM_EINVAL = -1,   /* bad argument (shape, index, enum) */
+    WSPM_EHIP = -2,     /* HIP runtime error */
+    WSPM_ENOMEM = -3,   /* allocation failed */
+    WSPM_ESTATE = -4    /* call out of order (e.g. step before init) */
+};
+
+enum { WSPM_SOIL_GARDNER = 0, WSPM_SOIL_VAN_GENUCHTEN = 1 };
+enum { WSPM_CG = 0, WSPM_BICGSTAB = 1, WSPM_GMRES = 2 };
+enum { WSPM_PRECOND_NONE = 0, WSPM_PRECOND_JACOBI = 1 };
+enum { WSPM_FP64 = 0, WSPM_MIXED = 1 };
+enum { WSPM_PICARD_REL = 0, WSPM_PICARD_ABS = 1 };
+
+typedef struct {
+    int32_t iters;        /* CG/BiCGSTAB iterations; GMRES restart cycles    */
+    int32_t arnoldi;      /* GMRES Arnoldi steps (0 for CG/BiCGSTAB)          */
+    int32_t code;         /* BiCGSTAB k (-10/-11 breakdown); GMRES info (-1 NaN) */
+    int32_t refinements;  /* mixed precision: FP64 refinement sweeps          */
+    double res_norm;      /* final |r| (|M r| for GMRES)                      */
+    double b_norm;        /* |b|                                              */
+} wspm_krylov_stats;
+
+typedef struct {
+    int32_t picard_iters;
+    int32_t krylov_iters; /* summed over the Picard iterations of the step   */
+    int32_t converged;    /* 1 if err < picard_tol, 0 if the cap was hit      */
+    int32_t reserved;
+    double err;           /* last Picard error (relative or absolute)        */
+} wspm_step_stats;
+
+typedef struct {
+    int32_t soil_model;   /* WSPM_SOIL_*                                      */
+    int32_t method;       /* WSPM_CG / WSPM_BICGSTAB / WSPM_GMRES             */
+    int32_t precond;      /* WSPM_PRECOND_*                                   */
+    int32_t precision;    /* WSPM_FP64 or WSPM_MIXED                          */
+    double phi[6];        /* gardner: alpha0, theta_s, theta_r, Ks
+                             van Genuchten: alpha, theta_s, theta_r, n, m, Ks
+                             (config/settings.py:19-22, 47-48)               */
+    double tol;           /* Krylov rtol (reference 1e-10)                    */
+    double atol;          /* Krylov atol (reference 0)                        */
+    int32_t maxiter;      /* reference 100                                    */
+    int32_t restart;      /* GMRES restart (reference 30, <= 64)              */
+    double picard_tol;
+    int32_t picard_maxit;
+    int32_t picard_mode;  /* WSPM_PICARD_REL | WSPM_PICARD_ABS               */
+    double inner_tol;     /* mixed precision: FP32 inner relative tolerance   */
+} wspm_richards_params;
+
+/* Library ------------------------------------------------------------------ */
+int32_t wspm_version(void);
+/* Last error of ctx, or of the last failed wspm_ctx_create when ctx is NULL. */
+const char* wspm_last_error(const wspm_ctx* ctx);
+
+/* Mesh context ------------------------------------------------------------ */
+/* conn: host int32 [n_elems * (dim+1)], 0-based; coords: host double
+ * [n_nodes * dim] row-major (x0,y0[,z0],x1,...).  stream: hipStream_t or NULL. */
+int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems,
+                        const int32_t* conn, const double* coords, void* stream,
+                        wspm_ctx** out);
+void wspm_ctx_destroy(wspm_ctx* ctx);
+int32_t wspm_set_stream(wspm_ctx* ctx, void* stream);
+/* info (host int64[8]): n_nodes, n_elems, nnz, dim, max_row_len, n_bc, 0, 0 */
+int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info);
+/* Copy the CSR pattern to host: row_ptr int32[n+1], cols int32[nnz]. */
+int32_t wspm_get_pattern(const wspm_ctx* ctx, int32_t* row_ptr, int32_t* cols);
+
+/* Dirichlet rows: host arrays, applied in list order (a node listed twice takes
+ * the later value, as top-then-bottom in the reference). n_bc = 0 clears. */
+int32_t wspm_bc_set(wspm_ctx* ctx, int32_t n_bc, const int32_t* nodes, const double* values);
+
+/* Pointwise soil model over n nodes (dev pointers).  params as phi above. */
+int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
+                  const double* h, double* C, double* K, double* theta);
+
+/* Fused P1 assembly of the mixed-form Richards system into the fixed CSR
+ * pattern, with Dirichlet rows applied.  All dev pointers.  Ks_elem: per-element
+ * saturated conductivity (then K holds the relative conductivity) or NULL.
+ * dinv (Jacobi inverse diagonal) may be NULL. */
+int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
+                               const double* C, const double* K, const double* theta_m,
+                               const double* theta_n, double dt,
+                               double* A_vals, double* b, double* dinv);
+
+/* y = A x on the context's pattern (dev pointers). */
+int32_t wspm_spmv(wspm_ctx* ctx, const double* A_vals, const double* x, double* y);
+
+/* Preconditioned Krylov solve A x = b from x0 (dev pointers; x0 may equal x).
+ * dinv NULL = no preconditioner, else Jacobi with that inverse diagonal. */
+int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv,
+                    const double* b, const double* x0, double* x, double tol, double atol,
+                    int32_t maxiter, int32_t restart, wspm_krylov_stats* stats);
+
+/* Mixed precision: FP32 Krylov inner solves inside FP64 iterative refinement
+ * on the true residual; stops when |b - A x| <= max(tol |b|, atol) (FP64). */
+int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv,
+                          const double* b, const double* x0, double* x, double tol, double atol,
+                          int32_t maxiter, int32_t restart, double inner_tol, int32_t max_refine,
+                          wspm_krylov_stats* stats);
+
+/* Device-resident Richards time stepping ----------------------------------- */
+/* h0 dev [n].  Ks_elem dev [n_elems] or NULL.  Copies both. */
+int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm,
+                           const double* Ks_elem, const double* h0);
+/* One implicit step of size dt: Picard to convergence (or the cap). */
+int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats);
+/* Copy state (dev pointers, each may be NULL): h, theta(h), theta at the start
+ * of the last step. */
+int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* theta_n);
+
+/* Timing hooks for the benchmark: when enabled, the SpMV launches of the
+ * Krylov solvers are bracketed by HIP events on the context's stream. */
+int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
+/* Returns number of SpMV launches timed and their summed duration (ms). */
+int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WSPM_H */

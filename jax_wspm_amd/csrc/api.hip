@@ -1,0 +1,961 @@
+// C ABI, per-mesh context and host drivers of the implicit time step.
+//
+// The host side only enqueues work on the context's stream.  It reads the
+// device back at three places: after a Krylov solve (iteration chunk done?),
+// after each Picard iteration (the Picard error, which decides the loop), and
+// after each mixed-precision refinement sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "assemble.h"
+#include "krylov_kernels.h"
+#include "wspm_device.h"
+#include "wspm_internal.h"
+
+using namespace wspm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Err {};
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+            throw Err{};                                                                \
+        }                                                                               \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Soil / Picard kernels
+// ---------------------------------------------------------------------------
+struct SoilOp {
+    static constexpr int ND = 0;
+    SoilParams sp;
+    const double* h;
+    double *C, *K, *th;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double*) const {
+        double c, k, t;
+        soil_eval(sp, h[i], c, k, t);
+        C[i] = c; K[i] = k; th[i] = t;
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// |h - h_m|, |h| and the soil model at the new iterate (used by the next
+// Picard iteration, and as theta(h) once the step has converged).
+struct PicardOp {
+    static constexpr int ND = 2;
+    SoilParams sp;
+    const double* h;
+    const double* hm;
+    double *C, *K, *th;
+    int rel;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        const double hi = h[i];
+        const double d = hi - hm[i];
+        acc[0] += d * d;
+        acc[1] += hi * hi;
+        double c, k, t;
+        soil_eval(sp, hi, c, k, t);
+        C[i] = c; K[i] = k; th[i] = t;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->picard_d2 = t[0];
+        s->picard_h2 = t[1];
+        s->picard_err = rel ? sqrt(t[0]) / sqrt(t[1]) : sqrt(t[0]);
+    }
+};
+
+// Mixed precision: FP64 residual r = b - A x, |r|, |b|
+struct RefResid {
+    static constexpr int ND = 2;
+    const double* b;
+    const double* x;
+    double* r;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return x[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const double bi = b[i];
+        const double ri = bi - ax;
+        r[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->rnorm = sqrt(t[0]);
+        s->bnorm = sqrt(t[1]);
+    }
+};
+
+struct RefScale {  // r32 = r / |r|; d32 = 0
+    static constexpr int ND = 0;
+    const double* r;
+    float* r32;
+    float* d32;
+    double inv;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState* s) { inv = s->rnorm > 0.0 ? 1.0 / s->rnorm : 0.0; }
+    __device__ void row(int i, double*) const {
+        r32[i] = (float)(r[i] * inv);
+        d32[i] = 0.0f;
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct RefCorrect {  // x += |r| d32
+    static constexpr int ND = 0;
+    double* x;
+    const float* d32;
+    double scale;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState* s) { scale = s->rnorm; }
+    __device__ void row(int i, double*) const { x[i] += scale * (double)d32[i]; }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct ToF32 {
+    static constexpr int ND = 0;
+    const double* a;
+    float* o;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double*) const { o[i] = (float)a[i]; }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// GMRES pieces that are not elementwise --------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_gm_dots(KVec<T> v, int j, Red red) {
+    KState* s = red.st;
+    if (s->done || !s->active) return;
+    const int c0 = blockIdx.y * 8;
+    const int nb = gridDim.x;
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < v.n; i += nb * BLOCK) {
+        const double wi = (double)v.w[i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u <= j) acc[u] += (double)v.V[(long)(c0 + u) * v.ldv + i] * wi;
+    }
+    __shared__ double sh[8 * NWAVE];
+    __shared__ int is_last;
+    block_sum<8>(acc, sh);
+    const unsigned nblk = gridDim.x * gridDim.y;
+    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) red.partials[(size_t)bid * MAX_DOTS + u] = acc[u];
+        __threadfence();
+        const unsigned t = atomicAdd(red.counter, 1u);
+        is_last = (t == nblk - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    const volatile double* P = red.partials;
+    for (int l = threadIdx.x; l <= j; l += BLOCK) {
+        const int by = l / 8, u = l % 8;
+        double sum = 0.0;
+        for (int bx = 0; bx < nb; ++bx) sum += P[(size_t)(by * nb + bx) * MAX_DOTS + u];
+        s->hbuf[l] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *red.counter = 0u;
+        __threadfence();
+    }
+}
+
+__global__ void k_gm_cycle_begin(KState* s, double* H, int R) {
+    if (s->done) return;
+    for (int idx = threadIdx.x; idx < R * (R + 1); idx += blockDim.x) {
+        const int i = idx / (R + 1), l = idx % (R + 1);
+        H[idx] = (i == l) ? 1.0 : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        s->active = 1;
+        s->jdone = -1;
+    }
+}
+
+// y = (H H^T)^{-1} (H beta e1) by Cholesky (JAX _lstsq: normal equations +
+// solve(assume_a='pos')).  One workgroup, R <= 64.
+__global__ __launch_bounds__(64) void k_gm_lstsq(KState* s, const double* H, int R) {
+    if (s->done) return;
+    __shared__ double a2[MAX_RESTART * MAX_RESTART];
+    __shared__ double b2[MAX_RESTART];
+    const double beta = s->rnorm;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < R * R; idx += blockDim.x) {
+        const int i = idx / R, k = idx % R;
+        double acc = 0.0;
+        for (int l = 0; l <= R; ++l) acc += H[i * (R + 1) + l] * H[k * (R + 1) + l];
+        a2[idx] = acc;
+    }
+    for (int i = tid; i < R; i += blockDim.x) b2[i] = H[i * (R + 1)] * beta;
+    __syncthreads();
+    // in-place lower Cholesky
+    for (int c = 0; c < R; ++c) {
+        if (tid == 0) {
+            double d = a2[c * R + c];
+            for (int p = 0; p < c; ++p) d -= a2[c * R + p] * a2[c * R + p];
+            a2[c * R + c] = sqrt(d);
+        }
+        __syncthreads();
+        for (int i = c + 1 + tid; i < R; i += blockDim.x) {
+            double v = a2[i * R + c];
+            for (int p = 0; p < c; ++p) v -= a2[i * R + p] * a2[c * R + p];
+            a2[i * R + c] = v / a2[c * R + c];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double z[MAX_RESTART];
+        for (int i = 0; i < R; ++i) {
+            double v = b2[i];
+            for (int p = 0; p < i; ++p) v -= a2[i * R + p] * z[p];
+            z[i] = v / a2[i * R + i];
+        }
+        for (int i = R - 1; i >= 0; --i) {
+            double v = z[i];
+            for (int p = i + 1; p < R; ++p) v -= a2[p * R + i] * s->y[p];
+            s->y[i] = v / a2[i * R + i];
+        }
+    }
+}
+
+__global__ void k_nan_check(KState* s, const double* x, int n) {
+    // GMRES info: -1 if x has a NaN (jnp.isnan(norm(x))); run on one block.
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    int local = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) local |= (x[i] != x[i]);
+    if (local) atomicOr(&bad, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s->nan_flag = bad;
+        if (bad) s->code = -1;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+template <class T>
+struct KWork {
+    int n = 0;
+    T *r = nullptr, *rh = nullptr, *p = nullptr, *ph = nullptr, *q = nullptr, *sh = nullptr,
+      *t = nullptr, *z = nullptr, *w = nullptr, *V = nullptr;
+    int vcols = 0;
+};
+
+struct wspm_ctx {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int dim = 0, n = 0, E = 0;
+    int64_t nnz = 0;
+    int max_row = 0, max_block_nnz = 0;
+    int lanes = 8;
+    int spmv_grid = 1, vec_grid = 1;
+    HostPattern hp;
+    // device mesh/pattern
+    int32_t *row_ptr = nullptr, *cols = nullptr, *diag_pos = nullptr, *inc_ptr = nullptr,
+            *inc_code = nullptr, *conn = nullptr;
+    uint32_t* inc_slot = nullptr;
+    double *X = nullptr, *Y = nullptr, *Z = nullptr;
+    uint8_t* bc_flag = nullptr;
+    double* bc_val = nullptr;
+    int n_bc = 0;
+    QuadConst qc;
+    // reductions
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    KState* states = nullptr;     // [3]: main, inner, refinement
+    KState* hstate = nullptr;     // pinned mirror
+    double* H = nullptr;          // GMRES Hessenberg [MAX_RESTART][MAX_RESTART+1]
+    double* H32 = nullptr;
+    KWork<double> w64;
+    KWork<float> w32;
+    // mixed precision
+    double* ref_r = nullptr;
+    float *ref_r32 = nullptr, *ref_d32 = nullptr, *A32 = nullptr, *dinv32 = nullptr;
+    // Richards state
+    bool rich_init = false;
+    bool has_ks = false;
+    wspm_richards_params prm{};
+    SoilParams sp{};
+    double *hA = nullptr, *hB = nullptr, *Cn = nullptr, *Kn = nullptr, *thm = nullptr,
+           *thn = nullptr, *Ks = nullptr, *Av = nullptr, *bv = nullptr, *dinv = nullptr;
+    int last_iters[3] = {1, 1, 1};
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<void*> allocs;
+    std::vector<void*> host_allocs;
+};
+
+namespace {
+
+template <class P>
+P* dalloc(wspm_ctx* ctx, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(P));
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+        throw Err{};
+    }
+    ctx->allocs.push_back(p);
+    return static_cast<P*>(p);
+}
+
+template <class P>
+P* dupload(wspm_ctx* ctx, const P* src, size_t count) {
+    P* d = dalloc<P>(ctx, count);
+    HIPCHK(hipMemcpy(d, src, count * sizeof(P), hipMemcpyHostToDevice));
+    return d;
+}
+
+// Address of a field of a device-resident KState (no host dereference).
+template <class F>
+F* state_field(KState* ds, size_t off) {
+    return reinterpret_cast<F*>(reinterpret_cast<char*>(ds) + off);
+}
+
+Red red_of(wspm_ctx* ctx, int which) {
+    return Red{ctx->partials, ctx->counter, ctx->states + which};
+}
+
+template <class Op>
+void launch_vec(wspm_ctx* ctx, const Op& op, int which) {
+    hipLaunchKernelGGL(k_vec<Op>, dim3(ctx->vec_grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op,
+                       red_of(ctx, which));
+    HIPCHK(hipGetLastError());
+}
+
+hipEvent_t next_event(wspm_ctx* ctx) {
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+template <class T, class Epi>
+void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false) {
+    const bool t = timed && ctx->timing && ctx->ev_used + 2 <= (size_t)1 << 20;
+    if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
+    const Red red = red_of(ctx, which);
+    const dim3 g(ctx->spmv_grid), b(BLOCK);
+    switch (ctx->lanes) {
+        case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        case 8: hipLaunchKernelGGL((k_spmv<8, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        case 16: hipLaunchKernelGGL((k_spmv<16, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        case 32: hipLaunchKernelGGL((k_spmv<32, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        default: hipLaunchKernelGGL((k_spmv<64, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+    }
+    HIPCHK(hipGetLastError());
+    if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
+}
+
+void poll(wspm_ctx* ctx, int which) {
+    HIPCHK(hipMemcpyAsync(ctx->hstate + which, ctx->states + which, sizeof(KState),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+template <class T>
+KWork<T>& work_of(wspm_ctx* ctx);
+template <>
+KWork<double>& work_of<double>(wspm_ctx* ctx) { return ctx->w64; }
+template <>
+KWork<float>& work_of<float>(wspm_ctx* ctx) { return ctx->w32; }
+
+template <class T>
+KWork<T>& ensure_work(wspm_ctx* ctx, int vcols) {
+    KWork<T>& w = work_of<T>(ctx);
+    const int n = ctx->n;
+    if (w.r == nullptr) {
+        w.n = n;
+        w.r = dalloc<T>(ctx, n); w.rh = dalloc<T>(ctx, n); w.p = dalloc<T>(ctx, n);
+        w.ph = dalloc<T>(ctx, n); w.q = dalloc<T>(ctx, n); w.sh = dalloc<T>(ctx, n);
+        w.t = dalloc<T>(ctx, n); w.z = dalloc<T>(ctx, n); w.w = dalloc<T>(ctx, n);
+    }
+    if (vcols > w.vcols) {
+        w.V = dalloc<T>(ctx, (size_t)n * vcols);
+        HIPCHK(hipMemsetAsync(w.V, 0, (size_t)n * vcols * sizeof(T), ctx->stream));
+        w.vcols = vcols;
+    }
+    return w;
+}
+
+template <class T>
+KVec<T> make_kvec(wspm_ctx* ctx, KWork<T>& w, const T* dinv, const T* b, const T* x0, T* x) {
+    KVec<T> v{};
+    v.n = ctx->n;
+    v.b = b; v.x = x; v.x0 = x0;
+    v.r = w.r; v.rh = w.rh; v.q = w.q; v.t = w.t; v.w = w.w;
+    v.p = w.p;
+    v.ph = dinv ? w.ph : w.p;     // M = identity: p_hat is p
+    v.sh = dinv ? w.sh : w.r;     // s lives in r
+    v.z = dinv ? w.z : w.r;
+    v.V = w.V; v.ldv = ctx->n;
+    v.dinv = dinv;
+    return v;
+}
+
+// One Krylov solve.  Kernels early-exit once the device marks the solve done,
+// so iterations are enqueued in chunks and the host reads back once per chunk.
+template <class T>
+void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T* b, const T* x0, T* x,
+                  double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st) {
+    const int n = ctx->n;
+    KState& hs = ctx->hstate[which];
+    if (method == WSPM_BICGSTAB || method == WSPM_CG) {
+        KWork<T>& w = ensure_work<T>(ctx, 0);
+        KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+        if (method == WSPM_BICGSTAB) {
+            BicgInit<T> e{v, tol, atol, maxiter};
+            launch_spmv<T>(ctx, A, e, which, true);
+        } else {
+            CgInit<T> e{v, tol, atol, maxiter};
+            launch_spmv<T>(ctx, A, e, which, true);
+        }
+        int chunk = std::max(1, ctx->last_iters[method]);
+        int enq = 0;
+        while (true) {
+            for (int c = 0; c < chunk && enq < maxiter; ++c, ++enq) {
+                if (method == WSPM_BICGSTAB) {
+                    launch_vec(ctx, BicgP<T>{v, 0.0, 0.0}, which);
+                    launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true);
+                    launch_vec(ctx, BicgS<T>{v, 0.0}, which);
+                    launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true);
+                    launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which);
+                } else {
+                    launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true);
+                    launch_vec(ctx, CgXR<T>{v, 0.0}, which);
+                    launch_vec(ctx, CgP<T>{v, 0.0}, which);
+                }
+            }
+            poll(ctx, which);
+            if (hs.done || enq >= maxiter) break;
+            chunk = 1;
+        }
+        ctx->last_iters[method] = std::max(1, hs.steps);
+        if (st) {
+            st->iters = hs.steps;
+            st->arnoldi = 0;
+            st->code = hs.code;
+            st->res_norm = std::sqrt(hs.rs);
+            st->b_norm = std::sqrt(hs.bs);
+        }
+        return;
+    }
+    // GMRES (JAX solve_method='batched')
+    const int R = std::min(std::min(restart, n), MAX_RESTART);
+    KWork<T>& w = ensure_work<T>(ctx, R + 1);
+    KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+    double* H = (which == 1) ? ctx->H32 : ctx->H;
+    launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
+    launch_vec(ctx, GmScale<T>{v, 0.0}, which);
+    KState* ds = ctx->states + which;
+    const int nbx = std::min((n + BLOCK - 1) / BLOCK, 256);
+    for (int cyc = 0; cyc < maxiter; ++cyc) {
+        hipLaunchKernelGGL(k_gm_cycle_begin, dim3(1), dim3(256), 0, ctx->stream, ds, H, R);
+        HIPCHK(hipGetLastError());
+        for (int j = 0; j < R; ++j) {
+            launch_spmv<T>(ctx, (const T*)A, GmArnoldi<T>{v, j}, which, true);
+            hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
+                               red_of(ctx, which));
+            HIPCHK(hipGetLastError());
+            GmUpdate<T> up{v, j, H};
+            up.hb = state_field<double>(ds, offsetof(KState, hbuf));
+            launch_vec(ctx, up, which);
+            launch_vec(ctx, GmNormalize<T>{v, j, 0.0}, which);
+        }
+        hipLaunchKernelGGL(k_gm_lstsq, dim3(1), dim3(64), 0, ctx->stream, ds, H, R);
+        HIPCHK(hipGetLastError());
+        launch_vec(ctx, GmUpdX<T>{v, R, state_field<double>(ds, offsetof(KState, y))}, which);
+        launch_spmv<T>(ctx, A, GmResid<T>{v}, which, true);
+        launch_vec(ctx, GmScale<T>{v, 0.0}, which);
+        poll(ctx, which);
+        if (hs.done) break;
+    }
+    if (st) {
+        st->iters = hs.cycles;
+        st->arnoldi = hs.arnoldi;
+        st->code = 0;
+        st->res_norm = hs.rnorm;
+        st->b_norm = hs.bnorm;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+#define API_BEGIN try {
+#define API_END                                     \
+    }                                               \
+    catch (const Err&) {                            \
+        return ctx->err.find("hipMalloc") != std::string::npos ? WSPM_ENOMEM : WSPM_EHIP; \
+    }                                               \
+    catch (const std::exception& ex) {              \
+        ctx->err = ex.what();                       \
+        return WSPM_EHIP;                           \
+    }
+
+static int fail(wspm_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+extern "C" {
+
+int32_t wspm_version(void) { return 10000; }
+
+const char* wspm_last_error(const wspm_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+static void free_ctx(wspm_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    else (void)hipDeviceSynchronize();
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    for (void* p : ctx->host_allocs) (void)hipHostFree(p);
+    delete ctx;
+}
+
+int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32_t* conn,
+                        const double* coords, void* stream, wspm_ctx** out) {
+    if (!out || !conn || !coords) {
+        g_create_error = "null argument";
+        return WSPM_EINVAL;
+    }
+    *out = nullptr;
+    wspm_ctx* ctx = new wspm_ctx();
+    ctx->stream = (hipStream_t)stream;
+    ctx->dim = dim;
+    ctx->n = n_nodes;
+    ctx->E = n_elems;
+    std::string perr;
+    int rc = build_pattern(dim, n_nodes, n_elems, conn, ctx->hp, perr);
+    if (rc != WSPM_OK) {
+        g_create_error = perr;
+        delete ctx;
+        return rc;
+    }
+    try {
+        const HostPattern& P = ctx->hp;
+        ctx->nnz = (int64_t)P.cols.size();
+        ctx->max_row = P.max_row;
+        ctx->max_block_nnz = P.max_block_nnz;
+        if ((size_t)P.max_block_nnz * sizeof(double) > 64 * 1024) {
+            g_create_error = "assembly window exceeds 64 KiB of LDS (rows too long)";
+            delete ctx;
+            return WSPM_EINVAL;
+        }
+        const double avg = (double)ctx->nnz / n_nodes;
+        int lanes = 2;
+        while (lanes < 64 && lanes < avg) lanes *= 2;
+        ctx->lanes = lanes;
+        const long rows_per_block = BLOCK / lanes;
+        ctx->spmv_grid = (int)std::min<long>((n_nodes + rows_per_block - 1) / rows_per_block, MAX_BLOCKS);
+        ctx->vec_grid = std::min((n_nodes + BLOCK - 1) / BLOCK, 1024);
+        ctx->row_ptr = dupload(ctx, P.row_ptr.data(), P.row_ptr.size());
+        ctx->cols = dupload(ctx, P.cols.data(), P.cols.size());
+        ctx->diag_pos = dupload(ctx, P.diag_pos.data(), P.diag_pos.size());
+        ctx->inc_ptr = dupload(ctx, P.inc_ptr.data(), P.inc_ptr.size());
+        ctx->inc_code = dupload(ctx, P.inc_code.data(), P.inc_code.size());
+        ctx->inc_slot = dupload(ctx, P.inc_slot.data(), P.inc_slot.size());
+        ctx->conn = dupload(ctx, conn, (size_t)n_elems * (dim + 1));
+        std::vector<double> c(n_nodes);
+        double* dst[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < dim; ++d) {
+            for (int i = 0; i < n_nodes; ++i) c[i] = coords[(size_t)i * dim + d];
+            dst[d] = dupload(ctx, c.data(), n_nodes);
+        }
+        ctx->X = dst[0]; ctx->Y = dst[1]; ctx->Z = dst[2];
+        ctx->bc_flag = dalloc<uint8_t>(ctx, n_nodes);
+        ctx->bc_val = dalloc<double>(ctx, n_nodes);
+        HIPCHK(hipMemset(ctx->bc_flag, 0, n_nodes));
+        HIPCHK(hipMemset(ctx->bc_val, 0, n_nodes * sizeof(double)));
+        ctx->partials = dalloc<double>(ctx, (size_t)MAX_BLOCKS * MAX_DOTS);
+        ctx->counter = dalloc<unsigned>(ctx, 1);
+        HIPCHK(hipMemset(ctx->counter, 0, sizeof(unsigned)));
+        ctx->states = dalloc<KState>(ctx, 3);
+        HIPCHK(hipMemset(ctx->states, 0, 3 * sizeof(KState)));
+        void* hp = nullptr;
+        HIPCHK(hipHostMalloc(&hp, 3 * sizeof(KState), hipHostMallocDefault));
+        ctx->host_allocs.push_back(hp);
+        ctx->hstate = static_cast<KState*>(hp);
+        std::memset(ctx->hstate, 0, 3 * sizeof(KState));
+        ctx->H = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
+        ctx->H32 = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
+        compute_quad_const(dim, ctx->qc);
+        HIPCHK(hipDeviceSynchronize());
+    } catch (const Err&) {
+        g_create_error = ctx->err;
+        free_ctx(ctx);
+        return WSPM_EHIP;
+    }
+    *out = ctx;
+    return WSPM_OK;
+}
+
+void wspm_ctx_destroy(wspm_ctx* ctx) { free_ctx(ctx); }
+
+int32_t wspm_set_stream(wspm_ctx* ctx, void* stream) {
+    if (!ctx) return WSPM_EINVAL;
+    ctx->stream = (hipStream_t)stream;
+    return WSPM_OK;
+}
+
+int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info) {
+    if (!ctx || !info) return WSPM_EINVAL;
+    info[0] = ctx->n; info[1] = ctx->E; info[2] = ctx->nnz; info[3] = ctx->dim;
+    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->max_block_nnz;
+    return WSPM_OK;
+}
+
+int32_t wspm_get_pattern(const wspm_ctx* ctx, int32_t* row_ptr, int32_t* cols) {
+    if (!ctx || !row_ptr || !cols) return WSPM_EINVAL;
+    std::memcpy(row_ptr, ctx->hp.row_ptr.data(), ctx->hp.row_ptr.size() * sizeof(int32_t));
+    std::memcpy(cols, ctx->hp.cols.data(), ctx->hp.cols.size() * sizeof(int32_t));
+    return WSPM_OK;
+}
+
+int32_t wspm_bc_set(wspm_ctx* ctx, int32_t n_bc, const int32_t* nodes, const double* values) {
+    if (!ctx) return WSPM_EINVAL;
+    if (n_bc < 0 || (n_bc > 0 && (!nodes || !values))) return fail(ctx, WSPM_EINVAL, "bad BC arrays");
+    std::vector<uint8_t> flag(ctx->n, 0);
+    std::vector<double> val(ctx->n, 0.0);
+    for (int q = 0; q < n_bc; ++q) {
+        const int i = nodes[q];
+        if (i < 0 || i >= ctx->n) return fail(ctx, WSPM_EINVAL, "BC node out of range");
+        flag[i] = 1;
+        val[i] = values[q];  // later entries win
+    }
+    API_BEGIN
+    HIPCHK(hipMemcpyAsync(ctx->bc_flag, flag.data(), ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->bc_val, val.data(), ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->n_bc = n_bc;
+    return WSPM_OK;
+    API_END
+}
+
+static SoilParams make_soil(int model, const double* p) {
+    SoilParams sp{};
+    sp.model = model;
+    for (int k = 0; k < 6; ++k) sp.p[k] = p[k];
+    return sp;
+}
+
+int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n, const double* h,
+                  double* C, double* K, double* theta) {
+    if (!ctx) return WSPM_EINVAL;
+    if (model != WSPM_SOIL_GARDNER && model != WSPM_SOIL_VAN_GENUCHTEN) return fail(ctx, WSPM_EINVAL, "unknown soil model");
+    if (!params || !h || !C || !K || !theta || n != ctx->n) return fail(ctx, WSPM_EINVAL, "bad soil arguments");
+    API_BEGIN
+    double p[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < (model == WSPM_SOIL_GARDNER ? 4 : 6); ++k) p[k] = params[k];
+    launch_vec(ctx, SoilOp{make_soil(model, p), h, C, K, theta}, 0);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
+                        const double* K, const double* theta_m, const double* theta_n, double dt,
+                        double* A, double* b, double* dinv, float* A32, float* dinv32) {
+    AsmArgs a{};
+    a.n = ctx->n;
+    a.row_ptr = ctx->row_ptr; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
+    a.inc_code = ctx->inc_code; a.inc_slot = ctx->inc_slot; a.conn = ctx->conn;
+    a.X = ctx->X; a.Y = ctx->Y; a.Z = ctx->Z;
+    a.Kn = K; a.Ks_elem = Ks_elem; a.C = C; a.theta_m = theta_m; a.theta_n = theta_n; a.h_m = h_m;
+    a.inv_dt = 1.0 / dt;
+    a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
+    a.bc_val = ctx->bc_val;
+    a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
+    a.qc = ctx->qc;
+    HIPCHK(launch_assemble(ctx->dim, a, ctx->max_block_nnz, ctx->stream));
+}
+
+int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
+                               const double* K, const double* theta_m, const double* theta_n, double dt,
+                               double* A_vals, double* b, double* dinv) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!h_m || !C || !K || !theta_m || !theta_n || !A_vals || !b) return fail(ctx, WSPM_EINVAL, "null array");
+    if (!(dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
+    API_BEGIN
+    do_assemble(ctx, Ks_elem, h_m, C, K, theta_m, theta_n, dt, A_vals, b, dinv, nullptr, nullptr);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_spmv(wspm_ctx* ctx, const double* A_vals, const double* x, double* y) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!A_vals || !x || !y) return fail(ctx, WSPM_EINVAL, "null array");
+    API_BEGIN
+    launch_spmv<double>(ctx, A_vals, EpiPlain<double>{x, y}, 0, true);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+static int check_krylov_args(wspm_ctx* ctx, int method, int maxiter, int restart) {
+    if (method != WSPM_CG && method != WSPM_BICGSTAB && method != WSPM_GMRES)
+        return fail(ctx, WSPM_EINVAL, "Unsupported solver type");
+    if (maxiter < 0) return fail(ctx, WSPM_EINVAL, "maxiter must be >= 0");
+    if (method == WSPM_GMRES && (restart < 1 || restart > MAX_RESTART))
+        return fail(ctx, WSPM_EINVAL, "restart must be in [1, 64]");
+    return WSPM_OK;
+}
+
+int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv, const double* b,
+                    const double* x0, double* x, double tol, double atol, int32_t maxiter, int32_t restart,
+                    wspm_krylov_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!A_vals || !b || !x0 || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    int rc = check_krylov_args(ctx, method, maxiter, restart);
+    if (rc) return rc;
+    API_BEGIN
+    wspm_krylov_stats st{};
+    krylov_solve<double>(ctx, method, A_vals, dinv, b, x0, x, tol, atol, maxiter, restart, 0, &st);
+    if (method == WSPM_GMRES) {
+        hipLaunchKernelGGL(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
+        HIPCHK(hipGetLastError());
+        poll(ctx, 0);
+        st.code = ctx->hstate[0].nan_flag ? -1 : 0;
+    }
+    if (stats) *stats = st;
+    return WSPM_OK;
+    API_END
+}
+
+static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double* dinv, const float* A32,
+                        const float* dinv32, const double* b, double* x, double tol, double atol,
+                        int maxiter, int restart, double inner_tol, int max_refine, wspm_krylov_stats* st) {
+    if (!ctx->ref_r) {
+        ctx->ref_r = dalloc<double>(ctx, ctx->n);
+        ctx->ref_r32 = dalloc<float>(ctx, ctx->n);
+        ctx->ref_d32 = dalloc<float>(ctx, ctx->n);
+    }
+    int inner_total = 0, arn = 0, sweeps = 0;
+    KState& hs = ctx->hstate[2];
+    while (true) {
+        launch_spmv<double>(ctx, A, RefResid{b, x, ctx->ref_r}, 2, true);
+        poll(ctx, 2);
+        const double thresh = std::max(tol * hs.bnorm, atol);
+        if (!(hs.rnorm > thresh) || sweeps >= max_refine) break;
+        launch_vec(ctx, RefScale{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 2);
+        wspm_krylov_stats ist{};
+        krylov_solve<float>(ctx, method, A32, dinv32, ctx->ref_r32, ctx->ref_d32, ctx->ref_d32, inner_tol, 0.0,
+                            maxiter, restart, 1, &ist);
+        inner_total += ist.iters;
+        arn += ist.arnoldi;
+        launch_vec(ctx, RefCorrect{x, ctx->ref_d32, 0.0}, 2);
+        ++sweeps;
+    }
+    if (st) {
+        st->iters = inner_total;
+        st->arnoldi = arn;
+        st->code = 0;
+        st->refinements = sweeps;
+        st->res_norm = hs.rnorm;
+        st->b_norm = hs.bnorm;
+    }
+}
+
+int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv,
+                          const double* b, const double* x0, double* x, double tol, double atol, int32_t maxiter,
+                          int32_t restart, double inner_tol, int32_t max_refine, wspm_krylov_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!A_vals || !b || !x0 || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    int rc = check_krylov_args(ctx, method, maxiter, restart);
+    if (rc) return rc;
+    if (max_refine < 0) return fail(ctx, WSPM_EINVAL, "max_refine must be >= 0");
+    API_BEGIN
+    if (!ctx->A32) {
+        ctx->A32 = dalloc<float>(ctx, ctx->nnz);
+        ctx->dinv32 = dalloc<float>(ctx, ctx->n);
+    }
+    // FP32 copies of the operator (the Richards driver gets them from assembly)
+    {
+        ToF32 op{A_vals, ctx->A32};
+        const int g = (int)std::min<int64_t>((ctx->nnz + BLOCK - 1) / BLOCK, 4096);
+        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2));
+        HIPCHK(hipGetLastError());
+        if (dinv) launch_vec(ctx, ToF32{dinv, ctx->dinv32}, 2);
+    }
+    if (x != x0) HIPCHK(hipMemcpyAsync(x, x0, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    wspm_krylov_stats st{};
+    mixed_solve(ctx, method, A_vals, dinv, ctx->A32, dinv ? ctx->dinv32 : nullptr, b, x, tol, atol, maxiter, restart,
+                inner_tol, max_refine, &st);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats) *stats = st;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const double* Ks_elem,
+                           const double* h0) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!prm || !h0) return fail(ctx, WSPM_EINVAL, "null argument");
+    if (prm->soil_model != WSPM_SOIL_GARDNER && prm->soil_model != WSPM_SOIL_VAN_GENUCHTEN)
+        return fail(ctx, WSPM_EINVAL, "unknown soil model");
+    int rc = check_krylov_args(ctx, prm->method, prm->maxiter, prm->restart);
+    if (rc) return rc;
+    if (prm->precond != WSPM_PRECOND_NONE && prm->precond != WSPM_PRECOND_JACOBI)
+        return fail(ctx, WSPM_EINVAL, "unknown preconditioner");
+    if (prm->precision != WSPM_FP64 && prm->precision != WSPM_MIXED)
+        return fail(ctx, WSPM_EINVAL, "unknown precision");
+    if (prm->picard_maxit < 1) return fail(ctx, WSPM_EINVAL, "picard_maxit must be >= 1");
+    API_BEGIN
+    const int n = ctx->n;
+    if (!ctx->hA) {
+        ctx->hA = dalloc<double>(ctx, n); ctx->hB = dalloc<double>(ctx, n);
+        ctx->Cn = dalloc<double>(ctx, n); ctx->Kn = dalloc<double>(ctx, n);
+        ctx->thm = dalloc<double>(ctx, n); ctx->thn = dalloc<double>(ctx, n);
+        ctx->Av = dalloc<double>(ctx, ctx->nnz); ctx->bv = dalloc<double>(ctx, n);
+        ctx->dinv = dalloc<double>(ctx, n);
+    }
+    if (Ks_elem) {
+        if (!ctx->Ks) ctx->Ks = dalloc<double>(ctx, ctx->E);
+        HIPCHK(hipMemcpyAsync(ctx->Ks, Ks_elem, ctx->E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->prm = *prm;
+    double p[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; ++k) p[k] = prm->phi[k];
+    if (Ks_elem) p[prm->soil_model == WSPM_SOIL_GARDNER ? 3 : 5] = 1.0;  // nodal model gives Kr
+    ctx->sp = make_soil(prm->soil_model, p);
+    HIPCHK(hipMemcpyAsync(ctx->hA, h0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm}, 0);
+    if (prm->precision == WSPM_MIXED && !ctx->A32) {
+        ctx->A32 = dalloc<float>(ctx, ctx->nnz);
+        ctx->dinv32 = dalloc<float>(ctx, n);
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->rich_init = true;
+    ctx->has_ks = Ks_elem != nullptr;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!ctx->rich_init) return fail(ctx, WSPM_ESTATE, "wspm_richards_init not called");
+    if (!(dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
+    API_BEGIN
+    const int n = ctx->n;
+    const wspm_richards_params& P = ctx->prm;
+    const bool mixed = P.precision == WSPM_MIXED;
+    const bool jac = P.precond == WSPM_PRECOND_JACOBI;
+    const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
+    // theta_n = theta(h_n): the soil state at the start of the step is current
+    HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    int it = 0, kry = 0;
+    double err = INFINITY;
+    while (err >= P.picard_tol && it < P.picard_maxit) {
+        do_assemble(ctx, Ks, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
+                    jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr);
+        wspm_krylov_stats st{};
+        if (mixed) {
+            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
+                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st);
+        } else {
+            krylov_solve<double>(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
+                                 P.tol, P.atol, P.maxiter, P.restart, 0, &st);
+        }
+        kry += st.iters;
+        launch_vec(ctx, PicardOp{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm,
+                                 P.picard_mode == WSPM_PICARD_REL}, 0);
+        poll(ctx, 0);
+        err = ctx->hstate[0].picard_err;
+        std::swap(ctx->hA, ctx->hB);
+        ++it;
+        if (std::isnan(err)) break;
+    }
+    if (stats) {
+        stats->picard_iters = it;
+        stats->krylov_iters = kry;
+        stats->converged = err < P.picard_tol;
+        stats->reserved = 0;
+        stats->err = err;
+    }
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* theta_n) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!ctx->rich_init) return fail(ctx, WSPM_ESTATE, "wspm_richards_init not called");
+    API_BEGIN
+    const size_t bytes = ctx->n * sizeof(double);
+    if (h) HIPCHK(hipMemcpyAsync(h, ctx->hA, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (theta) HIPCHK(hipMemcpyAsync(theta, ctx->thm, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (theta_n) HIPCHK(hipMemcpyAsync(theta_n, ctx->thn, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on) {
+    if (!ctx) return WSPM_EINVAL;
+    API_BEGIN
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->timing = on != 0;
+    ctx->ev_used = 0;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms) {
+    if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
+    API_BEGIN
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+        tot += ms;
+    }
+    *launches = (int64_t)(ctx->ev_used / 2);
+    *total_ms = tot;
+    ctx->ev_used = 0;
+    return WSPM_OK;
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,478 @@
+// Krylov building blocks for gfx950: CSR SpMV with fused epilogues and fused
+// vector updates with deterministic dot products.
+//
+// Each Krylov step of jax.scipy.sparse.linalg.{cg,bicgstab,gmres} (called by
+// the reference at src/solvers/jax_linear_solver.py:282,305,329) is mapped to
+// a handful of kernels; every dot product the recurrence needs is computed in
+// the kernel that produces its operands, and the recurrence scalars are
+// updated on the device by the workgroup that finishes the reduction.  All
+// kernels return immediately once the solve is marked done, so the host can
+// enqueue several iterations ahead without reading anything back.
+#pragma once
+
+#include "wspm_device.h"
+
+namespace wspm {
+
+template <class T>
+struct KVec {
+    int n;
+    const T* b;
+    T* x;
+    const T* x0;
+    T *r, *rh, *p, *ph, *q, *sh, *t, *z, *w;
+    T* V;         // GMRES basis, column-major, leading dimension ldv
+    long ldv;
+    const T* dinv;  // Jacobi inverse diagonal or nullptr
+};
+
+// ---------------------------------------------------------------------------
+// CSR SpMV: LANES consecutive lanes own one row (LANES | 64), the row sum is
+// finished with a width-LANES butterfly, then lane 0 runs the epilogue.
+// ---------------------------------------------------------------------------
+template <int LANES, class T, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const T* __restrict__ av, Epi epi, Red red) {
+    if (epi.skip(red.st)) return;
+    constexpr int GPB = BLOCK / LANES;
+    constexpr int ND = Epi::ND;
+    const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
+    double acc[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
+    epi.load(red.st);
+    for (int row = blockIdx.x * GPB + g; row < n; row += gridDim.x * GPB) {
+        double s = 0.0;
+        const int kb = rp[row], ke = rp[row + 1];
+        for (int k = kb + l; k < ke; k += LANES) s += (double)av[k] * epi.xv(ci[k]);
+#pragma unroll
+        for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
+        if (l == 0) epi.row(row, s, acc);
+    }
+    if constexpr (ND > 0) {
+        double v[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) v[j] = acc[j];
+        grid_sum<ND>(v, red, [&](const double* t) { epi.fin(t, red.st); });
+    }
+}
+
+// Elementwise kernel with fused dot products.
+template <class Op>
+__global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red) {
+    if (op.skip(red.st)) return;
+    constexpr int ND = Op::ND;
+    double acc[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
+    op.load(red.st);
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) op.row(i, acc);
+    if constexpr (ND > 0) {
+        double v[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) v[j] = acc[j];
+        grid_sum<ND>(v, red, [&](const double* t) { op.fin(t, red.st); });
+    }
+}
+
+template <class T>
+__device__ inline double mval(const T* dinv, int i, double v) {
+    return dinv ? (double)dinv[i] * v : v;
+}
+
+// ------------------------------- plain SpMV --------------------------------
+template <class T>
+struct EpiPlain {
+    static constexpr int ND = 0;
+    const T* x;
+    T* y;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)x[j]; }
+    __device__ void row(int i, double ax, double*) const { y[i] = (T)ax; }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// ------------------------------- BiCGSTAB ----------------------------------
+template <class T>
+struct BicgInit {  // r = b - A x0; x = x0; r_hat = p = q = r
+    static constexpr int ND = 2;
+    KVec<T> v;
+    double tol, atol;
+    int maxiter;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.x0[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const double bi = (double)v.b[i];
+        const T rT = (T)(bi - ax);
+        v.x[i] = v.x0[i];
+        v.r[i] = rT; v.rh[i] = rT; v.p[i] = rT; v.q[i] = rT;
+        const double rr = (double)rT;
+        acc[0] += rr * rr;
+        acc[1] += bi * bi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->tol = tol; s->atol = atol; s->maxiter = maxiter;
+        s->bs = t[1];
+        s->atol2 = fmax(tol * tol * t[1], atol * atol);
+        s->rs = t[0];
+        s->rho_next = t[0];
+        s->rho = s->alpha = s->omega = 1.0;
+        s->k = 0; s->steps = 0; s->code = 0; s->exit_early = 0; s->nan_flag = 0;
+        s->done = !(s->rs > s->atol2 && 0 < maxiter);
+    }
+};
+
+template <class T>
+struct BicgP {  // p = r + beta (p - omega q); p_hat = M p
+    static constexpr int ND = 0;
+    KVec<T> v;
+    double beta, omega;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) {
+        beta = s->rho_next / s->rho * s->alpha / s->omega;
+        omega = s->omega;
+    }
+    __device__ void row(int i, double*) const {
+        const double pi = (double)v.r[i] + beta * ((double)v.p[i] - omega * (double)v.q[i]);
+        v.p[i] = (T)pi;
+        if (v.dinv) v.ph[i] = (T)((double)v.dinv[i] * (double)(T)pi);
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+template <class T>
+struct BicgQ {  // q = A p_hat; alpha = rho_ / (r_hat . q)
+    static constexpr int ND = 1;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.ph[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const T qT = (T)ax;
+        v.q[i] = qT;
+        acc[0] += (double)v.rh[i] * (double)qT;
+    }
+    __device__ void fin(const double* t, KState* s) const { s->alpha = s->rho_next / t[0]; }
+};
+
+template <class T>
+struct BicgS {  // s = r - alpha q (in place of r); s_hat = M s; |s|^2
+    static constexpr int ND = 1;
+    KVec<T> v;
+    double alpha;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { alpha = s->alpha; }
+    __device__ void row(int i, double* acc) const {
+        const T sT = (T)((double)v.r[i] - alpha * (double)v.q[i]);
+        v.r[i] = sT;
+        const double sd = (double)sT;
+        if (v.dinv) v.sh[i] = (T)((double)v.dinv[i] * sd);
+        acc[0] += sd * sd;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->ss = t[0];
+        s->exit_early = t[0] < s->atol2;
+    }
+};
+
+template <class T>
+struct BicgT {  // t = A s_hat; omega = (t.s)/(t.t)
+    static constexpr int ND = 2;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done || s->exit_early; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.sh[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const T tT = (T)ax;
+        v.t[i] = tT;
+        const double td = (double)tT;
+        acc[0] += td * (double)v.r[i];
+        acc[1] += td * td;
+    }
+    __device__ void fin(const double* t, KState* s) const { s->omega = t[0] / t[1]; }
+};
+
+template <class T>
+struct BicgXR {  // x, r update; |r|^2, r_hat.r; loop control
+    static constexpr int ND = 2;
+    KVec<T> v;
+    double alpha, omega;
+    int early;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) {
+        alpha = s->alpha; omega = s->omega; early = s->exit_early;
+    }
+    __device__ void row(int i, double* acc) const {
+        const double phi = (double)v.ph[i];
+        double ri;
+        if (early) {
+            v.x[i] = (T)((double)v.x[i] + alpha * phi);
+            ri = (double)v.r[i];
+        } else {
+            const double shi = (double)v.sh[i];
+            v.x[i] = (T)((double)v.x[i] + (alpha * phi + omega * shi));
+            const T rT = (T)((double)v.r[i] - omega * (double)v.t[i]);
+            v.r[i] = rT;
+            ri = (double)rT;
+        }
+        acc[0] += ri * ri;
+        acc[1] += (double)v.rh[i] * ri;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        int kk = s->k + 1;
+        if ((!s->exit_early && s->omega == 0.0) || s->alpha == 0.0) kk = -11;
+        if (s->rho_next == 0.0) kk = -10;
+        s->k = kk;
+        s->code = kk;
+        s->steps += 1;
+        s->rho = s->rho_next;
+        s->rho_next = t[1];
+        s->rs = t[0];
+        s->done = !(s->rs > s->atol2 && kk < s->maxiter && kk >= 0);
+    }
+};
+
+// ----------------------------------- CG ------------------------------------
+template <class T>
+struct CgInit {  // r = b - A x0; z = M r; p = z
+    static constexpr int ND = 3;
+    KVec<T> v;
+    double tol, atol;
+    int maxiter;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.x0[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const double bi = (double)v.b[i];
+        const T rT = (T)(bi - ax);
+        const double rd = (double)rT;
+        v.x[i] = v.x0[i];
+        v.r[i] = rT;
+        const T zT = v.dinv ? (T)((double)v.dinv[i] * rd) : rT;
+        if (v.dinv) v.z[i] = zT;
+        v.p[i] = zT;
+        acc[0] += rd * (double)zT;
+        acc[1] += rd * rd;
+        acc[2] += bi * bi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->tol = tol; s->atol = atol; s->maxiter = maxiter;
+        s->bs = t[2];
+        s->atol2 = fmax(tol * tol * t[2], atol * atol);
+        s->gamma = t[0];
+        s->rs = (v.dinv ? t[1] : t[0]);
+        s->k = 0; s->steps = 0; s->code = 0; s->nan_flag = 0;
+        s->done = !(s->rs > s->atol2 && 0 < maxiter);
+    }
+};
+
+template <class T>
+struct CgAp {  // q = A p; alpha = gamma / (p . q)
+    static constexpr int ND = 1;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.p[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const T qT = (T)ax;
+        v.q[i] = qT;
+        acc[0] += (double)v.p[i] * (double)qT;
+    }
+    __device__ void fin(const double* t, KState* s) const { s->alpha = s->gamma / t[0]; }
+};
+
+template <class T>
+struct CgXR {  // x += alpha p; r -= alpha q; z = M r; gamma', |r|^2
+    static constexpr int ND = 2;
+    KVec<T> v;
+    double alpha;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { alpha = s->alpha; }
+    __device__ void row(int i, double* acc) const {
+        v.x[i] = (T)((double)v.x[i] + alpha * (double)v.p[i]);
+        const T rT = (T)((double)v.r[i] - alpha * (double)v.q[i]);
+        v.r[i] = rT;
+        const double rd = (double)rT;
+        const T zT = v.dinv ? (T)((double)v.dinv[i] * rd) : rT;
+        if (v.dinv) v.z[i] = zT;
+        acc[0] += rd * (double)zT;
+        acc[1] += rd * rd;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->beta = t[0] / s->gamma;
+        s->gamma = t[0];
+        s->rs = v.dinv ? t[1] : t[0];
+        s->k += 1;
+        s->steps = s->k;
+        s->done = !(s->rs > s->atol2 && s->k < s->maxiter);
+    }
+};
+
+template <class T>
+struct CgP {  // p = z + beta p (the new p of JAX's body, computed after the test)
+    static constexpr int ND = 0;
+    KVec<T> v;
+    double beta;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { beta = s->beta; }
+    __device__ void row(int i, double*) const {
+        const double zi = v.dinv ? (double)v.z[i] : (double)v.r[i];
+        v.p[i] = (T)(zi + beta * (double)v.p[i]);
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// ---------------------------------- GMRES ----------------------------------
+__device__ inline double safe_norm(double sq, double thresh) {
+    const double nrm = sqrt(sq);
+    return nrm > thresh ? nrm : 0.0;
+}
+
+template <class T>
+struct GmInit {  // w = M (b - A x0); x = x0; |b|, |M b|, |w|
+    static constexpr int ND = 3;
+    KVec<T> v;
+    double tol, atol;
+    int maxiter, restart;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.x0[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const double bi = (double)v.b[i];
+        const T rT = (T)(bi - ax);
+        const T wT = v.dinv ? (T)((double)v.dinv[i] * (double)rT) : rT;
+        v.x[i] = v.x0[i];
+        v.w[i] = wT;
+        const double mb = (double)(v.dinv ? (T)((double)v.dinv[i] * bi) : (T)bi);
+        acc[0] += bi * bi;
+        acc[1] += mb * mb;
+        acc[2] += (double)wT * (double)wT;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->tol = tol; s->atol = atol; s->maxiter = maxiter; s->restart = restart;
+        s->bnorm = sqrt(t[0]);
+        const double atol_eff = fmax(tol * s->bnorm, atol);
+        s->mbnorm = sqrt(t[1]);
+        s->ptol = s->mbnorm * fmin(1.0, atol_eff / s->bnorm);
+        s->rnorm = safe_norm(t[2], 2.220446049250313e-16);
+        s->k = 0; s->cycles = 0; s->arnoldi = 0; s->code = 0; s->nan_flag = 0;
+        s->done = !(0 < maxiter && s->rnorm > s->ptol);
+    }
+};
+
+template <class T>
+struct GmResid {  // after a cycle: w = M (b - A x); |w|; loop control
+    static constexpr int ND = 1;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return (double)v.x[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const T rT = (T)((double)v.b[i] - ax);
+        const T wT = v.dinv ? (T)((double)v.dinv[i] * (double)rT) : rT;
+        v.w[i] = wT;
+        acc[0] += (double)wT * (double)wT;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->rnorm = safe_norm(t[0], 2.220446049250313e-16);
+        s->k += 1;
+        s->cycles = s->k;
+        s->done = !(s->k < s->maxiter && s->rnorm > s->ptol);
+    }
+};
+
+template <class T>
+struct GmScale {  // V[:, 0] = w / rnorm (0 when rnorm == 0)
+    static constexpr int ND = 0;
+    KVec<T> v;
+    double rn;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { rn = s->rnorm; }
+    __device__ void row(int i, double*) const {
+        v.V[i] = rn > 0.0 ? (T)((double)v.w[i] / rn) : (T)0;
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+template <class T>
+struct GmArnoldi {  // w = M (A V[:, j]); v0 = |w|
+    static constexpr int ND = 1;
+    KVec<T> v;
+    int j;
+    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int c) const { return (double)v.V[(long)j * v.ldv + c]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const T wT = v.dinv ? (T)((double)v.dinv[i] * (double)(T)ax) : (T)ax;
+        v.w[i] = wT;
+        acc[0] += (double)wT * (double)wT;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->v0 = safe_norm(t[0], 2.220446049250313e-16);
+    }
+};
+
+template <class T>
+struct GmUpdate {  // w -= V[:, :j+1] h; v1 = |w| (threshold eps * v0); H row j
+    static constexpr int ND = 1;
+    KVec<T> v;
+    int j;
+    double* H;   // [restart][restart+1], row-major
+    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        double wi = (double)v.w[i];
+        double qh = 0.0;
+        for (int l = 0; l <= j; ++l) qh += (double)v.V[(long)l * v.ldv + i] * hb[l];
+        const T wT = (T)(wi - qh);
+        v.w[i] = wT;
+        acc[0] += (double)wT * (double)wT;
+    }
+    const double* hb;
+    __device__ void fin(const double* t, KState* s) const {
+        const int R = s->restart;
+        const double v1 = safe_norm(t[0], 2.220446049250313e-16 * s->v0);
+        s->v1 = v1;
+        double* Hj = H + (long)j * (R + 1);
+        for (int l = 0; l <= R; ++l) Hj[l] = (l <= j) ? s->hbuf[l] : 0.0;
+        Hj[j + 1] = v1;
+        s->arnoldi += 1;
+        s->jdone = j;
+        if (v1 == 0.0) s->active = 0;
+    }
+};
+
+template <class T>
+struct GmNormalize {  // V[:, j+1] = w / v1 (0 when v1 == 0)
+    static constexpr int ND = 0;
+    KVec<T> v;
+    int j;
+    double v1;
+    __device__ bool skip(const KState* s) const { return s->done || s->jdone != j; }
+    __device__ void load(const KState* s) { v1 = s->v1; }
+    __device__ void row(int i, double*) const {
+        v.V[(long)(j + 1) * v.ldv + i] = v1 > 0.0 ? (T)((double)v.w[i] / v1) : (T)0;
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+template <class T>
+struct GmUpdX {  // x += V[:, :R] y
+    static constexpr int ND = 0;
+    KVec<T> v;
+    int R;
+    const double* y;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double*) const {
+        double d = 0.0;
+        for (int l = 0; l < R; ++l) d += (double)v.V[(long)l * v.ldv + i] * y[l];
+        v.x[i] = (T)((double)v.x[i] + d);
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+}  // namespace wspm

@@ -1,0 +1,108 @@
+// Host-side construction of the fixed CSR pattern and the row-owned
+// incidence map used by the deterministic assembly kernel.
+//
+// The reference rebuilds its sparse structure on every Picard iteration
+// (COO fill + BCOO.sum_duplicates, src/numerics/assembly_re_v2.py:119-152).
+// Here it is built once per mesh:
+//   * CSR pattern: union of the (i, j) pairs of every element, columns sorted;
+//   * diag_pos[i]: CSR index of A(i, i);
+//   * incidence lists: for each node i, the (element, local vertex) pairs
+//     that touch it, in increasing element order, plus for each such pair the
+//     in-row offsets of the element's vertices (one byte each, packed in a
+//     uint32).  The assembly kernel walks a row's incidences in this fixed
+//     order, so every CSR value is a sum in a fixed order: no atomics, and
+//     bitwise-repeatable results.
+#include "wspm_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace wspm {
+
+int build_pattern(int dim, int n, int E, const int32_t* conn, HostPattern& P, std::string& err) {
+    const int K = dim + 1;
+    if (dim != 2 && dim != 3) { err = "dim must be 2 or 3"; return WSPM_EINVAL; }
+    if (n <= 0 || E <= 0) { err = "empty mesh"; return WSPM_EINVAL; }
+    for (int64_t q = 0; q < (int64_t)E * K; ++q) {
+        if (conn[q] < 0 || conn[q] >= n) {
+            err = "element index out of bounds at entry " + std::to_string(q);
+            return WSPM_EINVAL;
+        }
+    }
+    if ((int64_t)E * 4 >= (int64_t)1 << 31) { err = "too many elements"; return WSPM_EINVAL; }
+
+    // node -> incident elements (counting sort, element order preserved)
+    P.inc_ptr.assign(n + 1, 0);
+    for (int64_t q = 0; q < (int64_t)E * K; ++q) P.inc_ptr[conn[q] + 1]++;
+    for (int i = 0; i < n; ++i) P.inc_ptr[i + 1] += P.inc_ptr[i];
+    const int64_t ninc = P.inc_ptr[n];
+    P.inc_code.resize(ninc);
+    {
+        std::vector<int32_t> fill(P.inc_ptr.begin(), P.inc_ptr.end() - 1);
+        for (int e = 0; e < E; ++e)
+            for (int a = 0; a < K; ++a) {
+                const int i = conn[(int64_t)e * K + a];
+                P.inc_code[fill[i]++] = e * 4 + a;
+            }
+    }
+
+    // CSR rows: sorted unique neighbours through incident elements
+    P.row_ptr.assign(n + 1, 0);
+    std::vector<int32_t> cols;
+    cols.reserve((size_t)ninc * 2);
+    std::vector<int32_t> tmp;
+    int max_row = 0;
+    for (int i = 0; i < n; ++i) {
+        tmp.clear();
+        for (int32_t q = P.inc_ptr[i]; q < P.inc_ptr[i + 1]; ++q) {
+            const int e = P.inc_code[q] >> 2;
+            for (int c = 0; c < K; ++c) tmp.push_back(conn[(int64_t)e * K + c]);
+        }
+        tmp.push_back(i);  // a node touched by no element still gets its diagonal
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        max_row = std::max<int>(max_row, (int)tmp.size());
+        cols.insert(cols.end(), tmp.begin(), tmp.end());
+        if ((int64_t)cols.size() >= ((int64_t)1 << 31) - 1) { err = "nnz exceeds int32"; return WSPM_EINVAL; }
+        P.row_ptr[i + 1] = (int32_t)cols.size();
+    }
+    if (max_row > 255) { err = "row longer than 255 entries"; return WSPM_EINVAL; }
+    P.cols.swap(cols);
+    P.max_row = max_row;
+
+    P.diag_pos.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int32_t* b = P.cols.data() + P.row_ptr[i];
+        const int32_t* e = P.cols.data() + P.row_ptr[i + 1];
+        P.diag_pos[i] = (int32_t)(std::lower_bound(b, e, i) - P.cols.data());
+    }
+
+    // in-row offsets of each incident element's vertices
+    P.inc_slot.resize(ninc);
+    for (int i = 0; i < n; ++i) {
+        const int32_t* b = P.cols.data() + P.row_ptr[i];
+        const int32_t* e = P.cols.data() + P.row_ptr[i + 1];
+        for (int32_t q = P.inc_ptr[i]; q < P.inc_ptr[i + 1]; ++q) {
+            const int el = P.inc_code[q] >> 2;
+            uint32_t packed = 0;
+            for (int c = 0; c < K; ++c) {
+                const int j = conn[(int64_t)el * K + c];
+                const uint32_t off = (uint32_t)(std::lower_bound(b, e, j) - b);
+                packed |= off << (8 * c);
+            }
+            P.inc_slot[q] = packed;
+        }
+    }
+
+    // LDS window per assembly block: nnz spanned by ASM_BLOCK consecutive rows
+    P.max_block_nnz = 0;
+    for (int r0 = 0; r0 < n; r0 += ASM_BLOCK) {
+        const int r1 = std::min(n, r0 + ASM_BLOCK);
+        P.max_block_nnz = std::max<int>(P.max_block_nnz, P.row_ptr[r1] - P.row_ptr[r0]);
+    }
+    return WSPM_OK;
+}
+
+}  // namespace wspm

@@ -1,0 +1,181 @@
+"""Device context over the C ABI: one mesh, its CSR pattern and its stream.
+
+PyTorch is used only for device memory (tensors whose ``data_ptr`` is handed
+to the C ABI) and for the current HIP stream.  Every computation runs in
+libwspm.so; if the library or a GPU is missing, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("jax_wspm_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _f64(t, device="cuda"):
+    if isinstance(t, torch.Tensor):
+        return t.to(device=device, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(t, dtype=np.float64), device=device)
+
+
+class Context:
+    """A mesh resident on the GPU (wraps ``wspm_ctx``)."""
+
+    def __init__(self, points: np.ndarray, elements: np.ndarray):
+        _require_gpu()
+        self.lib = _lib.load()
+        points = np.ascontiguousarray(points, dtype=np.float64)
+        conn = np.ascontiguousarray(elements, dtype=np.int32)
+        self.dim = points.shape[1]
+        if conn.shape[1] != self.dim + 1:
+            raise ValueError("elements must have dim+1 vertices")
+        self.n, self.n_elems = points.shape[0], conn.shape[0]
+        h = C.c_void_p()
+        rc = self.lib.wspm_ctx_create(self.dim, self.n, self.n_elems, ptr(conn), ptr(points), _stream(),
+                                      C.byref(h))
+        check(rc, None)
+        self.h = h
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        self.nnz = int(info[2])
+        self.max_row = int(info[4])
+        self.lanes = int(info[6])
+        self.device = torch.device("cuda")
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wspm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        check(rc, self.h)
+
+    # -- pattern ----------------------------------------------------------------
+    def pattern(self) -> Tuple[np.ndarray, np.ndarray]:
+        rp = np.empty(self.n + 1, dtype=np.int32)
+        ci = np.empty(self.nnz, dtype=np.int32)
+        self._ck(self.lib.wspm_get_pattern(self.h, ptr(rp), ptr(ci)))
+        return rp, ci
+
+    def set_dirichlet(self, bc_lists: Sequence[Tuple[np.ndarray, np.ndarray]]):
+        """Ordered (nodes, values) lists; a later list wins (top then bottom)."""
+        nodes, vals = [], []
+        for nd, v in bc_lists:
+            nd = np.asarray(nd, dtype=np.int32).ravel()
+            nodes.append(nd)
+            vals.append(np.broadcast_to(np.asarray(v, dtype=np.float64), nd.shape))
+        nodes = np.ascontiguousarray(np.concatenate(nodes) if nodes else np.zeros(0, np.int32))
+        vals = np.ascontiguousarray(np.concatenate(vals) if vals else np.zeros(0))
+        self._ck(self.lib.wspm_bc_set(self.h, int(nodes.size), ptr(nodes), ptr(vals)))
+
+    # -- kernels ------------------------------------------------------------------
+    def soil(self, model: str, params, h: torch.Tensor):
+        h = _f64(h)
+        p = np.zeros(6)
+        pa = np.asarray(params, dtype=np.float64)
+        p[: pa.size] = pa
+        Cc, K, th = (torch.empty_like(h) for _ in range(3))
+        self._ck(self.lib.wspm_soil(self.h, _lib.SOIL[model], ptr(p), self.n, ptr(h), ptr(Cc), ptr(K), ptr(th)))
+        return Cc, K, th
+
+    def assemble_richards(self, h_m, C_m, K_m, theta_m, theta_n, dt, Ks_elem=None, want_dinv=True):
+        args = [_f64(a) for a in (h_m, C_m, K_m, theta_m, theta_n)]
+        ks = _f64(Ks_elem) if Ks_elem is not None else None
+        A = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+        b = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        dinv = torch.empty(self.n, dtype=torch.float64, device=self.device) if want_dinv else None
+        self._ck(self.lib.wspm_assemble_richards(self.h, ptr(ks), *(ptr(a) for a in args), float(dt),
+                                                  ptr(A), ptr(b), ptr(dinv)))
+        return A, b, dinv
+
+    def spmv(self, A: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        x = _f64(x)
+        y = torch.empty_like(x)
+        self._ck(self.lib.wspm_spmv(self.h, ptr(A), ptr(x), ptr(y)))
+        return y
+
+    def krylov(self, method: str, A, b, x0, dinv=None, tol=1e-10, atol=0.0, maxiter=100, restart=30,
+               precision="fp64", inner_tol=1e-6, max_refine=50):
+        b, x0 = _f64(b), _f64(x0)
+        x = torch.empty_like(x0)
+        st = _lib.KrylovStats()
+        if method not in _lib.METHOD:
+            raise ValueError(f"Unsupported solver type: {method}")
+        if precision == "mixed":
+            rc = self.lib.wspm_krylov_mixed(self.h, _lib.METHOD[method], ptr(A), ptr(dinv), ptr(b), ptr(x0),
+                                            ptr(x), float(tol), float(atol), int(maxiter), int(restart),
+                                            float(inner_tol), int(max_refine), C.byref(st))
+        else:
+            rc = self.lib.wspm_krylov(self.h, _lib.METHOD[method], ptr(A), ptr(dinv), ptr(b), ptr(x0), ptr(x),
+                                      float(tol), float(atol), int(maxiter), int(restart), C.byref(st))
+        self._ck(rc)
+        return x, {"iters": st.iters, "arnoldi": st.arnoldi, "code": st.code, "refinements": st.refinements,
+                   "res_norm": st.res_norm, "b_norm": st.b_norm}
+
+    # -- device-resident time stepping --------------------------------------------
+    def richards_init(self, soil_model: str, phi, method: str, precond: Optional[str], h0, *,
+                      Ks_elem=None, tol=1e-10, atol=0.0, maxiter=100, restart=30, picard_tol=1e-6,
+                      picard_maxit=100, picard_mode="rel", precision="fp64", inner_tol=1e-6):
+        if method not in _lib.METHOD:
+            raise ValueError(f"Unsupported solver type: {method}")
+        if precond not in _lib.PRECOND:
+            raise NotImplementedError(f"preconditioner {precond!r} is not available on the MI355X path "
+                                      "(supported: none, jacobi)")
+        prm = _lib.RichardsParams()
+        prm.soil_model = _lib.SOIL[soil_model]
+        prm.method = _lib.METHOD[method]
+        prm.precond = _lib.PRECOND[precond]
+        prm.precision = _lib.PRECISION[precision]
+        pa = np.asarray(phi, dtype=np.float64)
+        for k in range(6):
+            prm.phi[k] = float(pa[k]) if k < pa.size else 0.0
+        prm.tol, prm.atol, prm.maxiter, prm.restart = float(tol), float(atol), int(maxiter), int(restart)
+        prm.picard_tol, prm.picard_maxit = float(picard_tol), int(picard_maxit)
+        prm.picard_mode = _lib.PICARD[picard_mode]
+        prm.inner_tol = float(inner_tol)
+        h0 = _f64(h0)
+        ks = _f64(Ks_elem) if Ks_elem is not None else None
+        self._ck(self.lib.wspm_richards_init(self.h, C.byref(prm), ptr(ks), ptr(h0)))
+
+    def richards_step(self, dt: float):
+        st = _lib.StepStats()
+        self._ck(self.lib.wspm_richards_step(self.h, float(dt), C.byref(st)))
+        return {"picard_iters": st.picard_iters, "krylov_iters": st.krylov_iters,
+                "converged": bool(st.converged), "err": st.err}
+
+    def richards_state(self):
+        h = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        th = torch.empty_like(h)
+        thn = torch.empty_like(h)
+        self._ck(self.lib.wspm_richards_state(self.h, ptr(h), ptr(th), ptr(thn)))
+        return h, th, thn
+
+    # -- timing hooks ---------------------------------------------------------------
+    def timing(self, on: bool):
+        self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self):
+        n = C.c_int64()
+        ms = C.c_double()
+        self._ck(self.lib.wspm_timing_read(self.h, C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)

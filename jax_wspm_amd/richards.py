@@ -1,0 +1,121 @@
+"""Richards solvers with the reference's ``Solver(config).solve() -> dict`` protocol.
+
+``RichardsSolver`` mirrors src/solvers/richards_solver_v2.py:42-352 and
+``RichardsSolver3D`` mirrors src/solvers/richards_solver_3D_v2.py:41-302.  The
+Picard loop of one step runs in libwspm (``wspm_richards_step``); this file
+keeps the host time loop of the reference: dt adaptation by iteration count
+(config/settings.py:85-97; richards_solver_v2.py:91-105), the time recorded
+before ``t += dt_new`` (richards_solver_v2.py:316,323), and theta stored at the
+end (2D v2) or at the start (3D, and the earlier 2D solver that produced the
+published CSVs) of each step.
+"""
+from __future__ import annotations
+
+import time
+from typing import Any, Dict, Optional, Union
+
+import numpy as np
+import torch
+
+from .config import SimulationConfig
+from .device import Context
+from .problems import Problem, from_config
+
+
+def adapt_time_step(dt: float, iters: int, m_it=5, M_it=7, amp=2.0, red=0.5) -> float:
+    if iters <= m_it:
+        return amp * dt
+    if iters <= M_it:
+        return dt
+    return red * dt
+
+
+class RichardsSolver:
+    """Solve a Richards problem on the GPU.
+
+    ``history``: "all" keeps h and theta of every step on the host (reference
+    behaviour), "last" keeps only the final step, an int k keeps every k-th
+    step plus the last."""
+
+    def __init__(self, config: Union[SimulationConfig, Problem], mesh_dir: Optional[str] = None,
+                 history: Union[str, int] = "all"):
+        self.problem = config if isinstance(config, Problem) else from_config(config, mesh_dir)
+        self.config = config
+        self.history = history
+        pb = self.problem
+        self.ctx = Context(pb.points, pb.elements)
+        self.ctx.set_dirichlet(pb.bc_lists)
+        self.points = pb.points
+        self.mesh_info = pb.mesh_info
+
+    def init_state(self):
+        pb = self.problem
+        self.ctx.richards_init(pb.soil_model, pb.phi, pb.solver, pb.precond, pb.h0, Ks_elem=pb.Ks_elem,
+                               tol=pb.tol, atol=pb.atol, maxiter=pb.maxiter, restart=pb.restart,
+                               picard_tol=pb.picard_tol, picard_maxit=pb.picard_maxit,
+                               picard_mode=pb.picard_mode, precision=pb.precision, inner_tol=pb.inner_tol)
+
+    def _keep(self, k: int) -> bool:
+        if self.history == "all":
+            return True
+        if isinstance(self.history, int) and self.history > 0:
+            return k % self.history == 0
+        return False
+
+    def solve(self, max_steps: Optional[int] = None) -> Dict[str, Any]:
+        pb = self.problem
+        self.init_state()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t = 0.0
+        dt = float(pb.dt0)
+        rec = {k: [] for k in ("pressure_head", "theta", "times", "iterations", "errors", "dt_values",
+                               "krylov_iterations")}
+        k = 0
+        last = None
+        while True:
+            if pb.nsteps is not None:
+                if k >= pb.nsteps:
+                    break
+            elif not t < pb.tmax:
+                break
+            if max_steps is not None and k >= max_steps:
+                break
+            st = self.ctx.richards_step(dt)
+            dt_new = (dt if pb.nsteps is not None
+                      else adapt_time_step(dt, st["picard_iters"], pb.m_it, pb.M_it, pb.lambda_amp, pb.lambda_red))
+            rec["times"].append(t)
+            rec["iterations"].append(st["picard_iters"])
+            rec["errors"].append(st["err"])
+            rec["dt_values"].append(dt_new)
+            rec["krylov_iterations"].append(st["krylov_iters"])
+            if self._keep(k):
+                h, th, thn = self.ctx.richards_state()
+                rec["pressure_head"].append(h.cpu().numpy())
+                rec["theta"].append((thn if pb.theta_at_start else th).cpu().numpy())
+                last = k
+            t += dt_new
+            dt = dt_new
+            k += 1
+        if k > 0 and last != k - 1:
+            h, th, thn = self.ctx.richards_state()
+            rec["pressure_head"].append(h.cpu().numpy())
+            rec["theta"].append((thn if pb.theta_at_start else th).cpu().numpy())
+        torch.cuda.synchronize()
+        sim = time.perf_counter() - t0
+        out: Dict[str, Any] = {key: np.array(v) for key, v in rec.items()}
+        out.update({
+            "points": pb.points,
+            ("tetrahedra" if pb.dim == 3 else "triangles"): pb.elements,
+            "simulation_time": sim,
+            "mesh_info": pb.mesh_info,
+            "config": self.config,
+        })
+        if out["theta"].size:
+            out["final_theta"] = out["theta"][-1]
+        return out
+
+
+class RichardsSolver3D(RichardsSolver):
+    """3D variant (src/solvers/richards_solver_3D_v2.py:41-302): same driver,
+    the problem definition carries the 3D Picard rule and theta-at-start."""

@@ -1,0 +1,181 @@
+"""GPU parity of the HIP kernels against the oracle, called through the C ABI.
+
+Tolerances: assembly entries agree to 1e-13 of their row's largest magnitude
+(the same sums in another order); b and the Jacobi diagonal to 1e-13
+relative; SpMV to 1e-14 relative per row scale; Krylov solutions to 1e-10
+relative L2 with identical iteration counts (the same recurrences, FP64).
+"""
+import numpy as np
+import pytest
+
+from oracle import assembly as oasm
+from oracle import krylov as okr
+from oracle import mesh as omesh
+from oracle import soil as osoil
+
+from tests._helpers import rel_l2, row_scaled_err
+
+pytestmark = pytest.mark.gpu
+
+VG = (0.1, 0.40, 0.06, 2.0, 0.5, 1e-3)
+GARDNER = (0.164, 0.45, 0.15, 0.1)
+
+
+def _ctx(points, elems):
+    from jax_wspm_amd.device import Context
+    return Context(points, elems)
+
+
+def _state(n, seed, lo=-1200.0, hi=-5.0):
+    rng = np.random.default_rng(seed)
+    h_m = rng.uniform(lo, hi, n)
+    h_n = h_m + rng.uniform(-3.0, 3.0, n)
+    return h_m, h_n
+
+
+CASES = [
+    ("tri8_vg", lambda: omesh.structured_tri(8, 1.0), "vg", VG, False),
+    ("tri25_gardner", lambda: omesh.structured_tri(25, 15.24), "gardner", GARDNER, False),
+    ("tri32_vg_ks", lambda: omesh.structured_tri(32, 1.0), "vg", VG, True),
+    ("tet6_vg", lambda: omesh.structured_tet(6, 1.0), "vg", VG, False),
+    ("tet20_gardner", lambda: omesh.structured_tet(20, 15.24), "gardner", GARDNER, False),
+    ("tet5_vg_ks", lambda: omesh.structured_tet(5, 1.0), "vg", VG, True),
+]
+
+
+@pytest.mark.parametrize("name,mk,model,phi,use_ks", CASES, ids=[c[0] for c in CASES])
+def test_pattern_soil_assembly(name, mk, model, phi, use_ks):
+    import torch
+    pts, el = mk()
+    n = len(pts)
+    print(f"[case] {name} n={n} E={len(el)}", flush=True)
+    ctx = _ctx(pts, el)
+    pat = oasm.Pattern(el, n)
+    rp, ci = ctx.pattern()
+    assert np.array_equal(rp, pat.indptr) and np.array_equal(ci, pat.indices)
+
+    h_m, h_n = _state(n, 1)
+    h_m[:: 7] = np.abs(h_m[:: 7]) * 0.01   # some saturated nodes (h >= 0 branch)
+    phi_nodal = np.array(phi, dtype=np.float64)
+    ks = None
+    if use_ks:
+        ks = np.exp(np.log(1e-3) + 0.5 * np.random.default_rng(3).standard_normal(len(el)))
+        phi_nodal = phi_nodal.copy()
+        phi_nodal[-1] = 1.0
+    Cm, Km, thm = osoil.soil(model, h_m, phi_nodal)
+    _, _, thn = osoil.soil(model, h_n, phi_nodal)
+
+    gC, gK, gth = ctx.soil(model, phi_nodal, torch.as_tensor(h_m, device="cuda"))
+    for g, o in ((gC, Cm), (gK, Km), (gth, thm)):
+        g = g.cpu().numpy()
+        assert np.allclose(g, o, rtol=1e-13, atol=1e-300), np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-300))
+
+    # Dirichlet: two overlapping ordered lists (later wins)
+    top = np.nonzero(pts[:, -1] == pts[:, -1].max())[0]
+    side = np.nonzero(pts[:, 0] == pts[:, 0].min())[0]
+    bcl = [(top, np.linspace(-10.0, -20.0, top.size)), (side, np.full(side.size, -15.0))]
+    ctx.set_dirichlet(bcl)
+    dt = 1e-4
+    vals, b = oasm.assemble(pat, pts, thm, thn, h_m, Km, Cm, dt, bcl, ks)
+    dinv = oasm.jacobi_inverse(pat, vals)
+    A, gb, gd = ctx.assemble_richards(h_m, Cm, Km, thm, thn, dt, Ks_elem=ks)
+    A, gb, gd = A.cpu().numpy(), gb.cpu().numpy(), gd.cpu().numpy()
+    assert row_scaled_err(A, vals, pat.indptr) <= 1e-13
+    assert rel_l2(gb, b) <= 1e-13 and np.max(np.abs(gb - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(gd - dinv) / np.abs(dinv)) <= 1e-13
+
+    # SpMV against scipy on the oracle's values
+    x = np.random.default_rng(5).standard_normal(n)
+    y = ctx.spmv(torch.as_tensor(vals, device="cuda"), torch.as_tensor(x, device="cuda")).cpu().numpy()
+    yr = pat.to_scipy(vals) @ x
+    scale = np.abs(pat.to_scipy(np.abs(vals)) @ np.abs(x))
+    assert np.max(np.abs(y - yr) / np.maximum(scale, 1e-300)) <= 1e-14
+    ctx.close()
+
+
+def _system(cells=12, dim=2, seed=0, dt=1e-4):
+    pts, el = (omesh.structured_tri if dim == 2 else omesh.structured_tet)(cells, 1.0)
+    n = len(pts)
+    pat = oasm.Pattern(el, n)
+    rng = np.random.default_rng(seed)
+    h_m = rng.uniform(-1000.0, -50.0, n)
+    Cm, Km, thm = osoil.van_genuchten(h_m, VG)
+    _, _, thn = osoil.van_genuchten(h_m - 1.0, VG)
+    top = np.nonzero(pts[:, -1] == 1.0)[0]
+    bcl = [(top, np.full(top.size, -75.0))]
+    vals, b = oasm.assemble(pat, pts, thm, thn, h_m, Km, Cm, dt, bcl)
+    return pts, el, pat, vals, b, h_m, bcl
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab", "gmres"])
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_krylov_matches_oracle(method, precond, dim):
+    import torch
+    print(f"[case] krylov {method} {precond} dim={dim}", flush=True)
+    # a stiffer system (large dt) so the solvers take several iterations
+    pts, el, pat, vals, b, h_m, _ = _system(cells=14 if dim == 2 else 5, dim=dim, dt=50.0)
+    A = pat.to_scipy(vals)
+    dinv = oasm.jacobi_inverse(pat, vals) if precond == "jacobi" else None
+    x_o, info = okr.solve(method, A.dot, b, h_m, dinv, 1e-10, 0.0, 100, 30)
+    ctx = _ctx(pts, el)
+    dv = torch.as_tensor(vals, device="cuda")
+    x_g, st = ctx.krylov(method, dv, torch.as_tensor(b, device="cuda"), torch.as_tensor(h_m, device="cuda"),
+                         dinv=None if dinv is None else torch.as_tensor(dinv, device="cuda"),
+                         tol=1e-10, maxiter=100, restart=30)
+    x_g = x_g.cpu().numpy()
+    assert st["iters"] == info["iters"], (st, info)
+    assert rel_l2(x_g, x_o) <= 1e-10
+    ctx.close()
+
+
+def test_krylov_small_restart_and_truncation():
+    """GMRES with restart 5 needs several cycles; CG truncated at maxiter=3."""
+    import torch
+    pts, el, pat, vals, b, h_m, _ = _system(cells=10, dim=2, dt=500.0)
+    A = pat.to_scipy(vals)
+    dinv = oasm.jacobi_inverse(pat, vals)
+    ctx = _ctx(pts, el)
+    dv, db, dx0, dd = (torch.as_tensor(a, device="cuda") for a in (vals, b, h_m, dinv))
+    x_o, info = okr.gmres(A.dot, b, h_m, 1e-10, 0.0, 5, 100, dinv)
+    x_g, st = ctx.krylov("gmres", dv, db, dx0, dinv=dd, restart=5)
+    assert st["iters"] == info["iters"] and st["arnoldi"] == info["arnoldi"]
+    assert rel_l2(x_g.cpu().numpy(), x_o) <= 1e-10
+    x_o, info = okr.cg(A.dot, b, h_m, 1e-10, 0.0, 3, dinv)
+    x_g, st = ctx.krylov("cg", dv, db, dx0, dinv=dd, maxiter=3)
+    assert st["iters"] == 3 == info["iters"]
+    assert rel_l2(x_g.cpu().numpy(), x_o) <= 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab", "gmres"])
+def test_mixed_precision_refinement(method):
+    import torch
+    print(f"[case] mixed {method}", flush=True)
+    pts, el, pat, vals, b, h_m, _ = _system(cells=16, dim=2, dt=5.0)
+    A = pat.to_scipy(vals)
+    dinv = oasm.jacobi_inverse(pat, vals)
+    x_ref = np.linalg.solve(A.toarray(), b)
+    ctx = _ctx(pts, el)
+    dv, db, dx0, dd = (torch.as_tensor(a, device="cuda") for a in (vals, b, h_m, dinv))
+    x_g, st = ctx.krylov(method, dv, db, dx0, dinv=dd, precision="mixed", tol=1e-12, inner_tol=1e-5)
+    assert st["refinements"] >= 1
+    assert rel_l2(x_g.cpu().numpy(), x_ref) <= 1e-10
+    ctx.close()
+
+
+def test_deterministic_repeat():
+    """Same inputs -> bitwise identical assembly and solve (no float atomics)."""
+    import torch
+    pts, el, pat, vals, b, h_m, bcl = _system(cells=40, dim=2, dt=1.0)
+    ctx = _ctx(pts, el)
+    ctx.set_dirichlet(bcl)
+    Cm, Km, thm = osoil.van_genuchten(h_m, VG)
+    outs = []
+    for _ in range(2):
+        A, gb, gd = ctx.assemble_richards(h_m, Cm, Km, thm, thm, 1.0)
+        x, _ = ctx.krylov("bicgstab", A, gb, torch.as_tensor(h_m, device="cuda"), dinv=gd)
+        outs.append((A.cpu().numpy(), gb.cpu().numpy(), x.cpu().numpy()))
+    for a, c in zip(outs[0], outs[1]):
+        assert np.array_equal(a, c)
+    ctx.close()

@@ -1,0 +1,129 @@
+"""End-to-end GPU parity of the implicit time step (Picard + Krylov + assembly).
+
+Bars: FP64 trajectories equal the oracle's to <= 1e-10 relative L2 in h and
+theta at every recorded step with identical Picard iteration counts; the
+reference's published fixtures (3D NPZ trajectories, Test1-3 CSV metrics)
+reproduce at their own precision; the mixed-precision form stays within 1e-8
+relative L2 of the FP64 form.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import picard as opicard
+from oracle import tracy
+
+from tests._helpers import rel_l2, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(test_case, **kw):
+    from jax_wspm_amd.config import PicardParameters, SimulationConfig, SolverParameters, TimeSteppingParameters
+    cfg = SimulationConfig(
+        time=TimeSteppingParameters(dt_init=kw.get("dt", 1e-3), Tmax=kw.get("tmax", 0.25)),
+        solver=SolverParameters(solver_type=kw.get("solver", "cg"), precond_type=kw.get("precond", "none")),
+        test_case=test_case, picard=PicardParameters(rule=kw.get("rule", "published")))
+    cfg.mesh.mesh_size = kw.get("mesh", "25")
+    return cfg
+
+
+def _run(pb):
+    from jax_wspm_amd.richards import RichardsSolver
+    s = RichardsSolver(pb)
+    out = s.solve()
+    s.ctx.close()
+    return out
+
+
+def _compare(g, o, tol=1e-10):
+    assert list(g["iterations"]) == list(o["iterations"])
+    assert np.allclose(g["times"], o["times"], rtol=0, atol=0)
+    for k in range(len(o["times"])):
+        assert rel_l2(g["pressure_head"][k], o["pressure_head"][k]) <= tol, k
+        assert rel_l2(g["theta"][k], o["theta"][k]) <= tol, k
+
+
+@pytest.mark.parametrize("tc", ["Test1", "Test2", "Test3"])
+def test_tracy_published_metrics(tc, golden):
+    """Test1-3 on the 25x25 mesh reproduce resultsRE2D/*.csv (5 significant
+    figures, summed Picard iterations) and the oracle trajectory."""
+    from jax_wspm_amd.problems import from_config
+    print(f"[case] {tc} mesh 25", flush=True)
+    pb = from_config(_cfg(tc, solver="cg", precond="none"))
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    _compare(g, o)
+    rows = json.loads((golden / "tracy_metrics.json").read_text())
+    row = next(r for r in rows if r["test_case"] == tc and r["mesh_size"] == "25" and r["solver"] == "cg"
+               and r["preconditioner"] == "none")
+    assert int(row["iterations"]) == int(np.sum(g["iterations"]))
+    e = tracy.final_errors(pb.points, g["pressure_head"][-1], g["theta"][-1], g["times"][-1], int(tc[-1]))
+    for key in ("l2_pressure", "linf_pressure", "l2_saturation", "linf_saturation", "l2_relative_pressure",
+                "l2_relative_saturation"):
+        assert f"{e[key]:.4e}" == row[key], (key, e[key], row[key])
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "gmres"])
+def test_test3d_matches_reference_npz(solver, golden):
+    """Test3D (20^3 Kuhn mesh, Gardner) against results3D/*.npz of the reference."""
+    from jax_wspm_amd.problems import from_config
+    print(f"[case] Test3D {solver}", flush=True)
+    pb = from_config(_cfg("Test3D", solver=solver, dt=1e-4, tmax=0.1))
+    g = _run(pb)
+    z = np.load(golden / f"test3d_{solver}.npz")
+    assert list(g["iterations"]) == list(z["iterations"])
+    assert np.array_equal(g["times"], z["times"]) and np.array_equal(g["dt_values"], z["dt_values"])
+    for s, h in zip(z["pressure_head_steps"], z["pressure_head"]):
+        assert rel_l2(g["pressure_head"][s], h) <= 1e-10
+    assert rel_l2(g["theta"][-1], z["theta_final"]) <= 1e-10
+    assert np.allclose(g["errors"], z["errors"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("solver,precond", [("gmres", "jacobi"), ("bicgstab", "jacobi"), ("cg", "none")])
+def test_vg_square_adaptive_matches_oracle(solver, precond):
+    """van Genuchten 25x25 unit square, top Dirichlet -75, adaptive dt from 1e-4 to 0.25."""
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print(f"[case] VGSquare 25 {solver} {precond}", flush=True)
+    pb = from_config(baseline_vg_config(25, 2, solver=solver, precond=precond, dt=1e-4, tmax=0.25))
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    _compare(g, o)
+
+
+def test_vg_square_heterogeneous_ks_fixed_steps():
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print("[case] VGSquare 48 Ks field", flush=True)
+    pb = from_config(baseline_vg_config(48, 2, solver="bicgstab", precond="jacobi", nsteps=6, dt=1e-4,
+                                        ks_field={"sigma": 0.5, "length": 0.05, "seed": 7}))
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    _compare(g, o)
+
+
+def test_vg_cube_fixed_steps():
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print("[case] VGCube 8", flush=True)
+    pb = from_config(baseline_vg_config(8, 3, solver="cg", precond="jacobi", nsteps=4, dt=1e-4))
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    _compare(g, o)
+
+
+@pytest.mark.parametrize("dim,solver", [(2, "bicgstab"), (3, "cg"), (2, "gmres")])
+def test_mixed_precision_vs_fp64(dim, solver):
+    """FP32 inner Krylov + FP64 refinement/Picard stays within 1e-8 of FP64."""
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print(f"[case] mixed dim={dim} {solver}", flush=True)
+    cells = 40 if dim == 2 else 10
+    pb64 = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", nsteps=5, dt=1e-4))
+    pbmx = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", nsteps=5, dt=1e-4,
+                                          precision="mixed"))
+    g64 = _run(pb64)
+    gmx = _run(pbmx)
+    for k in range(5):
+        err_h = rel_l2(gmx["pressure_head"][k], g64["pressure_head"][k])
+        err_t = rel_l2(gmx["theta"][k], g64["theta"][k])
+        print(f"  step {k}: rel L2 h {err_h:.2e} theta {err_t:.2e}")
+        assert err_h <= 1e-8 and err_t <= 1e-8
