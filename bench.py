@@ -1,0 +1,167 @@
+"""Benchmark of the implicit FEM time step on one MI355X.
+
+Workload (default): plain FP64 2D Richards at scale -- 512 x 512 structured
+triangular mesh of the unit square (263,169 nodes, 524,288 elements), van
+Genuchten alpha=0.1, n=2, m=0.5, theta_r=0.06, theta_s=0.40, per-element
+Ks = exp(N(ln 1e-3, 0.5^2)) spatially correlated (Gaussian kernel, length
+0.05, seed 0), h0 = -1000, top Dirichlet -75, fixed dt = 1e-4,
+Jacobi-BiCGSTAB (rtol 1e-10, maxiter 100), Picard relative 1e-6.
+
+A step is one implicit time step (Picard iterations to convergence, each an
+assembly + a Krylov solve).  value = steps/s over the timed steps, inputs
+resident in HBM.  The SpMV launches inside the timed region are bracketed by
+HIP events on the solver's stream; their mean duration gives the roofline
+line.  The CPU baseline runs the oracle (NumPy/SciPy FP64 restatement of the
+reference, one thread) on the first steps of the same run.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--workload c1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time steps/s + SpMV HBM GB/s (% peak): batched/mixed vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (description, cells, dim, solver, precision, ks_field, dt)
+    "c1": ("2D Richards 512x512 P1 triangles, heterogeneous log-normal Ks, fixed dt 1e-4, "
+           "Jacobi-BiCGSTAB, FP64", 512, 2, "bicgstab", "fp64", {"sigma": 0.5, "length": 0.05, "seed": 0}, 1e-4),
+    "c2": ("3D Richards 64^3 Kuhn tetrahedra, fixed dt 1e-4, Jacobi-CG, FP64", 64, 3, "cg", "fp64", None, 1e-4),
+    "c2-mixed": ("3D Richards 64^3 Kuhn tetrahedra, fixed dt 1e-4, Jacobi-CG, FP32 inner / FP64 outer",
+                 64, 3, "cg", "mixed", None, 1e-4),
+    "c1-mixed": ("2D Richards 512x512, heterogeneous Ks, Jacobi-BiCGSTAB, FP32 inner / FP64 outer",
+                 512, 2, "bicgstab", "mixed", {"sigma": 0.5, "length": 0.05, "seed": 0}, 1e-4),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--no-timing", action="store_true", help="do not bracket SpMV launches with events")
+    return ap.parse_args()
+
+
+def spmv_bytes(n, nnz, value_bytes=8):
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row pointers
+    + x gather (once per entry of x) + y store."""
+    return (value_bytes + 4) * nnz + 4 * (n + 1) + 2 * value_bytes * n
+
+
+def cpu_baseline(pb, steps):
+    """Oracle restatement of the reference path, timed on this host (1 thread)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+        limiter = None
+    from oracle import picard as opicard
+    opb = opicard.Problem(points=pb.points, elems=pb.elements, soil_model=pb.soil_model, phi=pb.phi,
+                          bc_lists=pb.bc_lists, h0=pb.h0, dt0=pb.dt0, nsteps=steps, adaptive=False,
+                          picard_tol=pb.picard_tol, picard_maxit=pb.picard_maxit, picard_mode=pb.picard_mode,
+                          solver=pb.solver, precond=pb.precond, tol=pb.tol, maxiter=pb.maxiter,
+                          restart=pb.restart, Ks_elem=pb.Ks_elem)
+    st = opicard.Stepper(opb)
+    import numpy as np
+    h = pb.h0.astype(np.float64).copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h, _, _, _ = st.step(h, pb.dt0)
+    el = time.perf_counter() - t0
+    if limiter is not None:
+        limiter.unregister() if hasattr(limiter, "unregister") else None
+    return steps / el, el, h
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": f"this build runs on one GPU; --gpus {args.gpus} is not supported "
+                                   "(the ensemble/mesh sharding design is described in DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    from jax_wspm_amd.richards import RichardsSolver
+
+    desc, cells, dim, solver, precision, ksf, dt = WORKLOADS[args.workload]
+    cfg = baseline_vg_config(cells, dim, solver=solver, precond="jacobi", precision=precision, ks_field=ksf,
+                             nsteps=args.warmup + args.steps, dt=dt)
+    pb = from_config(cfg)
+    rs = RichardsSolver(pb, history="last")
+    ctx = rs.ctx
+    rs.init_state()
+    print(f"[bench] {args.workload}: n={ctx.n} E={ctx.n_elems} nnz={ctx.nnz} lanes={ctx.lanes}", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        ctx.richards_step(dt)
+    torch.cuda.synchronize()
+    ctx.timing(not args.no_timing)
+    picard = kry = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = ctx.richards_step(dt)
+        picard += st["picard_iters"]
+        kry += st["krylov_iters"]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    launches, spmv_ms = ctx.timing_read()
+    ctx.timing(False)
+    h, th, _ = ctx.richards_state()
+    assert bool(torch.isfinite(h).all()), "non-finite pressure head"
+    value = args.steps / el
+    vbytes = 4 if precision == "mixed" else 8
+    # in mixed mode most SpMVs are FP32 (inner) -- the roofline line uses the
+    # FP64 byte count, which overstates nothing for FP64 and understates FP32
+    per_launch = spmv_bytes(ctx.n, ctx.nnz, 8)
+    avg_ms = spmv_ms / launches if launches else float("nan")
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f32+f64",
+        "data": "synthetic (structured mesh + seeded correlated Ks field generated in-process)",
+        "config": {"workload": desc, "nodes": ctx.n, "elements": ctx.n_elems, "nnz": ctx.nnz, "dt": dt,
+                   "solver": f"{solver}+jacobi", "precision": precision,
+                   "picard_iters_per_step": round(picard / args.steps, 3),
+                   "krylov_iters_per_step": round(kry / args.steps, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": "k_spmv (CSR SpMV, fused Krylov epilogues)",
+                     "bytes_per_launch": per_launch, "launches": launches,
+                     "avg_launch_us": round(avg_ms * 1e3, 3) if launches else None},
+    }
+    if args.cpu_steps > 0:
+        pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,
+                                                nsteps=args.cpu_steps, dt=dt))
+        cps, cel, _ = cpu_baseline(pb_cpu, args.cpu_steps)
+        out["cpu_baseline"] = {"value": round(cps, 4), "unit": "time steps/s", "cores": 1, "kind": "port",
+                               "sample": f"first {args.cpu_steps} steps of the same run ({cel:.1f} s), "
+                                         f"NumPy/SciPy FP64 oracle, 1 thread; host has {os.cpu_count()} "
+                                         "logical CPUs"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -68,7 +68,9 @@ def test_pattern_soil_assembly(name, mk, model, phi, use_ks):
     gC, gK, gth = ctx.soil(model, phi_nodal, torch.as_tensor(h_m, device="cuda"))
     for g, o in ((gC, Cm), (gK, Km), (gth, thm)):
         g = g.cpu().numpy()
-        assert np.allclose(g, o, rtol=1e-13, atol=1e-300), np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-300))
+        # device pow/exp vs NumPy differ by an ulp; K = Ks Se^0.5 (1-(1-Se^(1/m))^m)^2
+        # amplifies that through cancellation, hence 1e-12 rather than 1e-15
+        assert np.allclose(g, o, rtol=1e-12, atol=1e-300), np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-300))
 
     # Dirichlet: two overlapping ordered lists (later wins)
     top = np.nonzero(pts[:, -1] == pts[:, -1].max())[0]
@@ -107,17 +109,36 @@ def _system(cells=12, dim=2, seed=0, dt=1e-4):
     return pts, el, pat, vals, b, h_m, bcl
 
 
+def _gardner_system(cells, dim, dt, seed=0):
+    """Gardner soil on the 15.24 domain, top/bottom Dirichlet: all six
+    solver/preconditioner pairs converge here in 1-80 iterations (the
+    unpreconditioned solvers stall on the unit-domain van Genuchten rows)."""
+    pts, el = (omesh.structured_tri if dim == 2 else omesh.structured_tet)(cells, 15.24)
+    n = len(pts)
+    pat = oasm.Pattern(el, n)
+    rng = np.random.default_rng(seed)
+    h_m = rng.uniform(-15.24, -1.0, n)
+    Cm, Km, thm = osoil.exponential(h_m, GARDNER)
+    _, _, thn = osoil.exponential(h_m - 0.5, GARDNER)
+    top = np.nonzero(pts[:, -1] == 15.24)[0]
+    bot = np.nonzero(pts[:, -1] == 0.0)[0]
+    bcl = [(top, np.full(top.size, -5.0)), (bot, np.full(bot.size, -15.0))]
+    vals, b = oasm.assemble(pat, pts, thm, thn, h_m, Km, Cm, dt, bcl)
+    return pts, el, pat, vals, b, h_m, bcl
+
+
 @pytest.mark.parametrize("method", ["cg", "bicgstab", "gmres"])
 @pytest.mark.parametrize("precond", ["none", "jacobi"])
 @pytest.mark.parametrize("dim", [2, 3])
 def test_krylov_matches_oracle(method, precond, dim):
     import torch
     print(f"[case] krylov {method} {precond} dim={dim}", flush=True)
-    # a stiffer system (large dt) so the solvers take several iterations
-    pts, el, pat, vals, b, h_m, _ = _system(cells=14 if dim == 2 else 5, dim=dim, dt=50.0)
+    # dt large enough that the solvers take several (up to ~80) iterations
+    pts, el, pat, vals, b, h_m, _ = _gardner_system(14 if dim == 2 else 5, dim, 0.1 if dim == 2 else 1.0)
     A = pat.to_scipy(vals)
     dinv = oasm.jacobi_inverse(pat, vals) if precond == "jacobi" else None
     x_o, info = okr.solve(method, A.dot, b, h_m, dinv, 1e-10, 0.0, 100, 30)
+    assert np.linalg.norm(b - A @ x_o) <= 1e-9 * np.linalg.norm(b)   # the oracle converged
     ctx = _ctx(pts, el)
     dv = torch.as_tensor(vals, device="cuda")
     x_g, st = ctx.krylov(method, dv, torch.as_tensor(b, device="cuda"), torch.as_tensor(h_m, device="cuda"),
@@ -132,7 +153,7 @@ def test_krylov_matches_oracle(method, precond, dim):
 def test_krylov_small_restart_and_truncation():
     """GMRES with restart 5 needs several cycles; CG truncated at maxiter=3."""
     import torch
-    pts, el, pat, vals, b, h_m, _ = _system(cells=10, dim=2, dt=500.0)
+    pts, el, pat, vals, b, h_m, _ = _gardner_system(10, 2, 10.0)
     A = pat.to_scipy(vals)
     dinv = oasm.jacobi_inverse(pat, vals)
     ctx = _ctx(pts, el)
@@ -152,7 +173,7 @@ def test_krylov_small_restart_and_truncation():
 def test_mixed_precision_refinement(method):
     import torch
     print(f"[case] mixed {method}", flush=True)
-    pts, el, pat, vals, b, h_m, _ = _system(cells=16, dim=2, dt=5.0)
+    pts, el, pat, vals, b, h_m, _ = _gardner_system(16, 2, 1.0)
     A = pat.to_scipy(vals)
     dinv = oasm.jacobi_inverse(pat, vals)
     x_ref = np.linalg.solve(A.toarray(), b)
