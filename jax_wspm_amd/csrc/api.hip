@@ -155,32 +155,18 @@ __global__ __launch_bounds__(BLOCK) void k_gm_dots(KVec<T> v, int j, Red red) {
         for (int u = 0; u < 8; ++u)
             if (c0 + u <= j) acc[u] += (double)v.V[(long)(c0 + u) * v.ldv + i] * wi;
     }
-    __shared__ double sh[8 * NWAVE];
-    __shared__ int is_last;
-    block_sum<8>(acc, sh);
-    const unsigned nblk = gridDim.x * gridDim.y;
-    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) red.partials[(size_t)bid * MAX_DOTS + u] = acc[u];
-        __threadfence();
-        const unsigned t = atomicAdd(red.counter, 1u);
-        is_last = (t == nblk - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    const volatile double* P = red.partials;
+    store_partials<8>(acc, red.partials);
+}
+
+// h[l] = sum over the nb row-blocks of column l's partials (l <= j), in order.
+__global__ __launch_bounds__(BLOCK) void k_gm_dots_finish(KState* s, const double* __restrict__ partials, int j,
+                                                           int nb) {
+    if (s->done || !s->active) return;
     for (int l = threadIdx.x; l <= j; l += BLOCK) {
         const int by = l / 8, u = l % 8;
         double sum = 0.0;
-        for (int bx = 0; bx < nb; ++bx) sum += P[(size_t)(by * nb + bx) * MAX_DOTS + u];
+        for (int bx = 0; bx < nb; ++bx) sum += partials[(size_t)(by * nb + bx) * MAX_DOTS + u];
         s->hbuf[l] = sum;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *red.counter = 0u;
-        __threadfence();
     }
 }
 
@@ -290,7 +276,6 @@ struct wspm_ctx {
     QuadConst qc;
     // reductions
     double* partials = nullptr;
-    unsigned* counter = nullptr;
     KState* states = nullptr;     // [3]: main, inner, refinement
     KState* hstate = nullptr;     // pinned mirror
     double* H = nullptr;          // GMRES Hessenberg [MAX_RESTART][MAX_RESTART+1]
@@ -345,7 +330,16 @@ F* state_field(KState* ds, size_t off) {
 }
 
 Red red_of(wspm_ctx* ctx, int which) {
-    return Red{ctx->partials, ctx->counter, ctx->states + which};
+    return Red{ctx->partials, ctx->states + which};
+}
+
+template <class Op>
+void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
+    if constexpr (Op::ND > 0) {
+        hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials, nblk,
+                           ctx->states + which);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 template <class Op>
@@ -353,6 +347,7 @@ void launch_vec(wspm_ctx* ctx, const Op& op, int which) {
     hipLaunchKernelGGL(k_vec<Op>, dim3(ctx->vec_grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op,
                        red_of(ctx, which));
     HIPCHK(hipGetLastError());
+    launch_finish(ctx, op, ctx->vec_grid, which);
 }
 
 hipEvent_t next_event(wspm_ctx* ctx) {
@@ -380,6 +375,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
     }
     HIPCHK(hipGetLastError());
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
+    launch_finish(ctx, epi, ctx->spmv_grid, which);
 }
 
 void poll(wspm_ctx* ctx, int which) {
@@ -491,6 +487,8 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
             launch_spmv<T>(ctx, (const T*)A, GmArnoldi<T>{v, j}, which, true);
             hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
                                red_of(ctx, which));
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials, j, nbx);
             HIPCHK(hipGetLastError());
             GmUpdate<T> up{v, j, H};
             up.hb = state_field<double>(ds, offsetof(KState, hbuf));
@@ -608,8 +606,6 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         HIPCHK(hipMemset(ctx->bc_flag, 0, n_nodes));
         HIPCHK(hipMemset(ctx->bc_val, 0, n_nodes * sizeof(double)));
         ctx->partials = dalloc<double>(ctx, (size_t)MAX_BLOCKS * MAX_DOTS);
-        ctx->counter = dalloc<unsigned>(ctx, 1);
-        HIPCHK(hipMemset(ctx->counter, 0, sizeof(unsigned)));
         ctx->states = dalloc<KState>(ctx, 3);
         HIPCHK(hipMemset(ctx->states, 0, 3 * sizeof(KState)));
         void* hp = nullptr;
@@ -814,7 +810,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
     {
         ToF32 op{A_vals, ctx->A32};
         const int g = (int)std::min<int64_t>((ctx->nnz + BLOCK - 1) / BLOCK, 4096);
-        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2));
+        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2));  // ND = 0
         HIPCHK(hipGetLastError());
         if (dinv) launch_vec(ctx, ToF32{dinv, ctx->dinv32}, 2);
     }

@@ -26,6 +26,8 @@ struct KVec {
     const T* dinv;  // Jacobi inverse diagonal or nullptr
 };
 
+// Kernels with ND > 0 leave one partial row per workgroup; the host follows
+// them with k_finish<ND>, which runs the functor's fin() on the totals.
 // ---------------------------------------------------------------------------
 // CSR SpMV: LANES consecutive lanes own one row (LANES | 64), the row sum is
 // finished with a width-LANES butterfly, then lane 0 runs the epilogue.
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
         double v[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        grid_sum<ND>(v, red, [&](const double* t) { epi.fin(t, red.st); });
+        store_partials<ND>(v, red.partials);
     }
 }
 
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red) {
         double v[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        grid_sum<ND>(v, red, [&](const double* t) { op.fin(t, red.st); });
+        store_partials<ND>(v, red.partials);
     }
 }
 

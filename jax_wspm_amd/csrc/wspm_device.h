@@ -35,10 +35,14 @@ struct KState {
 
 // ---------------------------------------------------------------------------
 // Deterministic reductions.  Wavefront butterfly (fixed xor order), then the
-// NWAVE wave sums added in wave order by thread 0.  The grid-level reduction
-// writes one partial row per workgroup; the last workgroup to arrive (integer
-// ticket) adds the partials in workgroup order and runs the finishing step.
-// The result depends only on the launch shape, never on scheduling.
+// NWAVE wave sums added in wave order by thread 0.  Grid level: every
+// workgroup stores its partial row with plain stores; a one-workgroup finish
+// kernel, launched right after on the same stream, adds the rows in workgroup
+// order and updates the solver scalars.  The kernel boundary is the only
+// device-wide synchronisation: no atomics, no in-kernel fences (an agent-scope
+// release on gfx950 writes back the XCD's L2, which for a streaming kernel is
+// most of its output, once per workgroup).  Results depend only on the launch
+// shape, never on scheduling.
 // ---------------------------------------------------------------------------
 template <int ND>
 __device__ inline void block_sum(double (&v)[ND], double* sh) {
@@ -68,44 +72,36 @@ __device__ inline void block_sum(double (&v)[ND], double* sh) {
 
 struct Red {
     double* partials;     // [MAX_BLOCKS][MAX_DOTS]
-    unsigned* counter;    // ticket, 0 between kernels
     KState* st;
 };
 
-// Sum ND per-thread values over the whole grid; the finishing workgroup's
-// thread 0 calls fin(tot) with the grid totals.  Every workgroup of the grid
-// must call this exactly once.
-template <int ND, class Fin>
-__device__ inline void grid_sum(double (&v)[ND], const Red& red, Fin&& fin) {
-    __shared__ double sh[(ND > 0 ? ND : 1) * NWAVE];
-    __shared__ int is_last;
+// Store this workgroup's ND block sums as row blockIdx of the partials.
+template <int ND>
+__device__ inline void store_partials(double (&v)[ND], double* partials) {
+    __shared__ double sh[ND * NWAVE];
     block_sum<ND>(v, sh);
-    const unsigned nblk = gridDim.x * gridDim.y;
-    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0) {
+        const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
-        for (int j = 0; j < ND; ++j) red.partials[(size_t)bid * MAX_DOTS + j] = v[j];
-        __threadfence();
-        const unsigned t = atomicAdd(red.counter, 1u);
-        is_last = (t == nblk - 1);
+        for (int j = 0; j < ND; ++j) partials[(size_t)bid * MAX_DOTS + j] = v[j];
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// Sum nblk partial rows in order and run op.fin(totals, state).  One workgroup.
+template <int ND, class Op>
+__global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restrict__ partials, int nblk,
+                                                   KState* st) {
+    if (op.skip(st)) return;
     double acc[ND];
 #pragma unroll
     for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-    const volatile double* P = red.partials;
-    for (unsigned b = threadIdx.x; b < nblk; b += BLOCK) {
+    for (int b = threadIdx.x; b < nblk; b += BLOCK) {
 #pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += P[(size_t)b * MAX_DOTS + j];
+        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)b * MAX_DOTS + j];
     }
+    __shared__ double sh[ND * NWAVE];
     block_sum<ND>(acc, sh);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        *red.counter = 0u;
-        __threadfence();
-    }
+    if (threadIdx.x == 0) op.fin(acc, st);
 }
 
 // ---------------------------------------------------------------------------
