@@ -155,17 +155,17 @@ __global__ __launch_bounds__(BLOCK) void k_gm_dots(KVec<T> v, int j, Red red) {
         for (int u = 0; u < 8; ++u)
             if (c0 + u <= j) acc[u] += (double)v.V[(long)(c0 + u) * v.ldv + i] * wi;
     }
-    store_partials<8>(acc, red.partials);
+    store_partials<8>(acc, red.partials, red.prow);
 }
 
 // h[l] = sum over the nb row-blocks of column l's partials (l <= j), in order.
-__global__ __launch_bounds__(BLOCK) void k_gm_dots_finish(KState* s, const double* __restrict__ partials, int j,
-                                                           int nb) {
+__global__ __launch_bounds__(BLOCK) void k_gm_dots_finish(KState* s, const double* __restrict__ partials, int prow,
+                                                           int j, int nb) {
     if (s->done || !s->active) return;
     for (int l = threadIdx.x; l <= j; l += BLOCK) {
         const int by = l / 8, u = l % 8;
         double sum = 0.0;
-        for (int bx = 0; bx < nb; ++bx) sum += partials[(size_t)(by * nb + bx) * MAX_DOTS + u];
+        for (int bx = 0; bx < nb; ++bx) sum += partials[(size_t)u * prow + by * nb + bx];
         s->hbuf[l] = sum;
     }
 }
@@ -261,7 +261,10 @@ struct wspm_ctx {
     hipStream_t stream = nullptr;
     int dim = 0, n = 0, E = 0;
     int64_t nnz = 0;
-    int max_row = 0, max_block_nnz = 0;
+    int max_row = 0;
+    int asm_win = 0, asm_inc = 0, asm_grid = 0;   // assembly LDS window / incidence capacity
+    size_t asm_lds = 0;
+    int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
     int spmv_grid = 1, vec_grid = 1;
     HostPattern hp;
@@ -330,14 +333,14 @@ F* state_field(KState* ds, size_t off) {
 }
 
 Red red_of(wspm_ctx* ctx, int which) {
-    return Red{ctx->partials, ctx->states + which};
+    return Red{ctx->partials, ctx->prow, ctx->states + which};
 }
 
 template <class Op>
 void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
     if constexpr (Op::ND > 0) {
-        hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials, nblk,
-                           ctx->states + which);
+        hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials,
+                           ctx->prow, nblk, ctx->states + which);
         HIPCHK(hipGetLastError());
     }
 }
@@ -426,15 +429,19 @@ KVec<T> make_kvec(wspm_ctx* ctx, KWork<T>& w, const T* dinv, const T* b, const T
 
 // One Krylov solve.  Kernels early-exit once the device marks the solve done,
 // so iterations are enqueued in chunks and the host reads back once per chunk.
+// started: the first step (r0 = b - A x0, its dots and the solver scalars) was
+// already done by the assembly epilogue (k_assemble with a Krylov start mode).
 template <class T>
 void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T* b, const T* x0, T* x,
-                  double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st) {
+                  double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st,
+                  bool started = false) {
     const int n = ctx->n;
     KState& hs = ctx->hstate[which];
     if (method == WSPM_BICGSTAB || method == WSPM_CG) {
         KWork<T>& w = ensure_work<T>(ctx, 0);
         KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
-        if (method == WSPM_BICGSTAB) {
+        if (started) {
+        } else if (method == WSPM_BICGSTAB) {
             BicgInit<T> e{v, tol, atol, maxiter};
             launch_spmv<T>(ctx, A, e, which, true);
         } else {
@@ -476,7 +483,7 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
     KWork<T>& w = ensure_work<T>(ctx, R + 1);
     KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
     double* H = (which == 1) ? ctx->H32 : ctx->H;
-    launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
+    if (!started) launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
     launch_vec(ctx, GmScale<T>{v, 0.0}, which);
     KState* ds = ctx->states + which;
     const int nbx = std::min((n + BLOCK - 1) / BLOCK, 256);
@@ -488,7 +495,8 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
             hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
                                red_of(ctx, which));
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials, j, nbx);
+            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials, ctx->prow,
+                               j, nbx);
             HIPCHK(hipGetLastError());
             GmUpdate<T> up{v, j, H};
             up.hb = state_field<double>(ds, offsetof(KState, hbuf));
@@ -574,9 +582,15 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         const HostPattern& P = ctx->hp;
         ctx->nnz = (int64_t)P.cols.size();
         ctx->max_row = P.max_row;
-        ctx->max_block_nnz = P.max_block_nnz;
-        if ((size_t)P.max_block_nnz * sizeof(double) > 64 * 1024) {
-            g_create_error = "assembly window exceeds 64 KiB of LDS (rows too long)";
+        {
+            const int R = asm_rows_per_block(dim);
+            block_maxima(P, R, ctx->asm_win, ctx->asm_inc);
+            ctx->asm_lds = asm_lds_bytes(dim, ctx->asm_win, ctx->asm_inc);
+            ctx->asm_grid = (n_nodes + R - 1) / R;
+            ctx->prow = std::max(MAX_BLOCKS, ctx->asm_grid);
+        }
+        if (ctx->asm_lds > 64 * 1024) {
+            g_create_error = "assembly LDS staging exceeds 64 KiB (rows too long or too many incident elements)";
             delete ctx;
             return WSPM_EINVAL;
         }
@@ -605,7 +619,7 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         ctx->bc_val = dalloc<double>(ctx, n_nodes);
         HIPCHK(hipMemset(ctx->bc_flag, 0, n_nodes));
         HIPCHK(hipMemset(ctx->bc_val, 0, n_nodes * sizeof(double)));
-        ctx->partials = dalloc<double>(ctx, (size_t)MAX_BLOCKS * MAX_DOTS);
+        ctx->partials = dalloc<double>(ctx, (size_t)ctx->prow * MAX_DOTS);
         ctx->states = dalloc<KState>(ctx, 3);
         HIPCHK(hipMemset(ctx->states, 0, 3 * sizeof(KState)));
         void* hp = nullptr;
@@ -637,7 +651,7 @@ int32_t wspm_set_stream(wspm_ctx* ctx, void* stream) {
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info) {
     if (!ctx || !info) return WSPM_EINVAL;
     info[0] = ctx->n; info[1] = ctx->E; info[2] = ctx->nnz; info[3] = ctx->dim;
-    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->max_block_nnz;
+    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = (int64_t)ctx->asm_lds;
     return WSPM_OK;
 }
 
@@ -689,12 +703,18 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
     API_END
 }
 
+// Assembly (+ Dirichlet rows + Jacobi diagonal); with mode != ASM_INIT_NONE
+// the epilogue also starts the Krylov solve (BiCGSTAB / CG / GMRES state for
+// x0 -> x, or the FP64 refinement residual) and its finish kernel runs here.
 static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
                         const double* K, const double* theta_m, const double* theta_n, double dt,
-                        double* A, double* b, double* dinv, float* A32, float* dinv32) {
+                        double* A, double* b, double* dinv, float* A32, float* dinv32,
+                        int mode = ASM_INIT_NONE, double* x = nullptr, double tol = 0.0, double atol = 0.0,
+                        int maxiter = 0, int restart = 0) {
     AsmArgs a{};
     a.n = ctx->n;
-    a.row_ptr = ctx->row_ptr; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
+    a.win_cap = ctx->asm_win;
+    a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
     a.inc_code = ctx->inc_code; a.inc_slot = ctx->inc_slot; a.conn = ctx->conn;
     a.X = ctx->X; a.Y = ctx->Y; a.Z = ctx->Z;
     a.Kn = K; a.Ks_elem = Ks_elem; a.C = C; a.theta_m = theta_m; a.theta_n = theta_n; a.h_m = h_m;
@@ -703,7 +723,43 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.qc = ctx->qc;
-    HIPCHK(launch_assemble(ctx->dim, a, ctx->max_block_nnz, ctx->stream));
+    KrylovInit ki{};
+    if (mode != ASM_INIT_NONE) {
+        KWork<double>& w = ensure_work<double>(ctx, 0);
+        ki.x0 = h_m;
+        ki.x = x;
+        ki.r = mode == ASM_INIT_RESID ? ctx->ref_r : w.r;
+        ki.rh = w.rh; ki.p = w.p; ki.z = w.z; ki.w = w.w;
+        ki.jacobi = dinv != nullptr;
+    }
+    int grid = 0;
+    HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream));
+    if (mode == ASM_INIT_NONE) return;
+    KWork<double>& w = ensure_work<double>(ctx, 0);
+    KVec<double> v = make_kvec<double>(ctx, w, dinv, b, h_m, x);
+    const dim3 g1(1), b1(BLOCK);
+    KState* st0 = ctx->states;
+    switch (mode) {
+        case ASM_INIT_BICG:
+            hipLaunchKernelGGL((k_finish<3, BicgInit<double>>), g1, b1, 0, ctx->stream,
+                               BicgInit<double>{v, tol, atol, maxiter}, ctx->partials, ctx->prow, grid, st0);
+            break;
+        case ASM_INIT_CG:
+            hipLaunchKernelGGL((k_finish<3, CgInit<double>>), g1, b1, 0, ctx->stream,
+                               CgInit<double>{v, tol, atol, maxiter}, ctx->partials, ctx->prow, grid, st0);
+            break;
+        case ASM_INIT_GMRES: {
+            const int R = std::min(std::min(restart, ctx->n), MAX_RESTART);
+            hipLaunchKernelGGL((k_finish<3, GmInit<double>>), g1, b1, 0, ctx->stream,
+                               GmInit<double>{v, tol, atol, maxiter, R}, ctx->partials, ctx->prow, grid, st0);
+            break;
+        }
+        default:
+            hipLaunchKernelGGL((k_finish<3, RefResid>), g1, b1, 0, ctx->stream, RefResid{b, x, ctx->ref_r},
+                               ctx->partials, ctx->prow, grid, ctx->states + 2);
+            break;
+    }
+    HIPCHK(hipGetLastError());
 }
 
 int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
@@ -759,18 +815,26 @@ int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const d
     API_END
 }
 
-static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double* dinv, const float* A32,
-                        const float* dinv32, const double* b, double* x, double tol, double atol,
-                        int maxiter, int restart, double inner_tol, int max_refine, wspm_krylov_stats* st) {
+static void ensure_mixed(wspm_ctx* ctx) {
     if (!ctx->ref_r) {
         ctx->ref_r = dalloc<double>(ctx, ctx->n);
         ctx->ref_r32 = dalloc<float>(ctx, ctx->n);
         ctx->ref_d32 = dalloc<float>(ctx, ctx->n);
     }
+}
+
+// started: the first residual r = b - A x and its norms are already in
+// ctx->ref_r / state 2 (assembly epilogue).
+static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double* dinv, const float* A32,
+                        const float* dinv32, const double* b, double* x, double tol, double atol,
+                        int maxiter, int restart, double inner_tol, int max_refine, wspm_krylov_stats* st,
+                        bool started = false) {
+    ensure_mixed(ctx);
     int inner_total = 0, arn = 0, sweeps = 0;
     KState& hs = ctx->hstate[2];
     while (true) {
-        launch_spmv<double>(ctx, A, RefResid{b, x, ctx->ref_r}, 2, true);
+        if (!started) launch_spmv<double>(ctx, A, RefResid{b, x, ctx->ref_r}, 2, true);
+        started = false;
         poll(ctx, 2);
         const double thresh = std::max(tol * hs.bnorm, atol);
         if (!(hs.rnorm > thresh) || sweeps >= max_refine) break;
@@ -883,16 +947,23 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     int it = 0, kry = 0;
     double err = INFINITY;
     while (err >= P.picard_tol && it < P.picard_maxit) {
+        const int mode = mixed ? ASM_INIT_RESID
+                         : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG
+                         : P.method == WSPM_CG ? ASM_INIT_CG : ASM_INIT_GMRES;
+        if (mixed) {
+            ensure_mixed(ctx);
+            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        }
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
-                    jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr);
+                    jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
+                    mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
         wspm_krylov_stats st{};
         if (mixed) {
-            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
-                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st);
+                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, true);
         } else {
             krylov_solve<double>(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
-                                 P.tol, P.atol, P.maxiter, P.restart, 0, &st);
+                                 P.tol, P.atol, P.maxiter, P.restart, 0, &st, true);
         }
         kry += st.iters;
         launch_vec(ctx, PicardOp{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm,

@@ -1,11 +1,22 @@
-// Fused Richards assembly: argument block and launcher (assemble.hip).
+// Fused Richards assembly: argument blocks and launcher (assemble.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace wspm {
+
+// Rows per assembly workgroup.  2D rows have ~6 incident triangles and ~7
+// entries, 3D rows ~24 incident tets and ~15 entries; R sets the LDS staging
+// (incidences x (dim+2) doubles + the rows' CSR window) and the write-out
+// lanes per row (256 / R).
+constexpr int ASM_ROWS_2D = 64;
+constexpr int ASM_ROWS_3D = 32;
+
+// What the assembly epilogue also computes (the first step of the solver).
+enum { ASM_INIT_NONE = 0, ASM_INIT_BICG = 1, ASM_INIT_CG = 2, ASM_INIT_GMRES = 3, ASM_INIT_RESID = 4 };
 
 // Quadrature-derived constants, computed on the host from the reference's
 // literal points/weights (src/numerics/gauss.py:31-37, gauss_3D.py:29-39):
@@ -18,7 +29,9 @@ struct QuadConst {
 
 struct AsmArgs {
     int n;
+    int win_cap;             // LDS doubles reserved for the CSR window
     const int32_t* row_ptr;
+    const int32_t* cols;
     const int32_t* diag_pos;
     const int32_t* inc_ptr;
     const int32_t* inc_code;
@@ -44,7 +57,20 @@ struct AsmArgs {
     QuadConst qc;
 };
 
+// Vectors written by the fused Krylov start (FP64).
+struct KrylovInit {
+    const double* x0;
+    double* x;
+    double *r, *rh, *p, *z, *w;
+    int jacobi;
+};
+
 void compute_quad_const(int dim, QuadConst& qc);
-hipError_t launch_assemble(int dim, const AsmArgs& a, int max_block_nnz, hipStream_t s);
+int asm_rows_per_block(int dim);
+size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc);
+// Launch the assembly; grid_out receives the number of workgroups (partial
+// rows written when mode != ASM_INIT_NONE).
+hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
+                           int prow, size_t lds, int* grid_out, hipStream_t s);
 
 }  // namespace wspm

@@ -8,14 +8,22 @@
 //   A_e   = M_L diag(C)/dt + stiff,  b_e = src - M_L (theta_m - theta_n - C h)/dt,
 // then scatters 9E (16E) COO entries and sums duplicates.
 //
-// Here: one thread owns one row (node) i.  It walks the node's incident
-// elements in a fixed order (precomputed incidence list), recomputes the
-// element geometry in closed form (J is constant on a P1 element, so the
-// per-point inverses collapse to one), and adds row i of each element matrix
-// into an LDS window that covers the CSR values of the workgroup's 256 rows.
-// Dirichlet rows become identity rows in the same pass, the Jacobi inverse
-// diagonal is taken from the finished row, and the window is written back to
-// HBM with coalesced stores.  No atomics, no COO staging, fixed summation order.
+// Here a workgroup owns R consecutive rows (nodes).  Their incident
+// (element, local vertex) pairs are contiguous in the precomputed incidence
+// list, so:
+//   1. every thread takes incidences of the workgroup (many independent
+//      gathers in flight), recomputes the element geometry in closed form (J is
+//      constant on a P1 element, one inverse instead of one per quadrature
+//      point) and writes that element's row contribution (dim+1 matrix values
+//      + 1 source value) to LDS;
+//   2. one thread per row adds its incidences' contributions, in the fixed
+//      incidence order, into an LDS window over the rows' CSR values (identity
+//      row and value for Dirichlet rows), and the Jacobi inverse diagonal;
+//   3. LANES = 256/R threads per row stream the finished window to HBM with
+//      coalesced stores and, optionally, fuse the first step of the Krylov
+//      solve: r0 = b - A x0 and its dot products (the SpMV the solver would
+//      otherwise start with, on values that are still in LDS).
+// No atomics, no COO staging, fixed summation order: bitwise repeatable.
 #include "assemble.h"
 #include "wspm_device.h"
 #include "wspm_internal.h"
@@ -23,118 +31,197 @@
 namespace wspm {
 
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
+__device__ inline void element_row(const AsmArgs& a, int q, double (&out)[D + 2]) {
     constexpr int K = D + 1;
-    extern __shared__ double win[];
-    const int r0 = blockIdx.x * BLOCK;
-    const int r1 = min(a.n, r0 + BLOCK);
+    const int code = a.inc_code[q];
+    const int e = code >> 2;
+    const int la = code & 3;
+    int nd[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) nd[c] = a.conn[(size_t)e * K + c];
+    double G[D][K];
+    double det;
+    if constexpr (D == 2) {
+        const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]];
+        const double J00 = a.X[nd[1]] - x0, J01 = a.Y[nd[1]] - y0;
+        const double J10 = a.X[nd[2]] - x0, J11 = a.Y[nd[2]] - y0;
+        det = J00 * J11 - J01 * J10;
+        const double id = 1.0 / det;
+        G[0][1] = J11 * id;  G[1][1] = -J10 * id;
+        G[0][2] = -J01 * id; G[1][2] = J00 * id;
+    } else {
+        const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]], z0 = a.Z[nd[0]];
+        double r[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r[c][0] = a.X[nd[c + 1]] - x0;
+            r[c][1] = a.Y[nd[c + 1]] - y0;
+            r[c][2] = a.Z[nd[c + 1]] - z0;
+        }
+        // J rows r1, r2, r3; J^{-1} columns = (r2 x r3, r3 x r1, r1 x r2) / det
+        double cx[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double* u = r[(c + 1) % 3];
+            const double* v = r[(c + 2) % 3];
+            cx[c][0] = u[1] * v[2] - u[2] * v[1];
+            cx[c][1] = u[2] * v[0] - u[0] * v[2];
+            cx[c][2] = u[0] * v[1] - u[1] * v[0];
+        }
+        det = r[0][0] * cx[0][0] + r[0][1] * cx[0][1] + r[0][2] * cx[0][2];
+        const double id = 1.0 / det;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) G[d][c + 1] = cx[c][d] * id;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 1; c < K; ++c) s += G[d][c];
+        G[d][0] = -s;
+    }
+    double kbar = 0.0;
+#pragma unroll
+    for (int c = 0; c < K; ++c) kbar += a.qc.ck[c] * a.Kn[nd[c]];
+    if (a.Ks_elem != nullptr) kbar *= a.Ks_elem[e];
+    const double kd = kbar * det;
+    double ga[D];
+    double ml = 0.0;
+    int node = 0;
+#pragma unroll
+    for (int c = 0; c < K; ++c)
+        if (c == la) {
+            ml = a.qc.ml[c];
+            node = nd[c];
+#pragma unroll
+            for (int d = 0; d < D; ++d) ga[d] = G[d][c];
+        }
+    const double Ci = a.C[node];
+    const double rhs_i = a.theta_m[node] - a.theta_n[node] - Ci * a.h_m[node];
+    const double mlump = det * ml;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        double g = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) g += ga[d] * G[d][c];
+        double v = kd * g;
+        if (c == la) v += a.inv_dt * mlump * Ci;
+        out[c] = v;
+    }
+    out[K] = -kd * ga[D - 1] - a.inv_dt * mlump * rhs_i;
+}
+
+// Krylov start fused into the assembly epilogue (one row, lane 0 of its group):
+// r = b - A x0 and the method's first vectors and dot products.
+template <int MODE>
+__device__ inline void krylov_start(const AsmArgs& a, const KrylovInit& ki, int i, double bi, double ax,
+                                    double dinv, double (&acc)[3]) {
+    const double r = bi - ax;
+    if constexpr (MODE == ASM_INIT_BICG) {
+        ki.x[i] = ki.x0[i];
+        ki.r[i] = r;
+        ki.rh[i] = r;
+        acc[0] += r * r;
+        acc[1] += bi * bi;
+    } else if constexpr (MODE == ASM_INIT_CG) {
+        ki.x[i] = ki.x0[i];
+        ki.r[i] = r;
+        const double z = ki.jacobi ? dinv * r : r;
+        if (ki.jacobi) ki.z[i] = z;
+        ki.p[i] = z;
+        acc[0] += r * z;
+        acc[1] += r * r;
+        acc[2] += bi * bi;
+    } else if constexpr (MODE == ASM_INIT_GMRES) {
+        ki.x[i] = ki.x0[i];
+        const double w = ki.jacobi ? dinv * r : r;
+        ki.w[i] = w;
+        const double mb = ki.jacobi ? dinv * bi : bi;
+        acc[0] += bi * bi;
+        acc[1] += mb * mb;
+        acc[2] += w * w;
+    } else if constexpr (MODE == ASM_INIT_RESID) {
+        ki.r[i] = r;
+        acc[0] += r * r;
+        acc[1] += bi * bi;
+    }
+}
+
+template <int D, int R, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+    constexpr int K = D + 1;
+    constexpr int NC = K + 1;                 // values per incidence in LDS
+    constexpr int LANES = BLOCK / R;          // threads per row in the write-out
+    extern __shared__ double lds[];
+    __shared__ double bsh[R], dsh[R];
+    const int r0 = blockIdx.x * R;
+    const int r1 = min(a.n, r0 + R);
     const int base = a.row_ptr[r0];
     const int wlen = a.row_ptr[r1] - base;
+    const int q0 = a.inc_ptr[r0];
+    const int nq = a.inc_ptr[r1] - q0;
+    double* win = lds;                        // [max_block_nnz]
+    double* con = lds + a.win_cap;            // [max_block_inc][NC]
     for (int k = threadIdx.x; k < wlen; k += BLOCK) win[k] = 0.0;
+
+    // 1. element row contributions of every incidence of the workgroup
+    for (int t = threadIdx.x; t < nq; t += BLOCK) {
+        double out[D + 2];
+        element_row<D>(a, q0 + t, out);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
+    }
     __syncthreads();
 
-    const int i = r0 + threadIdx.x;
-    if (i < r1) {
+    // 2. one thread per row: fixed-order sums into the CSR window
+    if (threadIdx.x < r1 - r0) {
+        const int i = r0 + threadIdx.x;
         const int off = a.row_ptr[i] - base;
         const bool is_bc = a.bc_flag != nullptr && a.bc_flag[i];
         double bi = 0.0;
         if (!is_bc) {
-            const double Ci = a.C[i];
-            const double rhs_i = a.theta_m[i] - a.theta_n[i] - Ci * a.h_m[i];
             for (int q = a.inc_ptr[i]; q < a.inc_ptr[i + 1]; ++q) {
-                const int code = a.inc_code[q];
-                const int e = code >> 2;
-                const int la = code & 3;
                 const uint32_t slots = a.inc_slot[q];
-                int nd[K];
+                const double* cq = con + (q - q0) * NC;
 #pragma unroll
-                for (int c = 0; c < K; ++c) nd[c] = a.conn[(size_t)e * K + c];
-                double G[D][K];
-                double det;
-                if constexpr (D == 2) {
-                    const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]];
-                    const double J00 = a.X[nd[1]] - x0, J01 = a.Y[nd[1]] - y0;
-                    const double J10 = a.X[nd[2]] - x0, J11 = a.Y[nd[2]] - y0;
-                    det = J00 * J11 - J01 * J10;
-                    const double id = 1.0 / det;
-                    G[0][1] = J11 * id;  G[1][1] = -J10 * id;
-                    G[0][2] = -J01 * id; G[1][2] = J00 * id;
-                } else {
-                    const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]], z0 = a.Z[nd[0]];
-                    double r[3][3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        r[c][0] = a.X[nd[c + 1]] - x0;
-                        r[c][1] = a.Y[nd[c + 1]] - y0;
-                        r[c][2] = a.Z[nd[c + 1]] - z0;
-                    }
-                    // J rows r1, r2, r3; J^{-1} columns = (r2 x r3, r3 x r1, r1 x r2) / det
-                    double cx[3][3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double* u = r[(c + 1) % 3];
-                        const double* v = r[(c + 2) % 3];
-                        cx[c][0] = u[1] * v[2] - u[2] * v[1];
-                        cx[c][1] = u[2] * v[0] - u[0] * v[2];
-                        cx[c][2] = u[0] * v[1] - u[1] * v[0];
-                    }
-                    det = r[0][0] * cx[0][0] + r[0][1] * cx[0][1] + r[0][2] * cx[0][2];
-                    const double id = 1.0 / det;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) G[d][c + 1] = cx[c][d] * id;
-                }
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int c = 1; c < K; ++c) s += G[d][c];
-                    G[d][0] = -s;
-                }
-                double kbar = 0.0;
-#pragma unroll
-                for (int c = 0; c < K; ++c) kbar += a.qc.ck[c] * a.Kn[nd[c]];
-                if (a.Ks_elem != nullptr) kbar *= a.Ks_elem[e];
-                const double kd = kbar * det;
-                double ga[D];
-                double ml = 0.0;
-#pragma unroll
-                for (int c = 0; c < K; ++c)
-                    if (c == la) {
-                        ml = a.qc.ml[c];
-#pragma unroll
-                        for (int d = 0; d < D; ++d) ga[d] = G[d][c];
-                    }
-                const double mlump = det * ml;
-#pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    double g = 0.0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) g += ga[d] * G[d][c];
-                    double v = kd * g;
-                    if (c == la) v += a.inv_dt * mlump * Ci;
-                    win[off + ((slots >> (8 * c)) & 0xffu)] += v;
-                }
-                bi += -kd * ga[D - 1] - a.inv_dt * mlump * rhs_i;
+                for (int c = 0; c < K; ++c) win[off + ((slots >> (8 * c)) & 0xffu)] += cq[c];
+                bi += cq[K];
             }
         } else {
             win[a.diag_pos[i] - base] = 1.0;
             bi = a.bc_val[i];
         }
         a.b[i] = bi;
-        if (a.dinv != nullptr || a.dinv32 != nullptr) {
-            const double dg = win[a.diag_pos[i] - base];
-            const double di = 1.0 / (dg != 0.0 ? dg : 1.0);
-            if (a.dinv != nullptr) a.dinv[i] = di;
-            if (a.dinv32 != nullptr) a.dinv32[i] = (float)(1.0f / (dg != 0.0 ? (float)dg : 1.0f));
-        }
+        const double dg = win[a.diag_pos[i] - base];
+        const double di = 1.0 / (dg != 0.0 ? dg : 1.0);
+        if (a.dinv != nullptr) a.dinv[i] = di;
+        if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dg != 0.0 ? (float)dg : 1.0f);
+        bsh[threadIdx.x] = bi;
+        dsh[threadIdx.x] = di;
     }
     __syncthreads();
+
+    // 3. coalesced write-out, then the fused Krylov start
     for (int k = threadIdx.x; k < wlen; k += BLOCK) {
         const double v = win[k];
         a.A[base + k] = v;
         if (a.A32 != nullptr) a.A32[base + k] = (float)v;
+    }
+    if constexpr (MODE != ASM_INIT_NONE) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
+        const int i = r0 + g;
+        double s = 0.0;
+        if (i < r1) {
+            const int kb = a.row_ptr[i], ke = a.row_ptr[i + 1];
+            for (int k = kb + l; k < ke; k += LANES) s += win[k - base] * ki.x0[a.cols[k]];
+        }
+#pragma unroll
+        for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
+        if (i < r1 && l == 0) krylov_start<MODE>(a, ki, i, bsh[g], s, dsh[g], acc);
+        store_partials<3>(acc, partials, prow);
     }
 }
 
@@ -168,14 +255,37 @@ void compute_quad_const(int dim, QuadConst& qc) {
     }
 }
 
-hipError_t launch_assemble(int dim, const AsmArgs& a, int max_block_nnz, hipStream_t s) {
-    const int grid = (a.n + BLOCK - 1) / BLOCK;
-    const size_t lds = (size_t)max_block_nnz * sizeof(double);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_assemble_rows<2>, dim3(grid), dim3(BLOCK), lds, s, a);
-    else
-        hipLaunchKernelGGL(k_assemble_rows<3>, dim3(grid), dim3(BLOCK), lds, s, a);
+int asm_rows_per_block(int dim) { return dim == 2 ? ASM_ROWS_2D : ASM_ROWS_3D; }
+
+size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc) {
+    return ((size_t)max_block_nnz + (size_t)max_block_inc * (dim + 2)) * sizeof(double);
+}
+
+template <int D, int R>
+static hipError_t launch_mode(int mode, dim3 g, size_t lds, hipStream_t s, const AsmArgs& a,
+                              const KrylovInit& ki, double* partials, int prow) {
+    switch (mode) {
+        case ASM_INIT_BICG:
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_BICG>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+        case ASM_INIT_CG:
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_CG>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+        case ASM_INIT_GMRES:
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_GMRES>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+        case ASM_INIT_RESID:
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_RESID>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+        default:
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_NONE>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
+                           int prow, size_t lds, int* grid_out, hipStream_t s) {
+    const int R = asm_rows_per_block(dim);
+    const int grid = (a.n + R - 1) / R;
+    if (grid_out) *grid_out = grid;
+    if (dim == 2) return launch_mode<2, ASM_ROWS_2D>(mode, dim3(grid), lds, s, a, ki, partials, prow);
+    return launch_mode<3, ASM_ROWS_3D>(mode, dim3(grid), lds, s, a, ki, partials, prow);
 }
 
 }  // namespace wspm

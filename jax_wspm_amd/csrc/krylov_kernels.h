@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
         double v[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        store_partials<ND>(v, red.partials);
+        store_partials<ND>(v, red.partials, red.prow);
     }
 }
 
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red) {
         double v[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        store_partials<ND>(v, red.partials);
+        store_partials<ND>(v, red.partials, red.prow);
     }
 }
 
@@ -110,7 +110,7 @@ struct BicgInit {  // r = b - A x0; x = x0; r_hat = p = q = r
         const double bi = (double)v.b[i];
         const T rT = (T)(bi - ax);
         v.x[i] = v.x0[i];
-        v.r[i] = rT; v.rh[i] = rT; v.p[i] = rT; v.q[i] = rT;
+        v.r[i] = rT; v.rh[i] = rT;
         const double rr = (double)rT;
         acc[0] += rr * rr;
         acc[1] += bi * bi;
@@ -132,13 +132,18 @@ struct BicgP {  // p = r + beta (p - omega q); p_hat = M p
     static constexpr int ND = 0;
     KVec<T> v;
     double beta, omega;
+    int first;
     __device__ bool skip(const KState* s) const { return s->done; }
     __device__ void load(const KState* s) {
         beta = s->rho_next / s->rho * s->alpha / s->omega;
         omega = s->omega;
+        first = s->k == 0;
     }
     __device__ void row(int i, double*) const {
-        const double pi = (double)v.r[i] + beta * ((double)v.p[i] - omega * (double)v.q[i]);
+        // first iteration: p = q = r0 and omega = 1, so r + beta (p - omega q) = r
+        // exactly; the start kernels therefore do not write p and q.
+        const double pi = first ? (double)v.r[i]
+                                : (double)v.r[i] + beta * ((double)v.p[i] - omega * (double)v.q[i]);
         v.p[i] = (T)pi;
         if (v.dinv) v.ph[i] = (T)((double)v.dinv[i] * (double)(T)pi);
     }

@@ -96,13 +96,18 @@ int build_pattern(int dim, int n, int E, const int32_t* conn, HostPattern& P, st
         }
     }
 
-    // LDS window per assembly block: nnz spanned by ASM_BLOCK consecutive rows
-    P.max_block_nnz = 0;
-    for (int r0 = 0; r0 < n; r0 += ASM_BLOCK) {
-        const int r1 = std::min(n, r0 + ASM_BLOCK);
-        P.max_block_nnz = std::max<int>(P.max_block_nnz, P.row_ptr[r1] - P.row_ptr[r0]);
-    }
     return WSPM_OK;
+}
+
+void block_maxima(const HostPattern& P, int R, int& max_nnz, int& max_inc) {
+    const int n = (int)P.row_ptr.size() - 1;
+    max_nnz = 0;
+    max_inc = 0;
+    for (int r0 = 0; r0 < n; r0 += R) {
+        const int r1 = std::min(n, r0 + R);
+        max_nnz = std::max<int>(max_nnz, P.row_ptr[r1] - P.row_ptr[r0]);
+        max_inc = std::max<int>(max_inc, P.inc_ptr[r1] - P.inc_ptr[r0]);
+    }
 }
 
 }  // namespace wspm

@@ -9,8 +9,8 @@ namespace wspm {
 
 constexpr int BLOCK = 256;          // threads per workgroup (4 wavefronts of 64)
 constexpr int NWAVE = BLOCK / 64;
-constexpr int MAX_BLOCKS = 2048;    // cap on reduction grids (partials buffer rows)
-constexpr int MAX_DOTS = 72;        // partials buffer columns
+constexpr int MAX_BLOCKS = 2048;    // cap on the grids of SpMV / vector reductions
+constexpr int MAX_DOTS = 8;         // dot products per reduction kernel
 constexpr int MAX_RESTART = 64;
 
 // Scalars of one Krylov solve, resident in device memory.  Written only by
@@ -71,25 +71,27 @@ __device__ inline void block_sum(double (&v)[ND], double* sh) {
 }
 
 struct Red {
-    double* partials;     // [MAX_BLOCKS][MAX_DOTS]
+    double* partials;     // column-major [MAX_DOTS][prow]: dot j of workgroup b at j * prow + b
+    int prow;
     KState* st;
 };
 
-// Store this workgroup's ND block sums as row blockIdx of the partials.
+// Store this workgroup's ND block sums in column blockIdx of the partials.
 template <int ND>
-__device__ inline void store_partials(double (&v)[ND], double* partials) {
+__device__ inline void store_partials(double (&v)[ND], double* partials, int prow) {
     __shared__ double sh[ND * NWAVE];
     block_sum<ND>(v, sh);
     if (threadIdx.x == 0) {
         const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
-        for (int j = 0; j < ND; ++j) partials[(size_t)bid * MAX_DOTS + j] = v[j];
+        for (int j = 0; j < ND; ++j) partials[(size_t)j * prow + bid] = v[j];
     }
 }
 
-// Sum nblk partial rows in order and run op.fin(totals, state).  One workgroup.
+// Sum nblk partials per dot in workgroup order and run op.fin(totals, state).
+// One workgroup; the reads are coalesced.
 template <int ND, class Op>
-__global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restrict__ partials, int nblk,
+__global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restrict__ partials, int prow, int nblk,
                                                    KState* st) {
     if (op.skip(st)) return;
     double acc[ND];
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restric
     for (int j = 0; j < ND; ++j) acc[j] = 0.0;
     for (int b = threadIdx.x; b < nblk; b += BLOCK) {
 #pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)b * MAX_DOTS + j];
+        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
     }
     __shared__ double sh[ND * NWAVE];
     block_sum<ND>(acc, sh);
@@ -116,14 +118,21 @@ __device__ inline void soil_eval(const SoilParams& sp, double h, double& C, doub
     if (sp.model == 1) {
         const double alpha = sp.p[0], ts = sp.p[1], tr = sp.p[2], n = sp.p[3], m = sp.p[4], ks = sp.p[5];
         if (h < 0.0) {
+            // four pow calls instead of six: (a|h|)^(n-1) = (a|h|)^n / (a|h|),
+            // (1+(a|h|)^n)^(-m-1) = (1+(a|h|)^n)^(-m) / (1+(a|h|)^n), Se^0.5 = sqrt(Se)
+            // (each within an ulp of the reference's pow); the Mualem term keeps
+            // the reference's form 1 - (1 - Se^(1/m))^m, whose cancellation near
+            // saturation a rewrite would change.
             const double ah = alpha * fabs(h);
             const double ahn = pow(ah, n);
             const double base = 1.0 + ahn;
-            th = (ts - tr) * pow(base, -m) + tr;
+            const double bm = pow(base, -m);
+            th = (ts - tr) * bm + tr;
             const double se = (th - tr) / (ts - tr);
             const double inner = 1.0 - pow(1.0 - pow(se, 1.0 / m), m);
-            K = ks * (pow(se, 0.5) * (inner * inner));
-            C = (ts - tr) * alpha * m * n * pow(ah, n - 1.0) * pow(base, -m - 1.0);
+            K = ks * (sqrt(se) * (inner * inner));
+            const double ahn1 = ah > 0.0 ? ahn / ah : pow(ah, n - 1.0);
+            C = (ts - tr) * alpha * m * n * ahn1 * (bm / base);
         } else {
             th = ts;
             K = ks;

@@ -9,9 +9,6 @@
 
 namespace wspm {
 
-// Rows per assembly workgroup (one thread per row).
-constexpr int ASM_BLOCK = 256;
-
 struct HostPattern {
     std::vector<int32_t> row_ptr;    // n+1
     std::vector<int32_t> cols;       // nnz
@@ -20,9 +17,10 @@ struct HostPattern {
     std::vector<int32_t> inc_code;   // (element << 2) | local vertex
     std::vector<uint32_t> inc_slot;  // byte c = in-row offset of vertex c
     int max_row = 0;
-    int max_block_nnz = 0;
 };
 
 int build_pattern(int dim, int n, int E, const int32_t* conn, HostPattern& P, std::string& err);
+// Largest CSR-value and incidence counts spanned by R consecutive rows.
+void block_maxima(const HostPattern& P, int R, int& max_nnz, int& max_inc);
 
 }  // namespace wspm
