@@ -714,6 +714,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     AsmArgs a{};
     a.n = ctx->n;
     a.win_cap = ctx->asm_win;
+    a.inc_cap = ctx->asm_inc;
     a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
     a.inc_code = ctx->inc_code; a.inc_slot = ctx->inc_slot; a.conn = ctx->conn;
     a.X = ctx->X; a.Y = ctx->Y; a.Z = ctx->Z;

@@ -13,7 +13,7 @@ namespace wspm {
 // (incidences x (dim+2) doubles + the rows' CSR window) and the write-out
 // lanes per row (256 / R).
 constexpr int ASM_ROWS_2D = 64;
-constexpr int ASM_ROWS_3D = 32;
+constexpr int ASM_ROWS_3D = 16;
 
 // What the assembly epilogue also computes (the first step of the solver).
 enum { ASM_INIT_NONE = 0, ASM_INIT_BICG = 1, ASM_INIT_CG = 2, ASM_INIT_GMRES = 3, ASM_INIT_RESID = 4 };
@@ -30,6 +30,7 @@ struct QuadConst {
 struct AsmArgs {
     int n;
     int win_cap;             // LDS doubles reserved for the CSR window
+    int inc_cap;             // LDS incidence records reserved
     const int32_t* row_ptr;
     const int32_t* cols;
     const int32_t* diag_pos;

@@ -162,16 +162,23 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
     const int wlen = a.row_ptr[r1] - base;
     const int q0 = a.inc_ptr[r0];
     const int nq = a.inc_ptr[r1] - q0;
-    double* win = lds;                        // [max_block_nnz]
-    double* con = lds + a.win_cap;            // [max_block_inc][NC]
-    for (int k = threadIdx.x; k < wlen; k += BLOCK) win[k] = 0.0;
+    double* win = lds;                                    // [win_cap] CSR window
+    double* xw = lds + a.win_cap;                         // [win_cap] x0 at the window's columns
+    double* con = lds + 2 * a.win_cap;                    // [inc_cap][NC] incidence contributions
+    uint32_t* slw = reinterpret_cast<uint32_t*>(con + (size_t)a.inc_cap * NC);  // [inc_cap] slot words
+    for (int k = threadIdx.x; k < wlen; k += BLOCK) {
+        win[k] = 0.0;
+        if constexpr (MODE != ASM_INIT_NONE) xw[k] = ki.x0[a.cols[base + k]];
+    }
 
-    // 1. element row contributions of every incidence of the workgroup
+    // 1. element row contributions of every incidence of the workgroup; all
+    //    global reads of the kernel happen in this phase
     for (int t = threadIdx.x; t < nq; t += BLOCK) {
         double out[D + 2];
         element_row<D>(a, q0 + t, out);
 #pragma unroll
         for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
+        slw[t] = a.inc_slot[q0 + t];
     }
     __syncthreads();
 
@@ -183,7 +190,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
         double bi = 0.0;
         if (!is_bc) {
             for (int q = a.inc_ptr[i]; q < a.inc_ptr[i + 1]; ++q) {
-                const uint32_t slots = a.inc_slot[q];
+                const uint32_t slots = slw[q - q0];
                 const double* cq = con + (q - q0) * NC;
 #pragma unroll
                 for (int c = 0; c < K; ++c) win[off + ((slots >> (8 * c)) & 0xffu)] += cq[c];
@@ -216,7 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
         double s = 0.0;
         if (i < r1) {
             const int kb = a.row_ptr[i], ke = a.row_ptr[i + 1];
-            for (int k = kb + l; k < ke; k += LANES) s += win[k - base] * ki.x0[a.cols[k]];
+            for (int k = kb + l; k < ke; k += LANES) s += win[k - base] * xw[k - base];
         }
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
@@ -258,7 +265,8 @@ void compute_quad_const(int dim, QuadConst& qc) {
 int asm_rows_per_block(int dim) { return dim == 2 ? ASM_ROWS_2D : ASM_ROWS_3D; }
 
 size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc) {
-    return ((size_t)max_block_nnz + (size_t)max_block_inc * (dim + 2)) * sizeof(double);
+    return (2 * (size_t)max_block_nnz + (size_t)max_block_inc * (dim + 2)) * sizeof(double) +
+           (size_t)max_block_inc * sizeof(uint32_t);
 }
 
 template <int D, int R>
