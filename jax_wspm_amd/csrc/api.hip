@@ -427,76 +427,77 @@ KVec<T> make_kvec(wspm_ctx* ctx, KWork<T>& w, const T* dinv, const T* b, const T
     return v;
 }
 
-// One Krylov solve.  Kernels early-exit once the device marks the solve done,
-// so iterations are enqueued in chunks and the host reads back once per chunk.
-// started: the first step (r0 = b - A x0, its dots and the solver scalars) was
-// already done by the assembly epilogue (k_assemble with a Krylov start mode).
+// One Krylov solve, split so callers can enqueue other work between the
+// iterations and the read-back.  Kernels early-exit once the device marks the
+// solve done, so iterations are enqueued ahead in chunks.
 template <class T>
-void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T* b, const T* x0, T* x,
-                  double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st,
-                  bool started = false) {
-    const int n = ctx->n;
-    KState& hs = ctx->hstate[which];
-    if (method == WSPM_BICGSTAB || method == WSPM_CG) {
-        KWork<T>& w = ensure_work<T>(ctx, 0);
-        KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
-        if (started) {
-        } else if (method == WSPM_BICGSTAB) {
-            BicgInit<T> e{v, tol, atol, maxiter};
-            launch_spmv<T>(ctx, A, e, which, true);
+struct KrylovRun {
+    wspm_ctx* ctx;
+    int method;
+    const T* A;
+    const T* dinv;
+    KVec<T> v;
+    int which;
+    int maxiter;
+    int R = 0;          // GMRES restart actually used
+    double* H = nullptr;
+    KState* ds = nullptr;
+    int nbx = 0;
+    int enq = 0;        // iterations (CG/BiCGSTAB) or cycles (GMRES) enqueued
+
+    KrylovRun(wspm_ctx* c, int m, const T* A_, const T* dinv_, const T* b, const T* x0, T* x, double tol,
+              double atol, int maxit, int restart, int wh, bool started)
+        : ctx(c), method(m), A(A_), dinv(dinv_), which(wh), maxiter(maxit) {
+        const int n = ctx->n;
+        ds = ctx->states + which;
+        if (method == WSPM_GMRES) {
+            R = std::min(std::min(restart, n), MAX_RESTART);
+            KWork<T>& w = ensure_work<T>(ctx, R + 1);
+            v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+            H = (which == 1) ? ctx->H32 : ctx->H;
+            nbx = std::min((n + BLOCK - 1) / BLOCK, 256);
+            if (!started) launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
+            launch_vec(ctx, GmScale<T>{v, 0.0}, which);
         } else {
-            CgInit<T> e{v, tol, atol, maxiter};
-            launch_spmv<T>(ctx, A, e, which, true);
-        }
-        int chunk = std::max(1, ctx->last_iters[method]);
-        int enq = 0;
-        while (true) {
-            for (int c = 0; c < chunk && enq < maxiter; ++c, ++enq) {
-                if (method == WSPM_BICGSTAB) {
-                    launch_vec(ctx, BicgP<T>{v, 0.0, 0.0}, which);
-                    launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true);
-                    launch_vec(ctx, BicgS<T>{v, 0.0}, which);
-                    launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true);
-                    launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which);
-                } else {
-                    launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true);
-                    launch_vec(ctx, CgXR<T>{v, 0.0}, which);
-                    launch_vec(ctx, CgP<T>{v, 0.0}, which);
-                }
+            KWork<T>& w = ensure_work<T>(ctx, 0);
+            v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+            if (started) {
+            } else if (method == WSPM_BICGSTAB) {
+                launch_spmv<T>(ctx, A, BicgInit<T>{v, tol, atol, maxiter}, which, true);
+            } else {
+                launch_spmv<T>(ctx, A, CgInit<T>{v, tol, atol, maxiter}, which, true);
             }
-            poll(ctx, which);
-            if (hs.done || enq >= maxiter) break;
-            chunk = 1;
         }
-        ctx->last_iters[method] = std::max(1, hs.steps);
-        if (st) {
-            st->iters = hs.steps;
-            st->arnoldi = 0;
-            st->code = hs.code;
-            st->res_norm = std::sqrt(hs.rs);
-            st->b_norm = std::sqrt(hs.bs);
-        }
-        return;
     }
-    // GMRES (JAX solve_method='batched')
-    const int R = std::min(std::min(restart, n), MAX_RESTART);
-    KWork<T>& w = ensure_work<T>(ctx, R + 1);
-    KVec<T> v = make_kvec<T>(ctx, w, dinv, b, x0, x);
-    double* H = (which == 1) ? ctx->H32 : ctx->H;
-    if (!started) launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
-    launch_vec(ctx, GmScale<T>{v, 0.0}, which);
-    KState* ds = ctx->states + which;
-    const int nbx = std::min((n + BLOCK - 1) / BLOCK, 256);
-    for (int cyc = 0; cyc < maxiter; ++cyc) {
+
+    void enqueue(int count) {
+        for (int c = 0; c < count && enq < maxiter; ++c, ++enq) {
+            if (method == WSPM_BICGSTAB) {
+                launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which);
+                launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true);
+                launch_vec(ctx, BicgS<T>{v, 0.0}, which);
+                launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true);
+                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which);
+            } else if (method == WSPM_CG) {
+                launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true);
+                launch_vec(ctx, CgXR<T>{v, 0.0}, which);
+                launch_vec(ctx, CgP<T>{v, 0.0}, which);
+            } else {
+                gmres_cycle();
+            }
+        }
+    }
+
+    void gmres_cycle() {
         hipLaunchKernelGGL(k_gm_cycle_begin, dim3(1), dim3(256), 0, ctx->stream, ds, H, R);
         HIPCHK(hipGetLastError());
         for (int j = 0; j < R; ++j) {
-            launch_spmv<T>(ctx, (const T*)A, GmArnoldi<T>{v, j}, which, true);
+            launch_spmv<T>(ctx, A, GmArnoldi<T>{v, j}, which, true);
             hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
                                red_of(ctx, which));
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials, ctx->prow,
-                               j, nbx);
+            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials,
+                               ctx->prow, j, nbx);
             HIPCHK(hipGetLastError());
             GmUpdate<T> up{v, j, H};
             up.hb = state_field<double>(ds, offsetof(KState, hbuf));
@@ -508,16 +509,48 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
         launch_vec(ctx, GmUpdX<T>{v, R, state_field<double>(ds, offsetof(KState, y))}, which);
         launch_spmv<T>(ctx, A, GmResid<T>{v}, which, true);
         launch_vec(ctx, GmScale<T>{v, 0.0}, which);
+    }
+
+    // First chunk: as many iterations as the previous solve of this method took.
+    int first_chunk() const { return method == WSPM_GMRES ? 1 : std::max(1, ctx->last_iters[method]); }
+
+    // After a poll: can more iterations change anything?
+    bool more() const { return !ctx->hstate[which].done && enq < maxiter; }
+
+    void stats(wspm_krylov_stats* st) {
+        const KState& hs = ctx->hstate[which];
+        if (method != WSPM_GMRES) ctx->last_iters[method] = std::max(1, hs.steps);
+        if (!st) return;
+        if (method == WSPM_GMRES) {
+            st->iters = hs.cycles;
+            st->arnoldi = hs.arnoldi;
+            st->code = 0;
+            st->res_norm = hs.rnorm;
+            st->b_norm = hs.bnorm;
+        } else {
+            st->iters = hs.steps;
+            st->arnoldi = 0;
+            st->code = hs.code;
+            st->res_norm = std::sqrt(hs.rs);
+            st->b_norm = std::sqrt(hs.bs);
+        }
+    }
+};
+
+// started: the first step (r0 = b - A x0, its dots and the solver scalars) was
+// already done by the assembly epilogue (k_assemble with a Krylov start mode).
+template <class T>
+void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T* b, const T* x0, T* x,
+                  double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st,
+                  bool started = false) {
+    KrylovRun<T> run(ctx, method, A, dinv, b, x0, x, tol, atol, maxiter, restart, which, started);
+    run.enqueue(run.first_chunk());
+    poll(ctx, which);
+    while (run.more()) {
+        run.enqueue(1);
         poll(ctx, which);
-        if (hs.done) break;
     }
-    if (st) {
-        st->iters = hs.cycles;
-        st->arnoldi = hs.arnoldi;
-        st->code = 0;
-        st->res_norm = hs.rnorm;
-        st->b_norm = hs.bnorm;
-    }
+    run.stats(st);
 }
 
 }  // namespace
@@ -959,17 +992,32 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                     jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
                     mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
         wspm_krylov_stats st{};
+        const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm, P.picard_mode == WSPM_PICARD_REL};
         if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
                         ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, true);
+            launch_vec(ctx, pop, 0);
+            poll(ctx, 0);
         } else {
-            krylov_solve<double>(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
-                                 P.tol, P.atol, P.maxiter, P.restart, 0, &st, true);
+            // Enqueue the expected number of Krylov iterations and the Picard
+            // error behind them, then read both back with one synchronisation;
+            // if the solve needed more iterations, finish it and redo the error.
+            KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
+                                  P.tol, P.atol, P.maxiter, P.restart, 0, true);
+            run.enqueue(run.first_chunk());
+            launch_vec(ctx, pop, 0);
+            poll(ctx, 0);
+            if (run.more()) {
+                while (run.more()) {
+                    run.enqueue(1);
+                    poll(ctx, 0);
+                }
+                launch_vec(ctx, pop, 0);
+                poll(ctx, 0);
+            }
+            run.stats(&st);
         }
         kry += st.iters;
-        launch_vec(ctx, PicardOp{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm,
-                                 P.picard_mode == WSPM_PICARD_REL}, 0);
-        poll(ctx, 0);
         err = ctx->hstate[0].picard_err;
         std::swap(ctx->hA, ctx->hB);
         ++it;

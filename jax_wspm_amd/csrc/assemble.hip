@@ -16,13 +16,14 @@
 //      constant on a P1 element, one inverse instead of one per quadrature
 //      point) and writes that element's row contribution (dim+1 matrix values
 //      + 1 source value) to LDS;
-//   2. one thread per row adds its incidences' contributions, in the fixed
-//      incidence order, into an LDS window over the rows' CSR values (identity
-//      row and value for Dirichlet rows), and the Jacobi inverse diagonal;
-//   3. LANES = 256/R threads per row stream the finished window to HBM with
-//      coalesced stores and, optionally, fuse the first step of the Krylov
-//      solve: r0 = b - A x0 and its dot products (the SpMV the solver would
-//      otherwise start with, on values that are still in LDS).
+//   2. one thread per CSR entry of the rows sums, in the fixed incidence
+//      order, the contributions that land in its slot (identity rows for
+//      Dirichlet nodes), writes it to HBM (coalesced) and to an LDS copy of
+//      the rows' window; the diagonal's thread also sums the source term and
+//      takes the Jacobi inverse diagonal;
+//   3. optionally, LANES = 256/R threads per row fuse the first step of the
+//      Krylov solve: r0 = b - A x0 and its dot products (the SpMV the solver
+//      would otherwise start with, on values that are still in LDS).
 // No atomics, no COO staging, fixed summation order: bitwise repeatable.
 #include "assemble.h"
 #include "wspm_device.h"
@@ -153,26 +154,36 @@ template <int D, int R, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     constexpr int K = D + 1;
     constexpr int NC = K + 1;                 // values per incidence in LDS
-    constexpr int LANES = BLOCK / R;          // threads per row in the write-out
+    constexpr int LANES = BLOCK / R;          // threads per row in the fused Krylov start
     extern __shared__ double lds[];
+    __shared__ int rp_s[R + 1], ip_s[R + 1], dg_s[R];
+    __shared__ uint8_t bc_s[R];
     __shared__ double bsh[R], dsh[R];
     const int r0 = blockIdx.x * R;
     const int r1 = min(a.n, r0 + R);
+    const int nrows = r1 - r0;
     const int base = a.row_ptr[r0];
     const int wlen = a.row_ptr[r1] - base;
     const int q0 = a.inc_ptr[r0];
     const int nq = a.inc_ptr[r1] - q0;
-    double* win = lds;                                    // [win_cap] CSR window
+    double* win = lds;                                    // [win_cap] finished CSR window
     double* xw = lds + a.win_cap;                         // [win_cap] x0 at the window's columns
     double* con = lds + 2 * a.win_cap;                    // [inc_cap][NC] incidence contributions
     uint32_t* slw = reinterpret_cast<uint32_t*>(con + (size_t)a.inc_cap * NC);  // [inc_cap] slot words
-    for (int k = threadIdx.x; k < wlen; k += BLOCK) {
-        win[k] = 0.0;
-        if constexpr (MODE != ASM_INIT_NONE) xw[k] = ki.x0[a.cols[base + k]];
-    }
 
-    // 1. element row contributions of every incidence of the workgroup; all
-    //    global reads of the kernel happen in this phase
+    // 0. row-level metadata and the x0 gathers (all global reads happen in 0-1)
+    for (int j = threadIdx.x; j <= nrows; j += BLOCK) {
+        rp_s[j] = a.row_ptr[r0 + j] - base;
+        ip_s[j] = a.inc_ptr[r0 + j] - q0;
+        if (j < nrows) {
+            dg_s[j] = a.diag_pos[r0 + j] - base;
+            bc_s[j] = a.bc_flag != nullptr ? a.bc_flag[r0 + j] : 0;
+        }
+    }
+    if constexpr (MODE != ASM_INIT_NONE)
+        for (int k = threadIdx.x; k < wlen; k += BLOCK) xw[k] = ki.x0[a.cols[base + k]];
+
+    // 1. element row contributions of every incidence of the workgroup
     for (int t = threadIdx.x; t < nq; t += BLOCK) {
         double out[D + 2];
         element_row<D>(a, q0 + t, out);
@@ -182,52 +193,57 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
     }
     __syncthreads();
 
-    // 2. one thread per row: fixed-order sums into the CSR window
-    if (threadIdx.x < r1 - r0) {
-        const int i = r0 + threadIdx.x;
-        const int off = a.row_ptr[i] - base;
-        const bool is_bc = a.bc_flag != nullptr && a.bc_flag[i];
-        double bi = 0.0;
-        if (!is_bc) {
-            for (int q = a.inc_ptr[i]; q < a.inc_ptr[i + 1]; ++q) {
-                const uint32_t slots = slw[q - q0];
-                const double* cq = con + (q - q0) * NC;
-#pragma unroll
-                for (int c = 0; c < K; ++c) win[off + ((slots >> (8 * c)) & 0xffu)] += cq[c];
-                bi += cq[K];
-            }
-        } else {
-            win[a.diag_pos[i] - base] = 1.0;
-            bi = a.bc_val[i];
-        }
-        a.b[i] = bi;
-        const double dg = win[a.diag_pos[i] - base];
-        const double di = 1.0 / (dg != 0.0 ? dg : 1.0);
-        if (a.dinv != nullptr) a.dinv[i] = di;
-        if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dg != 0.0 ? (float)dg : 1.0f);
-        bsh[threadIdx.x] = bi;
-        dsh[threadIdx.x] = di;
-    }
-    __syncthreads();
-
-    // 3. coalesced write-out, then the fused Krylov start
+    // 2. one thread per CSR entry of the window: sum the contributions whose
+    //    vertex lands in this entry's slot, in incidence order (the diagonal
+    //    entry's thread also sums the row's source term).  Dirichlet rows are
+    //    identity rows.  Written straight to HBM, coalesced.
     for (int k = threadIdx.x; k < wlen; k += BLOCK) {
-        const double v = win[k];
+        int lo = 0, hi = nrows - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rp_s[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const int j = lo;
+        const uint32_t sl = (uint32_t)(k - rp_s[j]);
+        const bool diag = (k == dg_s[j]);
+        double v = 0.0, bi = 0.0;
+        if (bc_s[j]) {
+            v = diag ? 1.0 : 0.0;
+            bi = a.bc_val[r0 + j];
+        } else {
+            for (int t = ip_s[j]; t < ip_s[j + 1]; ++t) {
+                const uint32_t w = slw[t];
+#pragma unroll
+                for (int c = 0; c < K; ++c)
+                    if (((w >> (8 * c)) & 0xffu) == sl) v += con[t * NC + c];
+                if (diag) bi += con[t * NC + K];
+            }
+        }
+        win[k] = v;
         a.A[base + k] = v;
         if (a.A32 != nullptr) a.A32[base + k] = (float)v;
+        if (diag) {
+            const int i = r0 + j;
+            a.b[i] = bi;
+            const double di = 1.0 / (v != 0.0 ? v : 1.0);
+            if (a.dinv != nullptr) a.dinv[i] = di;
+            if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (v != 0.0 ? (float)v : 1.0f);
+            bsh[j] = bi;
+            dsh[j] = di;
+        }
     }
+
+    // 3. fused Krylov start on the finished window (LANES threads per row)
     if constexpr (MODE != ASM_INIT_NONE) {
+        __syncthreads();
         double acc[3] = {0.0, 0.0, 0.0};
         const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
-        const int i = r0 + g;
         double s = 0.0;
-        if (i < r1) {
-            const int kb = a.row_ptr[i], ke = a.row_ptr[i + 1];
-            for (int k = kb + l; k < ke; k += LANES) s += win[k - base] * xw[k - base];
-        }
+        if (g < nrows)
+            for (int k = rp_s[g] + l; k < rp_s[g + 1]; k += LANES) s += win[k] * xw[k];
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
-        if (i < r1 && l == 0) krylov_start<MODE>(a, ki, i, bsh[g], s, dsh[g], acc);
+        if (g < nrows && l == 0) krylov_start<MODE>(a, ki, r0 + g, bsh[g], s, dsh[g], acc);
         store_partials<3>(acc, partials, prow);
     }
 }
