@@ -75,10 +75,19 @@ _lib = None
 
 
 def load(path: os.PathLike | None = None):
-    """Load libwspm.so (once).  Raises if it is missing: there is no fallback."""
+    """Load libwspm.so (once).  Raises if it is missing: there is no fallback.
+
+    PyTorch is imported first: its wheel bundles a HIP runtime whose SONAME
+    (libamdhip64.so.7) is the one libwspm.so links against, so the library
+    binds to the runtime that owns the tensors and streams handed to it.
+    Loaded the other way round, the process would carry two HIP runtimes."""
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - plain C use without torch
+        pass
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise ImportError(f"{p} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -12,8 +12,18 @@ namespace wspm {
 // entries, 3D rows ~24 incident tets and ~15 entries; R sets the LDS staging
 // (incidences x (dim+2) doubles + the rows' CSR window) and the write-out
 // lanes per row (256 / R).
-constexpr int ASM_ROWS_2D = 64;
-constexpr int ASM_ROWS_3D = 16;
+#ifndef WSPM_ASM_ROWS_2D
+#define WSPM_ASM_ROWS_2D 64
+#endif
+#ifndef WSPM_ASM_ROWS_3D
+#define WSPM_ASM_ROWS_3D 16
+#endif
+#ifndef WSPM_ASM_THREADS
+#define WSPM_ASM_THREADS 256
+#endif
+constexpr int ASM_ROWS_2D = WSPM_ASM_ROWS_2D;
+constexpr int ASM_ROWS_3D = WSPM_ASM_ROWS_3D;
+constexpr int ASM_THREADS = WSPM_ASM_THREADS;   // threads per assembly workgroup
 
 // What the assembly epilogue also computes (the first step of the solver).
 enum { ASM_INIT_NONE = 0, ASM_INIT_BICG = 1, ASM_INIT_CG = 2, ASM_INIT_GMRES = 3, ASM_INIT_RESID = 4 };

@@ -151,10 +151,11 @@ __device__ inline void krylov_start(const AsmArgs& a, const KrylovInit& ki, int 
 }
 
 template <int D, int R, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+__global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     constexpr int K = D + 1;
     constexpr int NC = K + 1;                 // values per incidence in LDS
-    constexpr int LANES = BLOCK / R;          // threads per row in the fused Krylov start
+    constexpr int LANES = ASM_THREADS / R;    // threads per row in the fused Krylov start
+    static_assert(LANES >= 1 && LANES <= 64 && (LANES & (LANES - 1)) == 0, "bad assembly shape");
     extern __shared__ double lds[];
     __shared__ int rp_s[R + 1], ip_s[R + 1], dg_s[R];
     __shared__ uint8_t bc_s[R];
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
     uint32_t* slw = reinterpret_cast<uint32_t*>(con + (size_t)a.inc_cap * NC);  // [inc_cap] slot words
 
     // 0. row-level metadata and the x0 gathers (all global reads happen in 0-1)
-    for (int j = threadIdx.x; j <= nrows; j += BLOCK) {
+    for (int j = threadIdx.x; j <= nrows; j += ASM_THREADS) {
         rp_s[j] = a.row_ptr[r0 + j] - base;
         ip_s[j] = a.inc_ptr[r0 + j] - q0;
         if (j < nrows) {
@@ -181,10 +182,10 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
         }
     }
     if constexpr (MODE != ASM_INIT_NONE)
-        for (int k = threadIdx.x; k < wlen; k += BLOCK) xw[k] = ki.x0[a.cols[base + k]];
+        for (int k = threadIdx.x; k < wlen; k += ASM_THREADS) xw[k] = ki.x0[a.cols[base + k]];
 
     // 1. element row contributions of every incidence of the workgroup
-    for (int t = threadIdx.x; t < nq; t += BLOCK) {
+    for (int t = threadIdx.x; t < nq; t += ASM_THREADS) {
         double out[D + 2];
         element_row<D>(a, q0 + t, out);
 #pragma unroll
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
     //    vertex lands in this entry's slot, in incidence order (the diagonal
     //    entry's thread also sums the row's source term).  Dirichlet rows are
     //    identity rows.  Written straight to HBM, coalesced.
-    for (int k = threadIdx.x; k < wlen; k += BLOCK) {
+    for (int k = threadIdx.x; k < wlen; k += ASM_THREADS) {
         int lo = 0, hi = nrows - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(AsmArgs a, KrylovInit ki, do
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
         if (g < nrows && l == 0) krylov_start<MODE>(a, ki, r0 + g, bsh[g], s, dsh[g], acc);
-        store_partials<3>(acc, partials, prow);
+        store_partials<3, ASM_THREADS / 64>(acc, partials, prow);
     }
 }
 
@@ -290,15 +291,15 @@ static hipError_t launch_mode(int mode, dim3 g, size_t lds, hipStream_t s, const
                               const KrylovInit& ki, double* partials, int prow) {
     switch (mode) {
         case ASM_INIT_BICG:
-            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_BICG>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_BICG>), g, dim3(ASM_THREADS), lds, s, a, ki, partials, prow); break;
         case ASM_INIT_CG:
-            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_CG>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_CG>), g, dim3(ASM_THREADS), lds, s, a, ki, partials, prow); break;
         case ASM_INIT_GMRES:
-            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_GMRES>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_GMRES>), g, dim3(ASM_THREADS), lds, s, a, ki, partials, prow); break;
         case ASM_INIT_RESID:
-            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_RESID>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_RESID>), g, dim3(ASM_THREADS), lds, s, a, ki, partials, prow); break;
         default:
-            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_NONE>), g, dim3(BLOCK), lds, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble<D, R, ASM_INIT_NONE>), g, dim3(ASM_THREADS), lds, s, a, ki, partials, prow); break;
     }
     return hipGetLastError();
 }

@@ -44,7 +44,7 @@ struct KState {
 // most of its output, once per workgroup).  Results depend only on the launch
 // shape, never on scheduling.
 // ---------------------------------------------------------------------------
-template <int ND>
+template <int ND, int NW = NWAVE>
 __device__ inline void block_sum(double (&v)[ND], double* sh) {
 #pragma unroll
     for (int j = 0; j < ND; ++j) {
@@ -57,14 +57,14 @@ __device__ inline void block_sum(double (&v)[ND], double* sh) {
     __syncthreads();
     if (lane == 0) {
 #pragma unroll
-        for (int j = 0; j < ND; ++j) sh[j * NWAVE + wid] = v[j];
+        for (int j = 0; j < ND; ++j) sh[j * NW + wid] = v[j];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
             double s = 0.0;
-            for (int w = 0; w < NWAVE; ++w) s += sh[j * NWAVE + w];
+            for (int w = 0; w < NW; ++w) s += sh[j * NW + w];
             v[j] = s;
         }
     }
@@ -76,11 +76,12 @@ struct Red {
     KState* st;
 };
 
-// Store this workgroup's ND block sums in column blockIdx of the partials.
-template <int ND>
+// Store this workgroup's ND block sums in column blockIdx of the partials
+// (NW = wavefronts per workgroup).
+template <int ND, int NW = NWAVE>
 __device__ inline void store_partials(double (&v)[ND], double* partials, int prow) {
-    __shared__ double sh[ND * NWAVE];
-    block_sum<ND>(v, sh);
+    __shared__ double sh[ND * NW];
+    block_sum<ND, NW>(v, sh);
     if (threadIdx.x == 0) {
         const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
