@@ -39,17 +39,21 @@ struct Err {};
 // ---------------------------------------------------------------------------
 // Soil / Picard kernels
 // ---------------------------------------------------------------------------
+// K goes to K[i] and/or to the node record (krec[4 i], stride of double4).
 struct SoilOp {
     static constexpr int ND = 0;
     SoilParams sp;
     const double* h;
     double *C, *K, *th;
+    double* krec;
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState*) {}
     __device__ void row(int i, double*) const {
         double c, k, t;
         soil_eval(sp, h[i], c, k, t);
-        C[i] = c; K[i] = k; th[i] = t;
+        C[i] = c; th[i] = t;
+        if (K) K[i] = k;
+        if (krec) krec[4 * (size_t)i] = k;
     }
     __device__ void fin(const double*, KState*) const {}
 };
@@ -62,6 +66,7 @@ struct PicardOp {
     const double* h;
     const double* hm;
     double *C, *K, *th;
+    double* krec;
     int rel;
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState*) {}
@@ -72,7 +77,9 @@ struct PicardOp {
         acc[1] += hi * hi;
         double c, k, t;
         soil_eval(sp, hi, c, k, t);
-        C[i] = c; K[i] = k; th[i] = t;
+        C[i] = c; th[i] = t;
+        if (K) K[i] = k;
+        if (krec) krec[4 * (size_t)i] = k;
     }
     __device__ void fin(const double* t, KState* s) const {
         s->picard_d2 = t[0];
@@ -126,6 +133,16 @@ struct RefCorrect {  // x += |r| d32
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState* s) { scale = s->rnorm; }
     __device__ void row(int i, double*) const { x[i] += scale * (double)d32[i]; }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct SetK {  // node records' K from a caller array
+    static constexpr int ND = 0;
+    const double* K;
+    double* krec;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double*) const { krec[4 * (size_t)i] = K[i]; }
     __device__ void fin(const double*, KState*) const {}
 };
 
@@ -269,10 +286,11 @@ struct wspm_ctx {
     int spmv_grid = 1, vec_grid = 1;
     HostPattern hp;
     // device mesh/pattern
-    int32_t *row_ptr = nullptr, *cols = nullptr, *diag_pos = nullptr, *inc_ptr = nullptr,
-            *inc_code = nullptr, *conn = nullptr;
-    uint32_t* inc_slot = nullptr;
-    double *X = nullptr, *Y = nullptr, *Z = nullptr;
+    int32_t *row_ptr = nullptr, *cols = nullptr, *diag_pos = nullptr, *inc_ptr = nullptr;
+    int2* inc = nullptr;            // (element << 2 | vertex, slot word) per incidence
+    int4* conn4 = nullptr;          // padded connectivity
+    double4* nrec = nullptr;        // (x, y, K, -) / (x, y, z, K) node records
+    bool nrec_dirty = false;        // K in nrec changed by a direct assembly call
     uint8_t* bc_flag = nullptr;
     double* bc_val = nullptr;
     int n_bc = 0;
@@ -293,7 +311,7 @@ struct wspm_ctx {
     bool has_ks = false;
     wspm_richards_params prm{};
     SoilParams sp{};
-    double *hA = nullptr, *hB = nullptr, *Cn = nullptr, *Kn = nullptr, *thm = nullptr,
+    double *hA = nullptr, *hB = nullptr, *Cn = nullptr, *thm = nullptr,
            *thn = nullptr, *Ks = nullptr, *Av = nullptr, *bv = nullptr, *dinv = nullptr;
     int last_iters[3] = {1, 1, 1};
     // timing
@@ -325,6 +343,9 @@ P* dupload(wspm_ctx* ctx, const P* src, size_t count) {
     HIPCHK(hipMemcpy(d, src, count * sizeof(P), hipMemcpyHostToDevice));
     return d;
 }
+
+// K slot of the node records: component 2 (2D) or 3 (3D) of each double4.
+double* krec_of(wspm_ctx* ctx) { return reinterpret_cast<double*>(ctx->nrec) + (ctx->dim == 2 ? 2 : 3); }
 
 // Address of a field of a device-resident KState (no host dereference).
 template <class F>
@@ -638,16 +659,23 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         ctx->cols = dupload(ctx, P.cols.data(), P.cols.size());
         ctx->diag_pos = dupload(ctx, P.diag_pos.data(), P.diag_pos.size());
         ctx->inc_ptr = dupload(ctx, P.inc_ptr.data(), P.inc_ptr.size());
-        ctx->inc_code = dupload(ctx, P.inc_code.data(), P.inc_code.size());
-        ctx->inc_slot = dupload(ctx, P.inc_slot.data(), P.inc_slot.size());
-        ctx->conn = dupload(ctx, conn, (size_t)n_elems * (dim + 1));
-        std::vector<double> c(n_nodes);
-        double* dst[3] = {nullptr, nullptr, nullptr};
-        for (int d = 0; d < dim; ++d) {
-            for (int i = 0; i < n_nodes; ++i) c[i] = coords[(size_t)i * dim + d];
-            dst[d] = dupload(ctx, c.data(), n_nodes);
+        {
+            std::vector<int2> inc(P.inc_code.size());
+            for (size_t q = 0; q < inc.size(); ++q) inc[q] = make_int2(P.inc_code[q], (int)P.inc_slot[q]);
+            ctx->inc = dupload(ctx, inc.data(), inc.size());
+            std::vector<int4> c4(n_elems);
+            for (int e = 0; e < n_elems; ++e) {
+                const int32_t* c = conn + (size_t)e * (dim + 1);
+                c4[e] = make_int4(c[0], c[1], c[2], dim == 3 ? c[3] : c[0]);
+            }
+            ctx->conn4 = dupload(ctx, c4.data(), c4.size());
+            std::vector<double4> nr(n_nodes);
+            for (int i = 0; i < n_nodes; ++i) {
+                const double* x = coords + (size_t)i * dim;
+                nr[i] = make_double4(x[0], x[1], dim == 3 ? x[2] : 0.0, 0.0);
+            }
+            ctx->nrec = dupload(ctx, nr.data(), nr.size());
         }
-        ctx->X = dst[0]; ctx->Y = dst[1]; ctx->Z = dst[2];
         ctx->bc_flag = dalloc<uint8_t>(ctx, n_nodes);
         ctx->bc_val = dalloc<double>(ctx, n_nodes);
         HIPCHK(hipMemset(ctx->bc_flag, 0, n_nodes));
@@ -730,7 +758,7 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
     API_BEGIN
     double p[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < (model == WSPM_SOIL_GARDNER ? 4 : 6); ++k) p[k] = params[k];
-    launch_vec(ctx, SoilOp{make_soil(model, p), h, C, K, theta}, 0);
+    launch_vec(ctx, SoilOp{make_soil(model, p), h, C, K, theta, nullptr}, 0);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
@@ -749,9 +777,9 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.win_cap = ctx->asm_win;
     a.inc_cap = ctx->asm_inc;
     a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
-    a.inc_code = ctx->inc_code; a.inc_slot = ctx->inc_slot; a.conn = ctx->conn;
-    a.X = ctx->X; a.Y = ctx->Y; a.Z = ctx->Z;
-    a.Kn = K; a.Ks_elem = Ks_elem; a.C = C; a.theta_m = theta_m; a.theta_n = theta_n; a.h_m = h_m;
+    a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
+    if (K) launch_vec(ctx, SetK{K, krec_of(ctx)}, 0);   // caller's K into the node records
+    a.Ks_elem = Ks_elem; a.C = C; a.theta_m = theta_m; a.theta_n = theta_n; a.h_m = h_m;
     a.inv_dt = 1.0 / dt;
     a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
     a.bc_val = ctx->bc_val;
@@ -804,6 +832,7 @@ int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const doubl
     if (!(dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
     API_BEGIN
     do_assemble(ctx, Ks_elem, h_m, C, K, theta_m, theta_n, dt, A_vals, b, dinv, nullptr, nullptr);
+    ctx->nrec_dirty = true;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
@@ -939,7 +968,7 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
     const int n = ctx->n;
     if (!ctx->hA) {
         ctx->hA = dalloc<double>(ctx, n); ctx->hB = dalloc<double>(ctx, n);
-        ctx->Cn = dalloc<double>(ctx, n); ctx->Kn = dalloc<double>(ctx, n);
+        ctx->Cn = dalloc<double>(ctx, n);
         ctx->thm = dalloc<double>(ctx, n); ctx->thn = dalloc<double>(ctx, n);
         ctx->Av = dalloc<double>(ctx, ctx->nnz); ctx->bv = dalloc<double>(ctx, n);
         ctx->dinv = dalloc<double>(ctx, n);
@@ -954,7 +983,8 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
     if (Ks_elem) p[prm->soil_model == WSPM_SOIL_GARDNER ? 3 : 5] = 1.0;  // nodal model gives Kr
     ctx->sp = make_soil(prm->soil_model, p);
     HIPCHK(hipMemcpyAsync(ctx->hA, h0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm}, 0);
+    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx)}, 0);
+    ctx->nrec_dirty = false;
     if (prm->precision == WSPM_MIXED && !ctx->A32) {
         ctx->A32 = dalloc<float>(ctx, ctx->nnz);
         ctx->dinv32 = dalloc<float>(ctx, n);
@@ -976,6 +1006,10 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const bool mixed = P.precision == WSPM_MIXED;
     const bool jac = P.precond == WSPM_PRECOND_JACOBI;
     const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
+    if (ctx->nrec_dirty) {  // a direct assembly call replaced K in the node records
+        launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx)}, 0);
+        ctx->nrec_dirty = false;
+    }
     // theta_n = theta(h_n): the soil state at the start of the step is current
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     int it = 0, kry = 0;
@@ -988,11 +1022,12 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             ensure_mixed(ctx);
             HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         }
-        do_assemble(ctx, Ks, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
+        do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
                     mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
         wspm_krylov_stats st{};
-        const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, ctx->Kn, ctx->thm, P.picard_mode == WSPM_PICARD_REL};
+        const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
+                           P.picard_mode == WSPM_PICARD_REL};
         if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
                         ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, true);

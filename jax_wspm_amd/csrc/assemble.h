@@ -37,6 +37,14 @@ struct QuadConst {
     double ml[4];
 };
 
+// Packed device layout read by the assembly (built once per mesh):
+//   inc[q]   = (element << 2 | local vertex, in-row offsets of the element's
+//              vertices, one byte each) for the q-th incidence;
+//   conn4[e] = element vertices, padded to 4;
+//   nrec[i]  = node record (x, y, K, -) in 2D, (x, y, z, K) in 3D, where K is
+//              the nodal conductivity (or Kr with a per-element Ks) written by
+//              the soil-model kernels.
+// One incidence then costs 1 + 1 + 2 (dim+1) (+1 for Ks_e) vector loads.
 struct AsmArgs {
     int n;
     int win_cap;             // LDS doubles reserved for the CSR window
@@ -45,15 +53,11 @@ struct AsmArgs {
     const int32_t* cols;
     const int32_t* diag_pos;
     const int32_t* inc_ptr;
-    const int32_t* inc_code;
-    const uint32_t* inc_slot;
-    const int32_t* conn;
-    const double* X;
-    const double* Y;
-    const double* Z;
-    const double* Kn;        // nodal K (or Kr when Ks_elem is set)
+    const int2* inc;
+    const int4* conn4;
+    const double4* nrec;
     const double* Ks_elem;   // nullable
-    const double* C;
+    const double* C;         // row-node values (read once per row)
     const double* theta_m;
     const double* theta_n;
     const double* h_m;

@@ -14,8 +14,8 @@
 //   1. every thread takes incidences of the workgroup (many independent
 //      gathers in flight), recomputes the element geometry in closed form (J is
 //      constant on a P1 element, one inverse instead of one per quadrature
-//      point) and writes that element's row contribution (dim+1 matrix values
-//      + 1 source value) to LDS;
+//      point) and writes that element's row contribution (dim+1 stiffness
+//      values, the gravity source and the lumped mass) to LDS;
 //   2. one thread per CSR entry of the rows sums, in the fixed incidence
 //      order, the contributions that land in its slot (identity rows for
 //      Dirichlet nodes), writes it to HBM (coalesced) and to an LDS copy of
@@ -31,33 +31,51 @@
 
 namespace wspm {
 
+// Row contribution of one incidence (element e seen from its vertex la):
+//   out[c] (c <= dim)  stiffness K_e(la, c) = Kbar detJ G(:,la).G(:,c)
+//   out[dim+1]         gravity source -Kbar detJ dN_la/d(last coordinate)
+//   out[dim+2]         lumped mass detJ * ml[la]
+// The mass term M_L C/dt on the diagonal and -M_L (theta_m - theta_n - C h)/dt
+// in the source are linear in the row node's values: they are applied once per
+// row with the summed lumped mass.
 template <int D>
-__device__ inline void element_row(const AsmArgs& a, int q, double (&out)[D + 2]) {
+__device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D + 3]) {
     constexpr int K = D + 1;
-    const int code = a.inc_code[q];
-    const int e = code >> 2;
-    const int la = code & 3;
-    int nd[K];
+    const int2 ic = a.inc[q];
+    const int e = ic.x >> 2;
+    const int la = ic.x & 3;
+    const int4 cn = a.conn4[e];
+    const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
+    double px[K], py[K], pz[K], kn[K];
 #pragma unroll
-    for (int c = 0; c < K; ++c) nd[c] = a.conn[(size_t)e * K + c];
+    for (int c = 0; c < K; ++c) {
+        const double4 r = a.nrec[nd[c]];
+        px[c] = r.x;
+        py[c] = r.y;
+        if constexpr (D == 2) {
+            kn[c] = r.z;
+            pz[c] = 0.0;
+        } else {
+            pz[c] = r.z;
+            kn[c] = r.w;
+        }
+    }
     double G[D][K];
     double det;
     if constexpr (D == 2) {
-        const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]];
-        const double J00 = a.X[nd[1]] - x0, J01 = a.Y[nd[1]] - y0;
-        const double J10 = a.X[nd[2]] - x0, J11 = a.Y[nd[2]] - y0;
+        const double J00 = px[1] - px[0], J01 = py[1] - py[0];
+        const double J10 = px[2] - px[0], J11 = py[2] - py[0];
         det = J00 * J11 - J01 * J10;
         const double id = 1.0 / det;
         G[0][1] = J11 * id;  G[1][1] = -J10 * id;
         G[0][2] = -J01 * id; G[1][2] = J00 * id;
     } else {
-        const double x0 = a.X[nd[0]], y0 = a.Y[nd[0]], z0 = a.Z[nd[0]];
         double r[3][3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r[c][0] = a.X[nd[c + 1]] - x0;
-            r[c][1] = a.Y[nd[c + 1]] - y0;
-            r[c][2] = a.Z[nd[c + 1]] - z0;
+            r[c][0] = px[c + 1] - px[0];
+            r[c][1] = py[c + 1] - py[0];
+            r[c][2] = pz[c + 1] - pz[0];
         }
         // J rows r1, r2, r3; J^{-1} columns = (r2 x r3, r3 x r1, r1 x r2) / det
         double cx[3][3];
@@ -85,33 +103,28 @@ __device__ inline void element_row(const AsmArgs& a, int q, double (&out)[D + 2]
     }
     double kbar = 0.0;
 #pragma unroll
-    for (int c = 0; c < K; ++c) kbar += a.qc.ck[c] * a.Kn[nd[c]];
+    for (int c = 0; c < K; ++c) kbar += a.qc.ck[c] * kn[c];
     if (a.Ks_elem != nullptr) kbar *= a.Ks_elem[e];
     const double kd = kbar * det;
     double ga[D];
     double ml = 0.0;
-    int node = 0;
 #pragma unroll
     for (int c = 0; c < K; ++c)
         if (c == la) {
             ml = a.qc.ml[c];
-            node = nd[c];
 #pragma unroll
             for (int d = 0; d < D; ++d) ga[d] = G[d][c];
         }
-    const double Ci = a.C[node];
-    const double rhs_i = a.theta_m[node] - a.theta_n[node] - Ci * a.h_m[node];
-    const double mlump = det * ml;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
         double g = 0.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) g += ga[d] * G[d][c];
-        double v = kd * g;
-        if (c == la) v += a.inv_dt * mlump * Ci;
-        out[c] = v;
+        out[c] = kd * g;
     }
-    out[K] = -kd * ga[D - 1] - a.inv_dt * mlump * rhs_i;
+    out[K] = -kd * ga[D - 1];
+    out[K + 1] = det * ml;
+    return (uint32_t)ic.y;
 }
 
 // Krylov start fused into the assembly epilogue (one row, lane 0 of its group):
@@ -153,13 +166,13 @@ __device__ inline void krylov_start(const AsmArgs& a, const KrylovInit& ki, int 
 template <int D, int R, int MODE>
 __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     constexpr int K = D + 1;
-    constexpr int NC = K + 1;                 // values per incidence in LDS
+    constexpr int NC = K + 2;                 // values per incidence in LDS
     constexpr int LANES = ASM_THREADS / R;    // threads per row in the fused Krylov start
     static_assert(LANES >= 1 && LANES <= 64 && (LANES & (LANES - 1)) == 0, "bad assembly shape");
     extern __shared__ double lds[];
     __shared__ int rp_s[R + 1], ip_s[R + 1], dg_s[R];
     __shared__ uint8_t bc_s[R];
-    __shared__ double bsh[R], dsh[R];
+    __shared__ double bsh[R], dsh[R], cs_s[R], rs_s[R];
     const int r0 = blockIdx.x * R;
     const int r1 = min(a.n, r0 + R);
     const int nrows = r1 - r0;
@@ -177,8 +190,12 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
         rp_s[j] = a.row_ptr[r0 + j] - base;
         ip_s[j] = a.inc_ptr[r0 + j] - q0;
         if (j < nrows) {
-            dg_s[j] = a.diag_pos[r0 + j] - base;
-            bc_s[j] = a.bc_flag != nullptr ? a.bc_flag[r0 + j] : 0;
+            const int i = r0 + j;
+            dg_s[j] = a.diag_pos[i] - base;
+            bc_s[j] = a.bc_flag != nullptr ? a.bc_flag[i] : 0;
+            const double ci = a.C[i];
+            cs_s[j] = ci;
+            rs_s[j] = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
         }
     }
     if constexpr (MODE != ASM_INIT_NONE)
@@ -186,11 +203,10 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
 
     // 1. element row contributions of every incidence of the workgroup
     for (int t = threadIdx.x; t < nq; t += ASM_THREADS) {
-        double out[D + 2];
-        element_row<D>(a, q0 + t, out);
+        double out[D + 3];
+        slw[t] = element_row<D>(a, q0 + t, out);
 #pragma unroll
         for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
-        slw[t] = a.inc_slot[q0 + t];
     }
     __syncthreads();
 
@@ -212,12 +228,20 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
             v = diag ? 1.0 : 0.0;
             bi = a.bc_val[r0 + j];
         } else {
+            double msum = 0.0;
             for (int t = ip_s[j]; t < ip_s[j + 1]; ++t) {
                 const uint32_t w = slw[t];
 #pragma unroll
                 for (int c = 0; c < K; ++c)
                     if (((w >> (8 * c)) & 0xffu) == sl) v += con[t * NC + c];
-                if (diag) bi += con[t * NC + K];
+                if (diag) {
+                    bi += con[t * NC + K];
+                    msum += con[t * NC + K + 1];
+                }
+            }
+            if (diag) {
+                v += a.inv_dt * msum * cs_s[j];
+                bi -= a.inv_dt * msum * rs_s[j];
             }
         }
         win[k] = v;
@@ -282,7 +306,7 @@ void compute_quad_const(int dim, QuadConst& qc) {
 int asm_rows_per_block(int dim) { return dim == 2 ? ASM_ROWS_2D : ASM_ROWS_3D; }
 
 size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc) {
-    return (2 * (size_t)max_block_nnz + (size_t)max_block_inc * (dim + 2)) * sizeof(double) +
+    return (2 * (size_t)max_block_nnz + (size_t)max_block_inc * (dim + 3)) * sizeof(double) +
            (size_t)max_block_inc * sizeof(uint32_t);
 }
 
