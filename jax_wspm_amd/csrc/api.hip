@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -281,6 +282,7 @@ struct wspm_ctx {
     int max_row = 0;
     int asm_win = 0, asm_inc = 0, asm_grid = 0;   // assembly LDS window / incidence capacity
     size_t asm_lds = 0;
+    bool asm_reg = false;                         // rows fit the register assembly kernel
     int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
     int spmv_grid = 1, vec_grid = 1;
@@ -641,9 +643,13 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
             block_maxima(P, R, ctx->asm_win, ctx->asm_inc);
             ctx->asm_lds = asm_lds_bytes(dim, ctx->asm_win, ctx->asm_inc);
             ctx->asm_grid = (n_nodes + R - 1) / R;
-            ctx->prow = std::max(MAX_BLOCKS, ctx->asm_grid);
+            // WSPM_ASM_KERNEL=lds selects the LDS-staged kernel even for short rows
+            const char* ak = std::getenv("WSPM_ASM_KERNEL");
+            ctx->asm_reg = P.max_row <= asm_reg_max_row(dim) && !(ak && std::string(ak) == "lds");
+            if (ctx->asm_reg) ctx->asm_grid = (n_nodes + BLOCK - 1) / BLOCK;
+            ctx->prow = std::max(MAX_BLOCKS, std::max(ctx->asm_grid, (n_nodes + R - 1) / R));
         }
-        if (ctx->asm_lds > 64 * 1024) {
+        if (!ctx->asm_reg && ctx->asm_lds > 64 * 1024) {
             g_create_error = "assembly LDS staging exceeds 64 KiB (rows too long or too many incident elements)";
             delete ctx;
             return WSPM_EINVAL;
@@ -795,7 +801,8 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
         ki.jacobi = dinv != nullptr;
     }
     int grid = 0;
-    HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream));
+    HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream,
+                           ctx->asm_reg));
     if (mode == ASM_INIT_NONE) return;
     KWork<double>& w = ensure_work<double>(ctx, 0);
     KVec<double> v = make_kvec<double>(ctx, w, dinv, b, h_m, x);

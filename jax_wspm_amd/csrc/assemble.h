@@ -24,6 +24,9 @@ namespace wspm {
 constexpr int ASM_ROWS_2D = WSPM_ASM_ROWS_2D;
 constexpr int ASM_ROWS_3D = WSPM_ASM_ROWS_3D;
 constexpr int ASM_THREADS = WSPM_ASM_THREADS;   // threads per assembly workgroup
+// Longest row the one-thread-per-row register kernel handles (else LDS kernel).
+constexpr int ASM_REG_ROW_2D = 8;
+constexpr int ASM_REG_ROW_3D = 16;
 
 // What the assembly epilogue also computes (the first step of the solver).
 enum { ASM_INIT_NONE = 0, ASM_INIT_BICG = 1, ASM_INIT_CG = 2, ASM_INIT_GMRES = 3, ASM_INIT_RESID = 4 };
@@ -85,7 +88,8 @@ int asm_rows_per_block(int dim);
 size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc);
 // Launch the assembly; grid_out receives the number of workgroups (partial
 // rows written when mode != ASM_INIT_NONE).
+int asm_reg_max_row(int dim);
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
-                           int prow, size_t lds, int* grid_out, hipStream_t s);
+                           int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg);
 
 }  // namespace wspm

@@ -273,6 +273,75 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     }
 }
 
+// Register variant: one thread per row, no LDS.  Used when every row has at
+// most MAXR entries (8 in 2D, 16 in 3D covers the structured meshes); each
+// contribution is added to its slot with a compile-time select, so the row
+// never leaves registers and all 256 threads of a workgroup stay busy.  Same
+// fixed per-slot summation order as k_assemble.
+template <int D, int MAXR, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+    constexpr int K = D + 1;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (i < a.n) {
+        const int rs = a.row_ptr[i];
+        const int rl = a.row_ptr[i + 1] - rs;
+        const int dg = a.diag_pos[i] - rs;
+        double v[MAXR];
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) v[s] = 0.0;
+        double bi;
+        if (a.bc_flag != nullptr && a.bc_flag[i]) {
+#pragma unroll
+            for (int s = 0; s < MAXR; ++s) v[s] = (s == dg) ? 1.0 : 0.0;
+            bi = a.bc_val[i];
+        } else {
+            bi = 0.0;
+            double msum = 0.0;
+            const int qe = a.inc_ptr[i + 1];
+            for (int q = a.inc_ptr[i]; q < qe; ++q) {
+                double out[D + 3];
+                const uint32_t w = element_row<D>(a, q, out);
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    const uint32_t sl = (w >> (8 * c)) & 0xffu;
+#pragma unroll
+                    for (int s = 0; s < MAXR; ++s) v[s] += (sl == (uint32_t)s) ? out[c] : 0.0;
+                }
+                bi += out[K];
+                msum += out[K + 1];
+            }
+            const double ci = a.C[i];
+            const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+            const double madd = a.inv_dt * msum * ci;
+#pragma unroll
+            for (int s = 0; s < MAXR; ++s) v[s] += (s == dg) ? madd : 0.0;
+            bi -= a.inv_dt * msum * rhs;
+        }
+        double dgv = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) {
+            if (s < rl) {
+                a.A[rs + s] = v[s];
+                if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+            }
+            if (s == dg) dgv = v[s];
+        }
+        a.b[i] = bi;
+        const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
+        if (a.dinv != nullptr) a.dinv[i] = di;
+        if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
+        if constexpr (MODE != ASM_INIT_NONE) {
+            double ax = 0.0;
+#pragma unroll
+            for (int s = 0; s < MAXR; ++s)
+                if (s < rl) ax += v[s] * ki.x0[a.cols[rs + s]];
+            krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
+        }
+    }
+    if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow);
+}
+
 void compute_quad_const(int dim, QuadConst& qc) {
     // Literal rules of the reference.
     const double tq[3][2] = {{0.166666666666667, 0.166666666666667},
@@ -328,8 +397,34 @@ static hipError_t launch_mode(int mode, dim3 g, size_t lds, hipStream_t s, const
     return hipGetLastError();
 }
 
+template <int D, int MAXR>
+static hipError_t launch_reg(int mode, dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki,
+                             double* partials, int prow) {
+    switch (mode) {
+        case ASM_INIT_BICG:
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_BICG>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+        case ASM_INIT_CG:
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_CG>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+        case ASM_INIT_GMRES:
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_GMRES>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+        case ASM_INIT_RESID:
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_RESID>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+        default:
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_NONE>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+    }
+    return hipGetLastError();
+}
+
+int asm_reg_max_row(int dim) { return dim == 2 ? ASM_REG_ROW_2D : ASM_REG_ROW_3D; }
+
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
-                           int prow, size_t lds, int* grid_out, hipStream_t s) {
+                           int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg) {
+    if (use_reg) {
+        const int grid = (a.n + BLOCK - 1) / BLOCK;
+        if (grid_out) *grid_out = grid;
+        if (dim == 2) return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
+        return launch_reg<3, ASM_REG_ROW_3D>(mode, dim3(grid), s, a, ki, partials, prow);
+    }
     const int R = asm_rows_per_block(dim);
     const int grid = (a.n + R - 1) / R;
     if (grid_out) *grid_out = grid;

@@ -43,12 +43,17 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("kernel", ["auto", "lds"])
 @pytest.mark.parametrize("name,mk,model,phi,use_ks", CASES, ids=[c[0] for c in CASES])
-def test_pattern_soil_assembly(name, mk, model, phi, use_ks):
+def test_pattern_soil_assembly(name, mk, model, phi, use_ks, kernel, monkeypatch):
+    """Both assembly kernels: the register one (rows that fit) and the
+    LDS-staged one (forced with WSPM_ASM_KERNEL=lds, used for long rows)."""
     import torch
     pts, el = mk()
     n = len(pts)
-    print(f"[case] {name} n={n} E={len(el)}", flush=True)
+    print(f"[case] {name} n={n} E={len(el)} kernel={kernel}", flush=True)
+    if kernel == "lds":
+        monkeypatch.setenv("WSPM_ASM_KERNEL", "lds")
     ctx = _ctx(pts, el)
     pat = oasm.Pattern(el, n)
     rp, ci = ctx.pattern()
