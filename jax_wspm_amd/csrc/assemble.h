@@ -26,7 +26,8 @@ constexpr int ASM_ROWS_3D = WSPM_ASM_ROWS_3D;
 constexpr int ASM_THREADS = WSPM_ASM_THREADS;   // threads per assembly workgroup
 // Longest row the one-thread-per-row register kernel handles (else LDS kernel).
 constexpr int ASM_REG_ROW_2D = 8;
-constexpr int ASM_REG_ROW_3D = 16;
+constexpr int ASM_REG_ROW_3D = 0;      // 3D rows (~24 incidences) use the LDS kernel
+constexpr int ASM_REG_BATCH = 3;       // incidences whose loads are issued together
 
 // What the assembly epilogue also computes (the first step of the solver).
 enum { ASM_INIT_NONE = 0, ASM_INIT_BICG = 1, ASM_INIT_CG = 2, ASM_INIT_GMRES = 3, ASM_INIT_RESID = 4 };

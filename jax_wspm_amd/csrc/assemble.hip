@@ -39,27 +39,10 @@ namespace wspm {
 // in the source are linear in the row node's values: they are applied once per
 // row with the summed lumped mass.
 template <int D>
-__device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D + 3]) {
+__device__ inline void element_compute(const AsmArgs& a, const double (&px)[D + 1], const double (&py)[D + 1],
+                                       const double (&pz)[D + 1], const double (&kn)[D + 1], double ks, int la,
+                                       double (&out)[D + 3]) {
     constexpr int K = D + 1;
-    const int2 ic = a.inc[q];
-    const int e = ic.x >> 2;
-    const int la = ic.x & 3;
-    const int4 cn = a.conn4[e];
-    const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
-    double px[K], py[K], pz[K], kn[K];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-        const double4 r = a.nrec[nd[c]];
-        px[c] = r.x;
-        py[c] = r.y;
-        if constexpr (D == 2) {
-            kn[c] = r.z;
-            pz[c] = 0.0;
-        } else {
-            pz[c] = r.z;
-            kn[c] = r.w;
-        }
-    }
     double G[D][K];
     double det;
     if constexpr (D == 2) {
@@ -96,15 +79,15 @@ __device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D 
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        double s = 0.0;
+        double sg = 0.0;
 #pragma unroll
-        for (int c = 1; c < K; ++c) s += G[d][c];
-        G[d][0] = -s;
+        for (int c = 1; c < K; ++c) sg += G[d][c];
+        G[d][0] = -sg;
     }
     double kbar = 0.0;
 #pragma unroll
     for (int c = 0; c < K; ++c) kbar += a.qc.ck[c] * kn[c];
-    if (a.Ks_elem != nullptr) kbar *= a.Ks_elem[e];
+    kbar *= ks;
     const double kd = kbar * det;
     double ga[D];
     double ml = 0.0;
@@ -124,6 +107,33 @@ __device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D 
     }
     out[K] = -kd * ga[D - 1];
     out[K + 1] = det * ml;
+}
+
+// Row contribution of one incidence (element e seen from its vertex la):
+//   out[c] (c <= dim)  stiffness K_e(la, c) = Kbar detJ G(:,la).G(:,c)
+//   out[dim+1]         gravity source -Kbar detJ dN_la/d(last coordinate)
+//   out[dim+2]         lumped mass detJ * ml[la]
+// The mass term M_L C/dt on the diagonal and -M_L (theta_m - theta_n - C h)/dt
+// in the source are linear in the row node's values: they are applied once per
+// row with the summed lumped mass.  Returns the incidence's slot word.
+template <int D>
+__device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D + 3]) {
+    constexpr int K = D + 1;
+    const int2 ic = a.inc[q];
+    const int e = ic.x >> 2;
+    const int4 cn = a.conn4[e];
+    const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
+    double px[K], py[K], pz[K], kn[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const double4 r = a.nrec[nd[c]];
+        px[c] = r.x;
+        py[c] = r.y;
+        pz[c] = (D == 3) ? r.z : 0.0;
+        kn[c] = (D == 3) ? r.w : r.z;
+    }
+    const double ks = a.Ks_elem != nullptr ? a.Ks_elem[e] : 1.0;
+    element_compute<D>(a, px, py, pz, kn, ks, ic.x & 3, out);
     return (uint32_t)ic.y;
 }
 
@@ -299,17 +309,50 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
             bi = 0.0;
             double msum = 0.0;
             const int qe = a.inc_ptr[i + 1];
-            for (int q = a.inc_ptr[i]; q < qe; ++q) {
-                double out[D + 3];
-                const uint32_t w = element_row<D>(a, q, out);
+            // NB incidences per batch: their loads are issued together (three
+            // dependent round trips per batch: incidence, element, node records)
+            for (int qb = a.inc_ptr[i]; qb < qe; qb += ASM_REG_BATCH) {
+                int2 ic[ASM_REG_BATCH];
+                int4 cn[ASM_REG_BATCH];
+                double4 nr[ASM_REG_BATCH][K];
+                double ks[ASM_REG_BATCH];
 #pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    const uint32_t sl = (w >> (8 * c)) & 0xffu;
+                for (int u = 0; u < ASM_REG_BATCH; ++u)
+                    ic[u] = (qb + u < qe) ? a.inc[qb + u] : make_int2(0, 0);
 #pragma unroll
-                    for (int s = 0; s < MAXR; ++s) v[s] += (sl == (uint32_t)s) ? out[c] : 0.0;
+                for (int u = 0; u < ASM_REG_BATCH; ++u)
+                    cn[u] = (qb + u < qe) ? a.conn4[ic[u].x >> 2] : make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < ASM_REG_BATCH; ++u) {
+                    const int nd[4] = {cn[u].x, cn[u].y, cn[u].z, cn[u].w};
+#pragma unroll
+                    for (int c = 0; c < K; ++c)
+                        nr[u][c] = (qb + u < qe) ? a.nrec[nd[c]] : make_double4(0.0, 0.0, 0.0, 0.0);
+                    ks[u] = (a.Ks_elem != nullptr && qb + u < qe) ? a.Ks_elem[ic[u].x >> 2] : 1.0;
                 }
-                bi += out[K];
-                msum += out[K + 1];
+#pragma unroll
+                for (int u = 0; u < ASM_REG_BATCH; ++u) {
+                    if (qb + u >= qe) break;
+                    double px[K], py[K], pz[K], kn[K];
+#pragma unroll
+                    for (int c = 0; c < K; ++c) {
+                        px[c] = nr[u][c].x;
+                        py[c] = nr[u][c].y;
+                        pz[c] = (D == 3) ? nr[u][c].z : 0.0;
+                        kn[c] = (D == 3) ? nr[u][c].w : nr[u][c].z;
+                    }
+                    double out[D + 3];
+                    element_compute<D>(a, px, py, pz, kn, ks[u], ic[u].x & 3, out);
+                    const uint32_t w = (uint32_t)ic[u].y;
+#pragma unroll
+                    for (int c = 0; c < K; ++c) {
+                        const uint32_t sl = (w >> (8 * c)) & 0xffu;
+#pragma unroll
+                        for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? out[c] : 0.0;
+                    }
+                    bi += out[K];
+                    msum += out[K + 1];
+                }
             }
             const double ci = a.C[i];
             const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
@@ -422,8 +465,7 @@ hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit
     if (use_reg) {
         const int grid = (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
-        if (dim == 2) return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
-        return launch_reg<3, ASM_REG_ROW_3D>(mode, dim3(grid), s, a, ki, partials, prow);
+        return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
     }
     const int R = asm_rows_per_block(dim);
     const int grid = (a.n + R - 1) / R;
