@@ -113,7 +113,7 @@ int32_t wspm_get_pattern(const wspm_ctx* ctx, int32_t* row_ptr, int32_t* cols);
  * the later value, as top-then-bottom in the reference). n_bc = 0 clears. */
 int32_t wspm_bc_set(wspm_ctx* ctx, int32_t n_bc, const int32_t* nodes, const double* values);
 
-/* Pointwise soil model over n nodes (dev pointers).  params as phi above. */
+/* Pointwise soil model over n values (any n; dev pointers).  params as phi above. */
 int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
                   const double* h, double* C, double* K, double* theta);
 
@@ -152,10 +152,13 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats);
  * of the last step. */
 int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* theta_n);
 
-/* Timing hooks for the benchmark: when enabled, the SpMV launches of the
- * Krylov solvers are bracketed by HIP events on the context's stream. */
+/* Timing hooks for the benchmark (no counterpart in the reference): when
+ * enabled, the SpMV launches of the Krylov solvers are bracketed by HIP events
+ * on the context's stream, and each launch records on the device whether it
+ * did its work or returned early (solve already converged). */
 int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
-/* Returns number of SpMV launches timed and their summed duration (ms). */
+/* Number of SpMV launches that did their work since enable/last read, and
+ * their summed event-to-event duration (ms). */
 int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus

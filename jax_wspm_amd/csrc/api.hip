@@ -318,6 +318,7 @@ struct wspm_ctx {
     int last_iters[3] = {1, 1, 1};
     // timing
     bool timing = false;
+    int* tflags = nullptr;          // [TIMING_SLOTS] launch did work (1) / was skipped (0)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<void*> allocs;
@@ -385,19 +386,22 @@ hipEvent_t next_event(wspm_ctx* ctx) {
     return ctx->ev_pool[ctx->ev_used++];
 }
 
+constexpr size_t TIMING_SLOTS = (size_t)1 << 18;
+
 template <class T, class Epi>
 void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false) {
-    const bool t = timed && ctx->timing && ctx->ev_used + 2 <= (size_t)1 << 20;
+    const bool t = timed && ctx->timing && ctx->ev_used / 2 < TIMING_SLOTS;
+    int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
     const Red red = red_of(ctx, which);
     const dim3 g(ctx->spmv_grid), b(BLOCK);
     switch (ctx->lanes) {
-        case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
-        case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
-        case 8: hipLaunchKernelGGL((k_spmv<8, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
-        case 16: hipLaunchKernelGGL((k_spmv<16, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
-        case 32: hipLaunchKernelGGL((k_spmv<32, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
-        default: hipLaunchKernelGGL((k_spmv<64, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red); break;
+        case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        case 8: hipLaunchKernelGGL((k_spmv<8, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        case 16: hipLaunchKernelGGL((k_spmv<16, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        case 32: hipLaunchKernelGGL((k_spmv<32, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        default: hipLaunchKernelGGL((k_spmv<64, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
     }
     HIPCHK(hipGetLastError());
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
@@ -760,11 +764,16 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
                   double* C, double* K, double* theta) {
     if (!ctx) return WSPM_EINVAL;
     if (model != WSPM_SOIL_GARDNER && model != WSPM_SOIL_VAN_GENUCHTEN) return fail(ctx, WSPM_EINVAL, "unknown soil model");
-    if (!params || !h || !C || !K || !theta || n != ctx->n) return fail(ctx, WSPM_EINVAL, "bad soil arguments");
+    if (!params || !h || !C || !K || !theta || n < 0) return fail(ctx, WSPM_EINVAL, "bad soil arguments");
     API_BEGIN
     double p[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < (model == WSPM_SOIL_GARDNER ? 4 : 6); ++k) p[k] = params[k];
-    launch_vec(ctx, SoilOp{make_soil(model, p), h, C, K, theta, nullptr}, 0);
+    if (n > 0) {
+        const SoilOp op{make_soil(model, p), h, C, K, theta, nullptr};
+        const int grid = std::min((n + BLOCK - 1) / BLOCK, 1024);
+        hipLaunchKernelGGL(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0));
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
@@ -1093,6 +1102,8 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on) {
     if (!ctx) return WSPM_EINVAL;
     API_BEGIN
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (on && !ctx->tflags) ctx->tflags = dalloc<int>(ctx, TIMING_SLOTS);
+    if (on) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
     ctx->timing = on != 0;
     ctx->ev_used = 0;
     return WSPM_OK;
@@ -1103,15 +1114,22 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms) {
     if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
     API_BEGIN
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t pairs = ctx->ev_used / 2;
+    std::vector<int> flags(pairs, 0);
+    if (pairs) HIPCHK(hipMemcpy(flags.data(), ctx->tflags, pairs * sizeof(int), hipMemcpyDeviceToHost));
     double tot = 0.0;
-    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
+    int64_t worked = 0;
+    for (size_t k = 0; k < pairs; ++k) {
+        if (!flags[k]) continue;
         float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[2 * k], ctx->ev_pool[2 * k + 1]));
         tot += ms;
+        ++worked;
     }
-    *launches = (int64_t)(ctx->ev_used / 2);
+    *launches = worked;
     *total_ms = tot;
     ctx->ev_used = 0;
+    if (ctx->tflags) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
     return WSPM_OK;
     API_END
 }

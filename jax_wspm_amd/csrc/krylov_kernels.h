@@ -32,11 +32,14 @@ struct KVec {
 // CSR SpMV: LANES consecutive lanes own one row (LANES | 64), the row sum is
 // finished with a width-LANES butterfly, then lane 0 runs the epilogue.
 // ---------------------------------------------------------------------------
+// tflag (nullable): the benchmark's launch-timing slot, set when the launch did
+// its work (not skipped), so timed averages count working launches only.
 template <int LANES, class T, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ ci,
-                                                 const T* __restrict__ av, Epi epi, Red red) {
+                                                 const T* __restrict__ av, Epi epi, Red red, int* tflag) {
     if (epi.skip(red.st)) return;
+    if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = 1;
     constexpr int GPB = BLOCK / LANES;
     constexpr int ND = Epi::ND;
     const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;

@@ -95,7 +95,8 @@ class Context:
         pa = np.asarray(params, dtype=np.float64)
         p[: pa.size] = pa
         Cc, K, th = (torch.empty_like(h) for _ in range(3))
-        self._ck(self.lib.wspm_soil(self.h, _lib.SOIL[model], ptr(p), self.n, ptr(h), ptr(Cc), ptr(K), ptr(th)))
+        self._ck(self.lib.wspm_soil(self.h, _lib.SOIL[model], ptr(p), int(h.numel()), ptr(h), ptr(Cc), ptr(K),
+                                    ptr(th)))
         return Cc, K, th
 
     def assemble_richards(self, h_m, C_m, K_m, theta_m, theta_n, dt, Ks_elem=None, want_dinv=True):

@@ -145,3 +145,29 @@ def test_loader_validation(tmp_path):
     np.savetxt(tmp_path / "richards" / "t3.csv", np.array([[1, 2, 3]]), delimiter=",", fmt="%d")
     p, t, info = M.load_and_validate_mesh("3", str(tmp_path), "Test1")
     assert t.tolist() == [[0, 1, 2]] and info["num_points"] == 3
+
+
+def test_product_tracy_matches_oracle():
+    from jax_wspm_amd import tracy as pt
+    from oracle import tracy as ot
+    p, _ = omesh.structured_tri(25, 15.24)
+    eps0, d = np.exp(0.164 * -15.24), 0.164 * 0.3 / 0.1
+    for tn in (1, 2, 3):
+        for t in (0.02, 0.19, 0.7):
+            a = ot.exact_solution(p[:, 0], p[:, 1], t, tn, 15.24, 0.164, eps0, d)
+            b = pt.exact_solution(p[:, 0], p[:, 1], t, tn, 15.24, 0.164, eps0, d)
+            assert np.allclose(a[0], b[0], rtol=1e-13, atol=1e-12) and np.allclose(a[1], b[1], rtol=1e-13)
+
+
+def test_cli_flags_and_defaults_match_reference():
+    from jax_wspm_amd.cli import build_config, parse_args
+    a = parse_args([])
+    # src/cli.py:59-82 defaults
+    assert (a.mesh_size, a.test_case, a.solver, a.preconditioner, a.dt, a.tmax, a.output_dir,
+            a.save_frequency) == ("25", "Test1", "gmres", "none", 1e-6, 1.0, "results", 5)
+    cfg = build_config(parse_args(["--test-case", "Test3D", "--solver", "bicgstab", "--dt", "1e-4",
+                                   "--tmax", "0.1"]))
+    d = cfg.to_dict()
+    assert d["solver"]["solver_type"] == "bicgstab" and d["time"]["dt_init"] == 1e-4 and d["test_case"] == "Test3D"
+    with pytest.raises(SystemExit):
+        parse_args(["--solver", "lu"])

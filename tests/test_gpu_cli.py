@@ -1,0 +1,61 @@
+"""GPU: the CLI drop-in and the solve_system / assembly mirrors."""
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cli_test1_published_rule_reproduces_csv(tmp_path, golden):
+    """python -m jax_wspm_amd.cli --test-case Test1 --solver cg --dt 1e-3 --tmax 0.25 --picard-rule published
+    writes the reference's outputs, whose metrics equal resultsRE2D/benchmark_results_cpu_Test1_WA.csv:2."""
+    from jax_wspm_amd.cli import parse_args, run
+    print("[case] cli Test1", flush=True)
+    args = parse_args(["--test-case", "Test1", "--solver", "cg", "--preconditioner", "none", "--dt", "1e-3",
+                       "--tmax", "0.25", "--picard-rule", "published", "--output-dir", str(tmp_path)])
+    run(args)
+    cfg = json.loads((tmp_path / "simulation_config.json").read_text())
+    assert cfg["test_case"] == "Test1" and cfg["solver"]["solver_type"] == "cg"
+    z = np.load(tmp_path / "numerical_results_Test1.npz")
+    for k in ("pressure_head", "theta", "times", "points", "iterations", "errors", "dt_values", "simulation_time",
+              "memory_usage", "elements", "l2_pressure", "linf_pressure", "l2_saturation", "linf_saturation",
+              "l2_relative_pressure", "l2_relative_saturation"):
+        assert k in z.files, k
+    row = next(r for r in json.loads((golden / "tracy_metrics.json").read_text())
+               if r["test_case"] == "Test1" and r["mesh_size"] == "25" and r["solver"] == "cg"
+               and r["preconditioner"] == "none")
+    assert int(z["iterations"].sum()) == int(row["iterations"])
+    assert f"{float(z['l2_pressure']):.4e}" == row["l2_pressure"]
+    assert f"{float(z['l2_saturation']):.4e}" == row["l2_saturation"]
+
+
+def test_solve_system_mirror_matches_oracle():
+    import torch
+    from jax_wspm_amd.config import SolverParameters
+    from jax_wspm_amd.device import Context
+    from jax_wspm_amd.linear import assemble_global_matrices_sparse_re, solve_system, van_genuchten_model
+    from oracle import assembly as oasm, krylov as okr, mesh as omesh, soil as osoil
+    pts, el = omesh.structured_tri(20, 1.0)
+    n = len(pts)
+    vg = (0.1, 0.40, 0.06, 2.0, 0.5, 1e-3)
+    h = np.random.default_rng(0).uniform(-900.0, -80.0, n)
+    ctx = Context(pts, el)
+    top = np.nonzero(pts[:, 1] == 1.0)[0]
+    ctx.set_dirichlet([(top, np.full(top.size, -75.0))])
+    C, K, th = van_genuchten_model(ctx, torch.as_tensor(h, device="cuda"), vg)
+    A, b = assemble_global_matrices_sparse_re(ctx, th, th, torch.as_tensor(h, device="cuda"), K, C, 1e-3)
+    for solver in ("gmres", "bicgstab", "cg"):
+        x, conv = solve_system(A, b, torch.as_tensor(h, device="cuda"),
+                               SolverParameters(solver_type=solver, precond_type="jacobi"))
+        pat = oasm.Pattern(el, n)
+        Co, Ko, tho = osoil.van_genuchten(h, vg)
+        vals, bo = oasm.assemble(pat, pts, tho, tho, h, Ko, Co, 1e-3, [(top, np.full(top.size, -75.0))])
+        xo, _ = okr.solve(solver, pat.to_scipy(vals).dot, bo, h, oasm.jacobi_inverse(pat, vals))
+        assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-10
+        assert (conv == 0) if solver == "gmres" else conv is None
+    with pytest.raises(ValueError):
+        solve_system(A, b, b, SolverParameters(solver_type="lu"))
+    with pytest.raises(NotImplementedError):
+        solve_system(A, b, b, SolverParameters(solver_type="cg", precond_type="ilu"))
+    ctx.close()
