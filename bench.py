@@ -14,6 +14,13 @@ HIP events on the solver's stream; their mean duration gives the roofline
 line.  The CPU baseline runs the oracle (NumPy/SciPy FP64 restatement of the
 reference, one thread) on the first steps of the same run.
 
+Workload c3: the batched ensemble -- 256 independent 2D instances, side N
+drawn from {32, 48, ..., 256}, per-instance van Genuchten parameters
+(jax_wspm_amd.ensemble.vg_ensemble, seed 0), 100 steps of dt 1e-4,
+Jacobi-BiCGSTAB FP64, all instances in one ensemble context.  value =
+instance time steps/s (instances x steps / s); the plain single-instance GPU
+path is timed on every instance for comparison.
+
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--workload c1]
 """
 from __future__ import annotations
@@ -39,18 +46,27 @@ WORKLOADS = {
                  64, 3, "cg", "mixed", None, 1e-4),
     "c1-mixed": ("2D Richards 512x512, heterogeneous Ks, Jacobi-BiCGSTAB, FP32 inner / FP64 outer",
                  512, 2, "bicgstab", "mixed", {"sigma": 0.5, "length": 0.05, "seed": 0}, 1e-4),
+    "c3": ("batched ensemble: 256 independent 2D van Genuchten instances, side N in {32,48,...,256}, "
+           "per-instance alpha, n, Ks, theta_r, theta_s, fixed dt 1e-4, Jacobi-BiCGSTAB, FP64",
+           None, 2, "bicgstab", "fp64", None, 1e-4),
 }
+DEFAULT_STEPS = {"c3": 100}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; c3: 100)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not bracket SpMV launches with events")
-    return ap.parse_args()
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="c3: steps per instance of the plain single-instance path (0 = skip)")
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS.get(args.workload, 200)
+    return args
 
 
 def spmv_bytes(n, nnz, value_bytes=8):
@@ -84,12 +100,132 @@ def cpu_baseline(pb, steps):
     return steps / el, el, h
 
 
+def roofline(n, nnz, launches, spmv_ms, rows):
+    """Mean HBM rate of the timed SpMV launches: algorithmic bytes of the rows
+    each launch worked on (the full operator's bytes per row) over their
+    summed duration."""
+    if not launches:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None}
+    full = spmv_bytes(n, nnz, 8)
+    moved = full * (rows / n)
+    achieved = moved / (spmv_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_spmv (CSR SpMV, fused Krylov epilogues)", "bytes_per_launch": round(moved / launches),
+            "full_operator_bytes": full, "launches": launches,
+            "avg_launch_us": round(spmv_ms / launches * 1e3, 3)}
+
+
+def run_ensemble(args):
+    """c3: the whole ensemble in one context; the plain single-instance path and
+    the oracle are timed on the same instances for comparison."""
+    import numpy as np
+    import torch
+
+    from jax_wspm_amd.device import Context
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+
+    desc, _, _, solver, precision, _, dt = WORKLOADS[args.workload]
+    insts = vg_ensemble(256, seed=0)
+    ens = Ensemble(insts)
+    ens.richards_init("vg", solver, "jacobi")
+    print(f"[bench] c3: instances={ens.n_inst} n={ens.n} E={ens.n_elems} nnz={ens.nnz} lanes={ens.lanes}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ens.step(dt)
+    torch.cuda.synchronize()
+    ens.timing(not args.no_timing)
+    picard = kry = 0
+    conv = True
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = ens.step(dt)
+        picard += st["picard_iters"]
+        kry += st["krylov_iters"]
+        conv &= st["converged"]
+        if (k + 1) % 25 == 0:
+            print(f"[bench] c3 step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    launches, spmv_ms, rows = ens.timing_read()
+    ens.timing(False)
+    h = ens.state()[0]
+    assert bool(torch.isfinite(h).all()), "non-finite pressure head"
+    value = ens.n_inst * args.steps / el
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "time steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (structured meshes, seeded per-instance soil parameters generated in-process)",
+        "config": {"workload": desc, "instances": ens.n_inst, "nodes": ens.n, "elements": ens.n_elems,
+                   "nnz": ens.nnz, "dt": dt, "solver": f"{solver}+jacobi", "precision": precision,
+                   "value_counts": "instance time steps (each timed step advances every instance once)",
+                   "ensemble_steps_per_s": round(args.steps / el, 3),
+                   "max_picard_iters_per_step": round(picard / args.steps, 3),
+                   "krylov_iters_per_step_summed": round(kry / args.steps, 1),
+                   "all_converged": bool(conv), "spmv_lanes": ens.lanes},
+        "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows),
+    }
+    ens.close()
+    del ens
+    if args.plain_steps > 0:
+        # the plain FP64 single-instance path on every instance (one context each)
+        tot = 0.0
+        for i, inst in enumerate(insts):
+            ctx = Context(inst.points, inst.elements)
+            ctx.set_dirichlet(inst.bc_lists)
+            ctx.richards_init("vg", inst.phi, solver, "jacobi", inst.h0)
+            ctx.richards_step(dt)            # warm-up step (first Krylov chunk sizing)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.plain_steps):
+                ctx.richards_step(dt)
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t1
+            ctx.close()
+            if (i + 1) % 64 == 0:
+                print(f"[bench] c3 plain path {i + 1}/256", file=sys.stderr, flush=True)
+        plain = len(insts) * args.plain_steps / tot
+        out["config"]["plain_fp64_single_instance_steps_per_s"] = round(plain, 2)
+        out["config"]["speedup_vs_plain_fp64"] = round(value / plain, 2)
+        out["config"]["plain_sample"] = (f"steps 2..{1 + args.plain_steps} of each of the 256 instances, "
+                                         "one context per instance, run one after another")
+    if args.cpu_steps > 0:
+        from oracle import picard as opicard
+        try:
+            from threadpoolctl import threadpool_limits
+            threadpool_limits(1)
+        except Exception:  # pragma: no cover
+            pass
+        sample = insts[:8]
+        t1 = time.perf_counter()
+        for inst in sample:
+            opb = opicard.Problem(points=inst.points, elems=inst.elements, soil_model="vg", phi=inst.phi,
+                                  bc_lists=inst.bc_lists, h0=inst.h0, dt0=dt, nsteps=args.cpu_steps,
+                                  adaptive=False, solver=solver, precond="jacobi")
+            stp = opicard.Stepper(opb)
+            hh = inst.h0.astype(np.float64).copy()
+            for _ in range(args.cpu_steps):
+                hh, _, _, _ = stp.step(hh, dt)
+        cel = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(len(sample) * args.cpu_steps / cel, 4), "unit": "time steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"first {args.cpu_steps} steps of instances 0-7 "
+                                         f"(sides {[i.cells for i in sample]}), {cel:.1f} s, NumPy/SciPy FP64 "
+                                         f"oracle, 1 thread; host has {os.cpu_count()} logical CPUs"}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": f"this build runs on one GPU; --gpus {args.gpus} is not supported "
                                    "(the ensemble/mesh sharding design is described in DESIGN.md)"}))
         sys.exit(2)
+    if args.workload == "c3":
+        run_ensemble(args)
+        return
     import numpy as np
     import torch
 
@@ -118,17 +254,11 @@ def main():
         kry += st["krylov_iters"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    launches, spmv_ms = ctx.timing_read()
+    launches, spmv_ms, rows = ctx.timing_read()
     ctx.timing(False)
     h, th, _ = ctx.richards_state()
     assert bool(torch.isfinite(h).all()), "non-finite pressure head"
     value = args.steps / el
-    vbytes = 4 if precision == "mixed" else 8
-    # in mixed mode most SpMVs are FP32 (inner) -- the roofline line uses the
-    # FP64 byte count, which overstates nothing for FP64 and understates FP32
-    per_launch = spmv_bytes(ctx.n, ctx.nnz, 8)
-    avg_ms = spmv_ms / launches if launches else float("nan")
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -146,11 +276,9 @@ def main():
                    "solver": f"{solver}+jacobi", "precision": precision,
                    "picard_iters_per_step": round(picard / args.steps, 3),
                    "krylov_iters_per_step": round(kry / args.steps, 3)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "k_spmv (CSR SpMV, fused Krylov epilogues)",
-                     "bytes_per_launch": per_launch, "launches": launches,
-                     "avg_launch_us": round(avg_ms * 1e3, 3) if launches else None},
+        # in mixed mode most SpMVs are FP32 (inner): the FP64 byte count used
+        # here overstates their traffic, so the line is an upper bound there
+        "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows),
     }
     if args.cpu_steps > 0:
         pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,

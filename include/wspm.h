@@ -30,6 +30,10 @@
  *   wspm_richards_*        src/solvers/richards_solver_v2.py:107-168 and
  *                          src/solvers/richards_solver_3D_v2.py:177-222
  *                          (Picard loop of one implicit time step)
+ *   wspm_ens_*             no single counterpart: the reference runs one
+ *                          instance per process (src/cli.py:87-150); an
+ *                          ensemble context time-steps many independent
+ *                          instances of wspm_richards_* at once.
  */
 #ifndef WSPM_H
 #define WSPM_H
@@ -157,9 +161,39 @@ int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* the
  * on the context's stream, and each launch records on the device whether it
  * did its work or returned early (solve already converged). */
 int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
-/* Number of SpMV launches that did their work since enable/last read, and
- * their summed event-to-event duration (ms). */
-int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms);
+/* Number of SpMV launches that did their work since enable/last read, their
+ * summed event-to-event duration (ms) and the rows they worked on (rows may
+ * be NULL; an ensemble launch skips the rows of converged instances). */
+int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
+
+/* Ensembles ------------------------------------------------------------------
+ * n_inst independent 2D instances (own mesh, soil parameters, Dirichlet rows)
+ * in one context.  Nodes and elements are numbered instance after instance:
+ * instance i owns global nodes [sum(node_counts[:i]), +node_counts[i]) and
+ * elements likewise.  conn (host int32 [sum(elem_counts) * 3]) holds each
+ * instance's connectivity in its own LOCAL numbering; coords (host double
+ * [sum(node_counts) * 2]) are concatenated.  Every instance converges on its
+ * own: its Krylov solve and Picard loop stop independently, and its
+ * reductions cover its own rows only.  On an ensemble context use
+ * wspm_bc_set / wspm_spmv / wspm_soil with global numbering, and
+ * wspm_richards_step / wspm_richards_state (concatenated arrays);
+ * wspm_assemble_richards, wspm_krylov, wspm_krylov_mixed and
+ * wspm_richards_init return WSPM_EINVAL.  The SpMV row width is chosen at
+ * creation by timing the candidates (info[6]). */
+int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
+                        const int32_t* elem_counts, const int32_t* conn,
+                        const double* coords, void* stream, wspm_ctx** out);
+/* prm: shared settings (CG or BiCGSTAB, FP64); phi: host double [n_inst * 6]
+ * per-instance soil parameters (layout of wspm_richards_params.phi), or NULL
+ * for prm->phi everywhere.  Ks_elem dev [total elements] or NULL; h0 dev
+ * [total nodes]. */
+int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm,
+                               const double* phi, const double* Ks_elem, const double* h0);
+/* Per-instance results of the last wspm_richards_step (host arrays [n_inst],
+ * each may be NULL).  The step's own wspm_step_stats aggregates them: max
+ * Picard iterations, summed Krylov iterations, all converged, max error. */
+int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylov_iters,
+                            int32_t* converged, double* err);
 
 #ifdef __cplusplus
 }

@@ -68,7 +68,10 @@ PROTOTYPES = {
     "wspm_richards_step": (_I, [_P, _D, C.POINTER(StepStats)]),
     "wspm_richards_state": (_I, [_P, _P, _P, _P]),
     "wspm_timing_enable": (_I, [_P, _I]),
-    "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
+    "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),
 }
 
 _lib = None

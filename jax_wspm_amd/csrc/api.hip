@@ -41,14 +41,18 @@ struct Err {};
 // Soil / Picard kernels
 // ---------------------------------------------------------------------------
 // K goes to K[i] and/or to the node record (krec[4 i], stride of double4).
+// ens: ensemble mode, the instance's soil parameters come from its state.
 struct SoilOp {
     static constexpr int ND = 0;
     SoilParams sp;
     const double* h;
     double *C, *K, *th;
     double* krec;
+    int ens;
     __device__ bool skip(const KState*) const { return false; }
-    __device__ void load(const KState*) {}
+    __device__ void load(const KState* s) {
+        if (ens) sp = s->sp;
+    }
     __device__ void row(int i, double*) const {
         double c, k, t;
         soil_eval(sp, h[i], c, k, t);
@@ -61,6 +65,10 @@ struct SoilOp {
 
 // |h - h_m|, |h| and the soil model at the new iterate (used by the next
 // Picard iteration, and as theta(h) once the step has converged).
+// Ensemble mode (ens): an instance is processed once per Picard iteration,
+// after its Krylov solve is done, with its own soil parameters; the finish
+// counts its Picard/Krylov iterations and marks it converged (or capped), as
+// the loop condition of src/solvers/richards_solver_v2.py:141-168.
 struct PicardOp {
     static constexpr int ND = 2;
     SoilParams sp;
@@ -69,8 +77,11 @@ struct PicardOp {
     double *C, *K, *th;
     double* krec;
     int rel;
-    __device__ bool skip(const KState*) const { return false; }
-    __device__ void load(const KState*) {}
+    int ens;
+    __device__ bool skip(const KState* s) const { return ens && (s->pdone || !s->done || s->pev); }
+    __device__ void load(const KState* s) {
+        if (ens) sp = s->sp;
+    }
     __device__ void row(int i, double* acc) const {
         const double hi = h[i];
         const double d = hi - hm[i];
@@ -86,8 +97,63 @@ struct PicardOp {
         s->picard_d2 = t[0];
         s->picard_h2 = t[1];
         s->picard_err = rel ? sqrt(t[0]) / sqrt(t[1]) : sqrt(t[0]);
+        if (ens) {
+            s->pev = 1;
+            s->pit += 1;
+            s->kry += s->steps;
+            s->pdone = !(s->picard_err >= s->picard_tol && s->pit < s->picard_maxit);
+        }
     }
 };
+
+// Ensemble bookkeeping --------------------------------------------------------
+// Start of a time step: every instance iterates again.
+__global__ void k_ens_step_begin(KState* st, int n_inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        st[i].pdone = 0;
+        st[i].pev = 0;
+        st[i].pit = 0;
+        st[i].kry = 0;
+    }
+}
+
+// out[0] = instances whose Krylov solve is still running, out[1] = instances
+// still iterating Picard, out[2] = most Krylov steps of a solve.  One workgroup.
+__global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__ st, int n_inst, int* out) {
+    __shared__ int cnt[3];
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int kact = 0, pact = 0, ms = 0;
+    for (int i = threadIdx.x; i < n_inst; i += BLOCK) {
+        if (!st[i].pdone) {
+            ++pact;
+            if (!st[i].done) ++kact;
+        }
+        ms = max(ms, st[i].steps);
+    }
+    atomicAdd(&cnt[0], kact);
+    atomicAdd(&cnt[1], pact);
+    atomicMax(&cnt[2], ms);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = cnt[0];
+        out[1] = cnt[1];
+        out[2] = cnt[2];
+        out[3] = 0;
+    }
+}
+
+// Per-instance statistics of the last time step.
+__global__ void k_ens_collect(const KState* __restrict__ st, int n_inst, int* ip, int* ik, int* ic, double* er) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        ip[i] = st[i].pit;
+        ik[i] = st[i].kry;
+        ic[i] = st[i].picard_err < st[i].picard_tol;
+        er[i] = st[i].picard_err;
+    }
+}
 
 // Mixed precision: FP64 residual r = b - A x, |r|, |b|
 struct RefResid {
@@ -318,9 +384,17 @@ struct wspm_ctx {
     int last_iters[3] = {1, 1, 1};
     // timing
     bool timing = false;
-    int* tflags = nullptr;          // [TIMING_SLOTS] launch did work (1) / was skipped (0)
+    int* tflags = nullptr;          // [TIMING_SLOTS] rows each timed launch worked on (0: skipped)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    // ensemble (tab != nullptr): instances concatenated, one state per instance
+    int n_inst = 1;
+    RowTab* tab = nullptr;          // [n_tab] rows and instance of each workgroup
+    int n_tab = 0;
+    int* iblk = nullptr;            // [n_inst + 1] first workgroup of each instance
+    int* ens_status = nullptr;      // device int[4] (k_ens_status)
+    int* hstatus = nullptr;         // pinned mirror
+    int* ens_out = nullptr;         // k_ens_collect output: 3 int + 1 double per instance
     std::vector<void*> allocs;
     std::vector<void*> host_allocs;
 };
@@ -356,25 +430,35 @@ F* state_field(KState* ds, size_t off) {
     return reinterpret_cast<F*>(reinterpret_cast<char*>(ds) + off);
 }
 
+// State array `which` (0 main, 1 inner, 2 refinement): one KState per instance.
+KState* states_of(wspm_ctx* ctx, int which) { return ctx->states + (size_t)which * ctx->n_inst; }
+
 Red red_of(wspm_ctx* ctx, int which) {
-    return Red{ctx->partials, ctx->prow, ctx->states + which};
+    return Red{ctx->partials, ctx->prow, states_of(ctx, which)};
 }
 
-template <class Op>
+// Finish of a reduction kernel: one workgroup, or one per instance (ensemble).
+template <class Op, bool RESET_PEV = false>
 void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
     if constexpr (Op::ND > 0) {
-        hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials,
-                           ctx->prow, nblk, ctx->states + which);
+        if (ctx->tab) {
+            hipLaunchKernelGGL((k_finish_ens<Op::ND, Op, RESET_PEV>), dim3(ctx->n_inst), dim3(BLOCK), 0,
+                               ctx->stream, op, ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which));
+        } else {
+            hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials,
+                               ctx->prow, nblk, states_of(ctx, which));
+        }
         HIPCHK(hipGetLastError());
     }
 }
 
 template <class Op>
 void launch_vec(wspm_ctx* ctx, const Op& op, int which) {
-    hipLaunchKernelGGL(k_vec<Op>, dim3(ctx->vec_grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op,
-                       red_of(ctx, which));
+    const int grid = ctx->tab ? ctx->n_tab : ctx->vec_grid;
+    hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
+                       (const RowTab*)ctx->tab);
     HIPCHK(hipGetLastError());
-    launch_finish(ctx, op, ctx->vec_grid, which);
+    launch_finish(ctx, op, grid, which);
 }
 
 hipEvent_t next_event(wspm_ctx* ctx) {
@@ -394,23 +478,34 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
     const Red red = red_of(ctx, which);
-    const dim3 g(ctx->spmv_grid), b(BLOCK);
+    const int grid = ctx->tab ? ctx->n_tab : ctx->spmv_grid;
+    const dim3 g(grid), b(BLOCK);
+    const RowTab* tab = ctx->tab;
     switch (ctx->lanes) {
-        case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
-        case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
-        case 8: hipLaunchKernelGGL((k_spmv<8, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
-        case 16: hipLaunchKernelGGL((k_spmv<16, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
-        case 32: hipLaunchKernelGGL((k_spmv<32, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
-        default: hipLaunchKernelGGL((k_spmv<64, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag); break;
+        case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
+        case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
+        case 8: hipLaunchKernelGGL((k_spmv<8, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
+        case 16: hipLaunchKernelGGL((k_spmv<16, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
+        case 32: hipLaunchKernelGGL((k_spmv<32, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
+        default: hipLaunchKernelGGL((k_spmv<64, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
     }
     HIPCHK(hipGetLastError());
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
-    launch_finish(ctx, epi, ctx->spmv_grid, which);
+    launch_finish(ctx, epi, grid, which);
 }
 
 void poll(wspm_ctx* ctx, int which) {
-    HIPCHK(hipMemcpyAsync(ctx->hstate + which, ctx->states + which, sizeof(KState),
+    HIPCHK(hipMemcpyAsync(ctx->hstate + which, states_of(ctx, which), sizeof(KState),
                           hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+// Ensemble: counts of running instances (k_ens_status) read back.
+void poll_ens(wspm_ctx* ctx, int which) {
+    hipLaunchKernelGGL(k_ens_status, dim3(1), dim3(BLOCK), 0, ctx->stream, states_of(ctx, which), ctx->n_inst,
+                       ctx->ens_status);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->hstatus, ctx->ens_status, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -476,7 +571,7 @@ struct KrylovRun {
               double atol, int maxit, int restart, int wh, bool started)
         : ctx(c), method(m), A(A_), dinv(dinv_), which(wh), maxiter(maxit) {
         const int n = ctx->n;
-        ds = ctx->states + which;
+        ds = states_of(ctx, which);
         if (method == WSPM_GMRES) {
             R = std::min(std::min(restart, n), MAX_RESTART);
             KWork<T>& w = ensure_work<T>(ctx, R + 1);
@@ -769,9 +864,10 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
     double p[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < (model == WSPM_SOIL_GARDNER ? 4 : 6); ++k) p[k] = params[k];
     if (n > 0) {
-        const SoilOp op{make_soil(model, p), h, C, K, theta, nullptr};
+        const SoilOp op{make_soil(model, p), h, C, K, theta, nullptr, 0};
         const int grid = std::min((n + BLOCK - 1) / BLOCK, 1024);
-        hipLaunchKernelGGL(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0));
+        hipLaunchKernelGGL(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0),
+                           (const RowTab*)nullptr);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -800,6 +896,9 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.qc = ctx->qc;
+    a.tab = ctx->tab;
+    a.ens_st = states_of(ctx, 0);
+    a.n_tab = ctx->n_tab;
     KrylovInit ki{};
     if (mode != ASM_INIT_NONE) {
         KWork<double>& w = ensure_work<double>(ctx, 0);
@@ -815,29 +914,24 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     if (mode == ASM_INIT_NONE) return;
     KWork<double>& w = ensure_work<double>(ctx, 0);
     KVec<double> v = make_kvec<double>(ctx, w, dinv, b, h_m, x);
-    const dim3 g1(1), b1(BLOCK);
-    KState* st0 = ctx->states;
+    // the assembly leaves 3 partial columns per workgroup; each finish reads
+    // the ones its functor uses (column sums are independent)
     switch (mode) {
         case ASM_INIT_BICG:
-            hipLaunchKernelGGL((k_finish<3, BicgInit<double>>), g1, b1, 0, ctx->stream,
-                               BicgInit<double>{v, tol, atol, maxiter}, ctx->partials, ctx->prow, grid, st0);
+            launch_finish<BicgInit<double>, true>(ctx, BicgInit<double>{v, tol, atol, maxiter}, grid, 0);
             break;
         case ASM_INIT_CG:
-            hipLaunchKernelGGL((k_finish<3, CgInit<double>>), g1, b1, 0, ctx->stream,
-                               CgInit<double>{v, tol, atol, maxiter}, ctx->partials, ctx->prow, grid, st0);
+            launch_finish<CgInit<double>, true>(ctx, CgInit<double>{v, tol, atol, maxiter}, grid, 0);
             break;
         case ASM_INIT_GMRES: {
             const int R = std::min(std::min(restart, ctx->n), MAX_RESTART);
-            hipLaunchKernelGGL((k_finish<3, GmInit<double>>), g1, b1, 0, ctx->stream,
-                               GmInit<double>{v, tol, atol, maxiter, R}, ctx->partials, ctx->prow, grid, st0);
+            launch_finish<GmInit<double>, true>(ctx, GmInit<double>{v, tol, atol, maxiter, R}, grid, 0);
             break;
         }
         default:
-            hipLaunchKernelGGL((k_finish<3, RefResid>), g1, b1, 0, ctx->stream, RefResid{b, x, ctx->ref_r},
-                               ctx->partials, ctx->prow, grid, ctx->states + 2);
+            launch_finish<RefResid, true>(ctx, RefResid{b, x, ctx->ref_r}, grid, 2);
             break;
     }
-    HIPCHK(hipGetLastError());
 }
 
 int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
@@ -845,6 +939,7 @@ int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const doubl
                                double* A_vals, double* b, double* dinv) {
     if (!ctx) return WSPM_EINVAL;
     if (!h_m || !C || !K || !theta_m || !theta_n || !A_vals || !b) return fail(ctx, WSPM_EINVAL, "null array");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
     if (!(dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
     API_BEGIN
     do_assemble(ctx, Ks_elem, h_m, C, K, theta_m, theta_n, dt, A_vals, b, dinv, nullptr, nullptr);
@@ -878,6 +973,7 @@ int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const d
                     wspm_krylov_stats* stats) {
     if (!ctx) return WSPM_EINVAL;
     if (!A_vals || !b || !x0 || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
     int rc = check_krylov_args(ctx, method, maxiter, restart);
     if (rc) return rc;
     API_BEGIN
@@ -941,6 +1037,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
                           int32_t restart, double inner_tol, int32_t max_refine, wspm_krylov_stats* stats) {
     if (!ctx) return WSPM_EINVAL;
     if (!A_vals || !b || !x0 || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
     int rc = check_krylov_args(ctx, method, maxiter, restart);
     if (rc) return rc;
     if (max_refine < 0) return fail(ctx, WSPM_EINVAL, "max_refine must be >= 0");
@@ -953,7 +1050,8 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
     {
         ToF32 op{A_vals, ctx->A32};
         const int g = (int)std::min<int64_t>((ctx->nnz + BLOCK - 1) / BLOCK, 4096);
-        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2));  // ND = 0
+        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2),
+                           (const RowTab*)nullptr);  // ND = 0
         HIPCHK(hipGetLastError());
         if (dinv) launch_vec(ctx, ToF32{dinv, ctx->dinv32}, 2);
     }
@@ -967,10 +1065,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
     API_END
 }
 
-int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const double* Ks_elem,
-                           const double* h0) {
-    if (!ctx) return WSPM_EINVAL;
-    if (!prm || !h0) return fail(ctx, WSPM_EINVAL, "null argument");
+static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm) {
     if (prm->soil_model != WSPM_SOIL_GARDNER && prm->soil_model != WSPM_SOIL_VAN_GENUCHTEN)
         return fail(ctx, WSPM_EINVAL, "unknown soil model");
     int rc = check_krylov_args(ctx, prm->method, prm->maxiter, prm->restart);
@@ -980,7 +1075,12 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
     if (prm->precision != WSPM_FP64 && prm->precision != WSPM_MIXED)
         return fail(ctx, WSPM_EINVAL, "unknown precision");
     if (prm->picard_maxit < 1) return fail(ctx, WSPM_EINVAL, "picard_maxit must be >= 1");
-    API_BEGIN
+    return WSPM_OK;
+}
+
+// Time-stepping arrays, Ks_e and h0 (both device, copied).
+static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const double* Ks_elem,
+                           const double* h0) {
     const int n = ctx->n;
     if (!ctx->hA) {
         ctx->hA = dalloc<double>(ctx, n); ctx->hB = dalloc<double>(ctx, n);
@@ -993,18 +1093,29 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
         if (!ctx->Ks) ctx->Ks = dalloc<double>(ctx, ctx->E);
         HIPCHK(hipMemcpyAsync(ctx->Ks, Ks_elem, ctx->E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    ctx->prm = *prm;
-    double p[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 6; ++k) p[k] = prm->phi[k];
-    if (Ks_elem) p[prm->soil_model == WSPM_SOIL_GARDNER ? 3 : 5] = 1.0;  // nodal model gives Kr
-    ctx->sp = make_soil(prm->soil_model, p);
-    HIPCHK(hipMemcpyAsync(ctx->hA, h0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx)}, 0);
-    ctx->nrec_dirty = false;
     if (prm->precision == WSPM_MIXED && !ctx->A32) {
         ctx->A32 = dalloc<float>(ctx, ctx->nnz);
         ctx->dinv32 = dalloc<float>(ctx, n);
     }
+    ctx->prm = *prm;
+    HIPCHK(hipMemcpyAsync(ctx->hA, h0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const double* Ks_elem,
+                           const double* h0) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!prm || !h0) return fail(ctx, WSPM_EINVAL, "null argument");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "ensemble context: use wspm_ens_richards_init");
+    int rc = check_richards_params(ctx, prm);
+    if (rc) return rc;
+    API_BEGIN
+    richards_alloc(ctx, prm, Ks_elem, h0);
+    double p[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; ++k) p[k] = prm->phi[k];
+    if (Ks_elem) p[prm->soil_model == WSPM_SOIL_GARDNER ? 3 : 5] = 1.0;  // nodal model gives Kr
+    ctx->sp = make_soil(prm->soil_model, p);
+    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 0}, 0);
+    ctx->nrec_dirty = false;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->rich_init = true;
     ctx->has_ks = Ks_elem != nullptr;
@@ -1012,18 +1123,24 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
     API_END
 }
 
+static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats);
+
 int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     if (!ctx) return WSPM_EINVAL;
     if (!ctx->rich_init) return fail(ctx, WSPM_ESTATE, "wspm_richards_init not called");
     if (!(dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
     API_BEGIN
+    if (ctx->tab) {
+        ens_step(ctx, dt, stats);
+        return WSPM_OK;
+    }
     const int n = ctx->n;
     const wspm_richards_params& P = ctx->prm;
     const bool mixed = P.precision == WSPM_MIXED;
     const bool jac = P.precond == WSPM_PRECOND_JACOBI;
     const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
     if (ctx->nrec_dirty) {  // a direct assembly call replaced K in the node records
-        launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx)}, 0);
+        launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 0}, 0);
         ctx->nrec_dirty = false;
     }
     // theta_n = theta(h_n): the soil state at the start of the step is current
@@ -1110,7 +1227,7 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on) {
     API_END
 }
 
-int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms) {
+int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows) {
     if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
     API_BEGIN
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1118,18 +1235,299 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms) {
     std::vector<int> flags(pairs, 0);
     if (pairs) HIPCHK(hipMemcpy(flags.data(), ctx->tflags, pairs * sizeof(int), hipMemcpyDeviceToHost));
     double tot = 0.0;
-    int64_t worked = 0;
+    int64_t worked = 0, nrows = 0;
     for (size_t k = 0; k < pairs; ++k) {
         if (!flags[k]) continue;
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[2 * k], ctx->ev_pool[2 * k + 1]));
         tot += ms;
         ++worked;
+        nrows += flags[k];
     }
     *launches = worked;
     *total_ms = tot;
+    if (rows) *rows = nrows;
     ctx->ev_used = 0;
     if (ctx->tflags) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
+    return WSPM_OK;
+    API_END
+}
+
+
+// ---------------------------------------------------------------------------
+// Ensemble of independent instances
+// ---------------------------------------------------------------------------
+// Rows per ensemble workgroup: the register assembly runs one thread per row.
+static constexpr int ENS_ROWS = BLOCK;
+
+// Time rep launches of the plain SpMV at each candidate row width and keep
+// the fastest (ties: the narrower).  The values are zeros: timing only.
+static void ens_tune_lanes(wspm_ctx* ctx) {
+    const char* env = std::getenv("WSPM_LANES");
+    if (env) {
+        const int l = std::atoi(env);
+        if (l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64) {
+            ctx->lanes = l;
+            return;
+        }
+    }
+    double* av = nullptr;
+    double* xy = nullptr;
+    HIPCHK(hipMalloc(&av, ctx->nnz * sizeof(double)));
+    hipError_t e = hipMalloc(&xy, 2 * (size_t)ctx->n * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipFree(av);
+        HIPCHK(e);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        HIPCHK(hipMemsetAsync(av, 0, ctx->nnz * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(xy, 0, 2 * (size_t)ctx->n * sizeof(double), ctx->stream));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const EpiPlain<double> epi{xy, xy + ctx->n};
+        const int cands[3] = {4, 8, 16};
+        int best = ctx->lanes;
+        float best_ms = 1e30f;
+        for (int c : cands) {
+            ctx->lanes = c;
+            for (int r = 0; r < 3; ++r) launch_spmv<double>(ctx, av, epi, 0);
+            HIPCHK(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 20; ++r) launch_spmv<double>(ctx, av, epi, 0);
+            HIPCHK(hipEventRecord(e1, ctx->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms * 0.98f) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        ctx->lanes = best;
+    } catch (const Err&) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(av);
+        (void)hipFree(xy);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(av);
+    (void)hipFree(xy);
+}
+
+int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts, const int32_t* elem_counts,
+                        const int32_t* conn, const double* coords, void* stream, wspm_ctx** out) {
+    if (!out || !node_counts || !elem_counts || !conn || !coords) {
+        g_create_error = "null argument";
+        return WSPM_EINVAL;
+    }
+    *out = nullptr;
+    if (dim != 2) {
+        g_create_error = "ensembles are 2D (triangle meshes)";
+        return WSPM_EINVAL;
+    }
+    if (n_inst < 1) {
+        g_create_error = "n_inst must be >= 1";
+        return WSPM_EINVAL;
+    }
+    const int K = dim + 1;
+    std::vector<int> noff(n_inst + 1, 0), eoff(n_inst + 1, 0);
+    for (int i = 0; i < n_inst; ++i) {
+        if (node_counts[i] < 1 || elem_counts[i] < 1) {
+            g_create_error = "every instance needs nodes and elements";
+            return WSPM_EINVAL;
+        }
+        const int64_t nn = (int64_t)noff[i] + node_counts[i], ne = (int64_t)eoff[i] + elem_counts[i];
+        if (nn > INT32_MAX / 2 || ne > INT32_MAX / 4) {
+            g_create_error = "ensemble too large for 32-bit indices";
+            return WSPM_EINVAL;
+        }
+        noff[i + 1] = (int)nn;
+        eoff[i + 1] = (int)ne;
+    }
+    // instance-local connectivity -> global node numbers
+    std::vector<int32_t> gconn((size_t)eoff[n_inst] * K);
+    for (int i = 0; i < n_inst; ++i) {
+        for (int64_t q = (int64_t)eoff[i] * K; q < (int64_t)eoff[i + 1] * K; ++q) {
+            const int32_t v = conn[q];
+            if (v < 0 || v >= node_counts[i]) {
+                g_create_error = "instance " + std::to_string(i) + ": vertex index out of range";
+                return WSPM_EINVAL;
+            }
+            gconn[q] = v + noff[i];
+        }
+    }
+    wspm_ctx* ctx = nullptr;
+    int rc = wspm_ctx_create(dim, noff[n_inst], eoff[n_inst], gconn.data(), coords, stream, &ctx);
+    if (rc != WSPM_OK) return rc;
+    if (!ctx->asm_reg) {
+        g_create_error = "ensemble rows must have at most 8 entries (register assembly)";
+        wspm_ctx_destroy(ctx);
+        return WSPM_EINVAL;
+    }
+    try {
+        std::vector<RowTab> tab;
+        std::vector<int> iblk(n_inst + 1, 0);
+        for (int i = 0; i < n_inst; ++i) {
+            iblk[i] = (int)tab.size();
+            for (int r = noff[i]; r < noff[i + 1]; r += ENS_ROWS)
+                tab.push_back(make_int4(r, std::min(r + ENS_ROWS, noff[i + 1]), i,
+                                        r == noff[i] ? noff[i + 1] - noff[i] : 0));
+        }
+        iblk[n_inst] = (int)tab.size();
+        ctx->n_inst = n_inst;
+        ctx->n_tab = (int)tab.size();
+        ctx->tab = dupload(ctx, tab.data(), tab.size());
+        ctx->iblk = dupload(ctx, iblk.data(), iblk.size());
+        if (ctx->n_tab > ctx->prow) {
+            ctx->prow = ctx->n_tab;
+            ctx->partials = dalloc<double>(ctx, (size_t)ctx->prow * MAX_DOTS);
+        }
+        ctx->states = dalloc<KState>(ctx, 3 * (size_t)n_inst);
+        HIPCHK(hipMemset(ctx->states, 0, 3 * (size_t)n_inst * sizeof(KState)));
+        ctx->ens_status = dalloc<int>(ctx, 4);
+        ctx->ens_out = dalloc<int>(ctx, 5 * (size_t)n_inst + 2);  // 3 int + 1 double each
+        void* hp = nullptr;
+        HIPCHK(hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault));
+        ctx->host_allocs.push_back(hp);
+        ctx->hstatus = static_cast<int*>(hp);
+        std::memset(ctx->hstatus, 0, 4 * sizeof(int));
+        ens_tune_lanes(ctx);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } catch (const Err&) {
+        g_create_error = ctx->err;
+        free_ctx(ctx);
+        return WSPM_EHIP;
+    }
+    *out = ctx;
+    return WSPM_OK;
+}
+
+int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const double* phi,
+                               const double* Ks_elem, const double* h0) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!ctx->tab) return fail(ctx, WSPM_EINVAL, "not an ensemble context");
+    if (!prm || !h0) return fail(ctx, WSPM_EINVAL, "null argument");
+    int rc = check_richards_params(ctx, prm);
+    if (rc) return rc;
+    if (prm->method == WSPM_GMRES) return fail(ctx, WSPM_EINVAL, "ensembles solve with CG or BiCGSTAB");
+    if (prm->precision != WSPM_FP64) return fail(ctx, WSPM_EINVAL, "ensembles are FP64");
+    API_BEGIN
+    richards_alloc(ctx, prm, Ks_elem, h0);
+    const int ni = ctx->n_inst;
+    std::vector<KState> hs(ni);
+    std::memset(hs.data(), 0, ni * sizeof(KState));
+    for (int i = 0; i < ni; ++i) {
+        double p[6];
+        for (int k = 0; k < 6; ++k) p[k] = phi ? phi[6 * (size_t)i + k] : prm->phi[k];
+        if (prm->soil_model == WSPM_SOIL_GARDNER) p[4] = p[5] = 0.0;
+        if (Ks_elem) p[prm->soil_model == WSPM_SOIL_GARDNER ? 3 : 5] = 1.0;  // nodal model gives Kr
+        hs[i].sp = make_soil(prm->soil_model, p);
+        hs[i].picard_tol = prm->picard_tol;
+        hs[i].picard_maxit = prm->picard_maxit;
+        hs[i].picard_rel = prm->picard_mode == WSPM_PICARD_REL;
+        hs[i].done = 1;
+        hs[i].pdone = 1;
+    }
+    // hs is pageable and local: the stream is synchronised below, before it goes
+    HIPCHK(hipMemcpyAsync(states_of(ctx, 0), hs.data(), ni * sizeof(KState), hipMemcpyHostToDevice, ctx->stream));
+    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 1}, 0);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nrec_dirty = false;
+    ctx->rich_init = true;
+    ctx->has_ks = Ks_elem != nullptr;
+    return WSPM_OK;
+    API_END
+}
+
+// One time step of every instance: the Picard loop of
+// src/solvers/richards_solver_v2.py:141-168 per instance.  Each Picard
+// iteration is one assembly (+ solver start) over the instances still
+// iterating, their Krylov solves (instances drop out as they converge), and
+// the Picard error per instance; the host reads back three counters per
+// iteration chunk.
+static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
+    const wspm_richards_params& P = ctx->prm;
+    const bool jac = P.precond == WSPM_PRECOND_JACOBI;
+    const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
+    KState* st0 = states_of(ctx, 0);
+    const int ni = ctx->n_inst;
+    hipLaunchKernelGGL(k_ens_step_begin, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    const int mode = P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
+    int it = 0;
+    while (true) {
+        do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
+                    jac ? ctx->dinv : nullptr, nullptr, nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter,
+                    P.restart);
+        KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
+                              P.tol, P.atol, P.maxiter, P.restart, 0, true);
+        const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
+                           P.picard_mode == WSPM_PICARD_REL, 1};
+        run.enqueue(std::max(1, ctx->last_iters[P.method]));
+        launch_vec(ctx, pop, 0);
+        poll_ens(ctx, 0);
+        while (ctx->hstatus[0] > 0 && run.enq < P.maxiter) {
+            run.enqueue(2);
+            launch_vec(ctx, pop, 0);
+            poll_ens(ctx, 0);
+        }
+        ctx->last_iters[P.method] = std::max(1, ctx->hstatus[2]);
+        std::swap(ctx->hA, ctx->hB);
+        ++it;
+        if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
+    }
+    if (stats) {
+        int* ip = ctx->ens_out;
+        int* ik = ip + ni;
+        int* ic = ik + ni;
+        double* er = reinterpret_cast<double*>(ctx->ens_out + 3 * (size_t)ni + (ni & 1));
+        hipLaunchKernelGGL(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni,
+                           ip, ik, ic, er);
+        HIPCHK(hipGetLastError());
+        std::vector<int> hi(3 * (size_t)ni);
+        std::vector<double> he(ni);
+        HIPCHK(hipMemcpyAsync(hi.data(), ip, 3 * (size_t)ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(he.data(), er, ni * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        int pmax = 0, conv = 1;
+        int64_t ksum = 0;
+        double emax = 0.0;
+        for (int i = 0; i < ni; ++i) {
+            pmax = std::max(pmax, hi[i]);
+            ksum += hi[ni + i];
+            conv &= hi[2 * ni + i];
+            emax = std::isnan(he[i]) || std::isnan(emax) ? NAN : std::max(emax, he[i]);
+        }
+        stats->picard_iters = pmax;
+        stats->krylov_iters = (int32_t)std::min<int64_t>(ksum, INT32_MAX);
+        stats->converged = conv;
+        stats->reserved = 0;
+        stats->err = emax;
+    }
+}
+
+int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylov_iters, int32_t* converged,
+                            double* err) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!ctx->tab) return fail(ctx, WSPM_EINVAL, "not an ensemble context");
+    API_BEGIN
+    const int ni = ctx->n_inst;
+    int* ip = ctx->ens_out;
+    int* ik = ip + ni;
+    int* ic = ik + ni;
+    double* er = reinterpret_cast<double*>(ctx->ens_out + 3 * (size_t)ni + (ni & 1));
+    hipLaunchKernelGGL(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
+                       states_of(ctx, 0), ni, ip, ik, ic, er);
+    HIPCHK(hipGetLastError());
+    if (picard_iters) HIPCHK(hipMemcpyAsync(picard_iters, ip, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (krylov_iters) HIPCHK(hipMemcpyAsync(krylov_iters, ik, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (converged) HIPCHK(hipMemcpyAsync(converged, ic, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (err) HIPCHK(hipMemcpyAsync(err, er, ni * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
 }

@@ -8,6 +8,8 @@
 
 namespace wspm {
 
+struct KState;
+
 // Rows per assembly workgroup.  2D rows have ~6 incident triangles and ~7
 // entries, 3D rows ~24 incident tets and ~15 entries; R sets the LDS staging
 // (incidences x (dim+2) doubles + the rows' CSR window) and the write-out
@@ -74,6 +76,12 @@ struct AsmArgs {
     double* dinv;            // nullable
     float* dinv32;           // nullable
     QuadConst qc;
+    // Ensemble mode (register kernel only): workgroup b assembles rows
+    // [tab[b].x, tab[b].y) of instance tab[b].z; instances whose Picard
+    // iteration has converged (ens_st[z].pdone) only carry x0 over to x.
+    const int4* tab;         // nullable
+    const KState* ens_st;
+    int n_tab;
 };
 
 // Vectors written by the fused Krylov start (FP64).

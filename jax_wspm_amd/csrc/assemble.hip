@@ -291,9 +291,20 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
 template <int D, int MAXR, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     constexpr int K = D + 1;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int row_end = a.n;
+    if (a.tab != nullptr) {
+        const int4 tb = a.tab[blockIdx.x];
+        i = tb.x + threadIdx.x;
+        row_end = tb.y;
+        if (a.ens_st[tb.z].pdone) {   // uniform over the workgroup
+            if constexpr (MODE != ASM_INIT_NONE)
+                if (i < row_end) ki.x[i] = ki.x0[i];
+            return;
+        }
+    }
     double acc[3] = {0.0, 0.0, 0.0};
-    if (i < a.n) {
+    if (i < row_end) {
         const int rs = a.row_ptr[i];
         const int rl = a.row_ptr[i + 1] - rs;
         const int dg = a.diag_pos[i] - rs;
@@ -463,7 +474,7 @@ int asm_reg_max_row(int dim) { return dim == 2 ? ASM_REG_ROW_2D : ASM_REG_ROW_3D
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg) {
     if (use_reg) {
-        const int grid = (a.n + BLOCK - 1) / BLOCK;
+        const int grid = a.tab ? a.n_tab : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
         return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
     }

@@ -32,28 +32,73 @@ struct KVec {
 // CSR SpMV: LANES consecutive lanes own one row (LANES | 64), the row sum is
 // finished with a width-LANES butterfly, then lane 0 runs the epilogue.
 // ---------------------------------------------------------------------------
-// tflag (nullable): the benchmark's launch-timing slot, set when the launch did
-// its work (not skipped), so timed averages count working launches only.
+// tflag (nullable): the benchmark's launch-timing slot; receives the number of
+// rows the launch worked on (skipped instances count none), so timed averages
+// count working launches only and weigh them by their rows.
+// tab (nullable): ensemble mode; workgroup b owns rows [tab[b].x, tab[b].y)
+// and the state red.st[tab[b].z] of their instance.  Otherwise the grid
+// strides over all n rows with the state red.st[0].
+//
+// SPMV_UNROLL rows per lane group and pass: their row pointers, then their
+// first LANES entries (values + columns), then the x gathers are each issued
+// together, so a pass costs three dependent memory round trips for
+// SPMV_UNROLL rows instead of for one.
+constexpr int SPMV_UNROLL = 4;
+
 template <int LANES, class T, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ ci,
-                                                 const T* __restrict__ av, Epi epi, Red red, int* tflag) {
-    if (epi.skip(red.st)) return;
-    if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = 1;
+                                                 const T* __restrict__ av, Epi epi, Red red, int* tflag,
+                                                 const RowTab* __restrict__ tab) {
+    const RowTab tb = tab ? tab[blockIdx.x] : make_int4(0, n, 0, 0);
+    KState* st = red.st + tb.z;
+    if (epi.skip(st)) return;
+    if (tflag != nullptr && threadIdx.x == 0) {
+        if (!tab) {
+            if (blockIdx.x == 0) *tflag = n;
+        } else if (tb.w > 0) {
+            atomicAdd(tflag, tb.w);    // first workgroup of an instance: its row count
+        }
+    }
     constexpr int GPB = BLOCK / LANES;
     constexpr int ND = Epi::ND;
+    constexpr int U = SPMV_UNROLL;
     const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
     double acc[ND > 0 ? ND : 1];
 #pragma unroll
     for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
-    epi.load(red.st);
-    for (int row = blockIdx.x * GPB + g; row < n; row += gridDim.x * GPB) {
-        double s = 0.0;
-        const int kb = rp[row], ke = rp[row + 1];
-        for (int k = kb + l; k < ke; k += LANES) s += (double)av[k] * epi.xv(ci[k]);
+    epi.load(st);
+    const int stride = tab ? GPB : gridDim.x * GPB;
+    n = tb.y;
+    for (int r0 = (tab ? tb.x : blockIdx.x * GPB) + g; r0 < n; r0 += stride * U) {
+        int kb[U], ke[U];
 #pragma unroll
-        for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
-        if (l == 0) epi.row(row, s, acc);
+        for (int u = 0; u < U; ++u) {
+            const int row = r0 + u * stride;
+            kb[u] = row < n ? rp[row] : 0;
+            ke[u] = row < n ? rp[row + 1] : 0;
+        }
+        int col[U];
+        double val[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = kb[u] + l;
+            col[u] = k < ke[u] ? ci[k] : -1;
+            val[u] = k < ke[u] ? (double)av[k] : 0.0;
+        }
+        double s[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] = col[u] >= 0 ? val[u] * epi.xv(col[u]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            for (int k = kb[u] + l + LANES; k < ke[u]; k += LANES) s[u] += (double)av[k] * epi.xv(ci[k]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int o = LANES / 2; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, LANES);
+            const int row = r0 + u * stride;
+            if (l == 0 && row < n) epi.row(row, s[u], acc);
+        }
     }
     if constexpr (ND > 0) {
         double v[ND];
@@ -65,14 +110,20 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
 
 // Elementwise kernel with fused dot products.
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red) {
-    if (op.skip(red.st)) return;
+__global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, const RowTab* __restrict__ tab) {
+    const RowTab tb = tab ? tab[blockIdx.x] : make_int4(0, n, 0, 0);
+    KState* st = red.st + tb.z;
+    if (op.skip(st)) return;
     constexpr int ND = Op::ND;
     double acc[ND > 0 ? ND : 1];
 #pragma unroll
     for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
-    op.load(red.st);
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) op.row(i, acc);
+    op.load(st);
+    if (tab) {
+        for (int i = tb.x + threadIdx.x; i < tb.y; i += BLOCK) op.row(i, acc);
+    } else {
+        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) op.row(i, acc);
+    }
     if constexpr (ND > 0) {
         double v[ND];
 #pragma unroll

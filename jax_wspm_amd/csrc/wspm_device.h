@@ -13,6 +13,12 @@ constexpr int MAX_BLOCKS = 2048;    // cap on the grids of SpMV / vector reducti
 constexpr int MAX_DOTS = 8;         // dot products per reduction kernel
 constexpr int MAX_RESTART = 64;
 
+// Soil model and parameters (phi layout of the C ABI).
+struct SoilParams {
+    int model;        // 0 gardner, 1 van Genuchten
+    double p[6];
+};
+
 // Scalars of one Krylov solve, resident in device memory.  Written only by
 // the single finishing workgroup of a reduction kernel, read by all later
 // kernels of the stream.
@@ -29,6 +35,14 @@ struct KState {
     int32_t exit_early, pass2, active, code;
     int32_t cycles, arnoldi, restart, nan_flag;
     int32_t jdone, refine, pad0, pad1;
+    // Ensemble mode (one KState per instance): Picard control of the instance
+    // within the current time step, and its soil parameters.
+    int32_t pdone;                           // Picard converged or capped this step
+    int32_t pev;                             // Picard error of this iteration taken
+    int32_t pit, kry;                        // Picard / Krylov iterations this step
+    double picard_tol;
+    int32_t picard_maxit, picard_rel;
+    SoilParams sp;
     double hbuf[MAX_RESTART + 1];            // CGS projection of the current pass
     double y[MAX_RESTART];                   // GMRES least-squares solution
 };
@@ -73,8 +87,14 @@ __device__ inline void block_sum(double (&v)[ND], double* sh) {
 struct Red {
     double* partials;     // column-major [MAX_DOTS][prow]: dot j of workgroup b at j * prow + b
     int prow;
-    KState* st;
+    KState* st;           // solver state (ensemble: array indexed by instance)
 };
+
+// Ensemble row table: workgroup b owns rows [x, y) of instance z; w is the
+// instance's row count in its first workgroup's entry, 0 in the others.
+// The workgroups of one instance are consecutive; iblk[i] is the first of
+// instance i (iblk[n_inst] = number of workgroups).
+using RowTab = int4;
 
 // Store this workgroup's ND block sums in column blockIdx of the partials
 // (NW = wavefronts per workgroup).
@@ -107,13 +127,34 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restric
     if (threadIdx.x == 0) op.fin(acc, st);
 }
 
+// Ensemble finish: workgroup i adds the partials of instance i's workgroups
+// [iblk[i], iblk[i+1]) in order and runs op.fin on state st[i].  Instances
+// whose Picard iteration has converged are left alone.  RESET_PEV: this
+// reduction starts a Picard iteration (assembly + solver start).
+template <int ND, class Op, bool RESET_PEV = false>
+__global__ __launch_bounds__(BLOCK) void k_finish_ens(Op op, const double* __restrict__ partials, int prow,
+                                                       const int* __restrict__ iblk, KState* st0) {
+    KState* st = st0 + blockIdx.x;
+    if (st->pdone || op.skip(st)) return;
+    const int b0 = iblk[blockIdx.x], b1 = iblk[blockIdx.x + 1];
+    double acc[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += BLOCK) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
+    }
+    __shared__ double sh[ND * NWAVE];
+    block_sum<ND>(acc, sh);
+    if (threadIdx.x == 0) {
+        op.fin(acc, st);
+        if (RESET_PEV) st->pev = 0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Soil models (src/models/van_genuchten.py:9-47, src/models/exponential.py:10-42)
 // ---------------------------------------------------------------------------
-struct SoilParams {
-    int model;        // 0 gardner, 1 van Genuchten
-    double p[6];
-};
 
 __device__ inline void soil_eval(const SoilParams& sp, double h, double& C, double& K, double& th) {
     if (sp.model == 1) {

@@ -176,7 +176,9 @@ class Context:
         self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
 
     def timing_read(self):
+        """(working SpMV launches, their summed duration in ms, rows they worked on)."""
         n = C.c_int64()
         ms = C.c_double()
-        self._ck(self.lib.wspm_timing_read(self.h, C.byref(n), C.byref(ms)))
-        return int(n.value), float(ms.value)
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)

@@ -1,0 +1,179 @@
+"""Ensembles of independent Richards instances on one GPU (``wspm_ens_*``).
+
+The reference solves one instance per process (src/cli.py:87-150, one
+``RichardsSolver`` per run).  An :class:`Ensemble` time-steps many instances
+of the same implicit step (src/solvers/richards_solver_v2.py:107-168) at once:
+every instance has its own mesh, van Genuchten / Gardner parameters, Dirichlet
+rows and initial state; each one's Krylov solve and Picard loop stop on their
+own, and each one's result equals that of the single-instance path on the same
+inputs (its dot products cover its own rows only).
+
+:func:`vg_ensemble` builds the uneven-size van Genuchten ensemble: side N of
+each unit-square instance drawn from {32, 48, ..., 256}, alpha ~ U(0.01, 0.2),
+n ~ U(1.3, 3.0) (m = 1 - 1/n), Ks ~ logU(1e-4, 1e-2), theta_r ~ U(0.02, 0.1),
+theta_s ~ U(0.35, 0.5); h0 = -1000, h = -75 on the top edge.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import mesh as M
+from ._lib import check, ptr
+from .device import _f64, _require_gpu, _stream
+
+
+@dataclass
+class Instance:
+    """One member: mesh, soil parameters (phi layout of the C ABI), BCs, h0."""
+    points: np.ndarray
+    elements: np.ndarray
+    phi: Tuple[float, ...]
+    bc_lists: List[Tuple[np.ndarray, np.ndarray]]
+    h0: np.ndarray
+    Ks_elem: Optional[np.ndarray] = None
+    cells: int = 0
+
+
+def vg_ensemble(n_inst: int = 256, sides: Sequence[int] = tuple(range(32, 257, 16)), seed: int = 0,
+                h_init: float = -1000.0, h_top: float = -75.0) -> List[Instance]:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n_inst):
+        N = int(rng.choice(np.asarray(sides)))
+        alpha = rng.uniform(0.01, 0.2)
+        n = rng.uniform(1.3, 3.0)
+        ks = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2))))
+        tr = rng.uniform(0.02, 0.1)
+        ts = rng.uniform(0.35, 0.5)
+        pts, el = M.unit_square(N, 1.0)
+        top = np.nonzero(pts[:, 1] == 1.0)[0]
+        out.append(Instance(points=pts, elements=el, phi=(alpha, ts, tr, n, 1.0 - 1.0 / n, ks),
+                            bc_lists=[(top, np.full(top.shape, h_top))], h0=np.full(len(pts), h_init),
+                            cells=N))
+    return out
+
+
+class Ensemble:
+    """Instances resident on the GPU in one ensemble context."""
+
+    def __init__(self, instances: Sequence[Instance]):
+        _require_gpu()
+        self.lib = _lib.load()
+        self.instances = list(instances)
+        if not self.instances:
+            raise ValueError("empty ensemble")
+        dims = {inst.points.shape[1] for inst in self.instances}
+        if dims != {2}:
+            raise ValueError("ensembles are 2D")
+        self.n_inst = len(self.instances)
+        nc = np.array([inst.points.shape[0] for inst in self.instances], dtype=np.int32)
+        ec = np.array([inst.elements.shape[0] for inst in self.instances], dtype=np.int32)
+        self.node_off = np.concatenate([[0], np.cumsum(nc)]).astype(np.int64)
+        self.elem_off = np.concatenate([[0], np.cumsum(ec)]).astype(np.int64)
+        conn = np.ascontiguousarray(np.concatenate([inst.elements for inst in self.instances]), dtype=np.int32)
+        coords = np.ascontiguousarray(np.concatenate([inst.points for inst in self.instances]), dtype=np.float64)
+        h = C.c_void_p()
+        check(self.lib.wspm_ens_create(2, self.n_inst, ptr(nc), ptr(ec), ptr(conn), ptr(coords), _stream(),
+                                       C.byref(h)), None)
+        self.h = h
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        self.n, self.n_elems, self.nnz = int(info[0]), int(info[1]), int(info[2])
+        self.lanes = int(info[6])
+        self.device = torch.device("cuda")
+        nodes, vals = [], []
+        for i, inst in enumerate(self.instances):
+            for nd, v in inst.bc_lists:
+                nd = np.asarray(nd, dtype=np.int64).ravel()
+                nodes.append((nd + self.node_off[i]).astype(np.int32))
+                vals.append(np.broadcast_to(np.asarray(v, dtype=np.float64), nd.shape))
+        nodes = np.ascontiguousarray(np.concatenate(nodes) if nodes else np.zeros(0, np.int32))
+        vals = np.ascontiguousarray(np.concatenate(vals) if vals else np.zeros(0))
+        check(self.lib.wspm_bc_set(self.h, int(nodes.size), ptr(nodes), ptr(vals)), self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wspm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        check(rc, self.h)
+
+    def richards_init(self, soil_model: str = "vg", method: str = "bicgstab", precond: Optional[str] = "jacobi", *,
+                      tol=1e-10, atol=0.0, maxiter=100, picard_tol=1e-6, picard_maxit=100, picard_mode="rel"):
+        if method not in ("cg", "bicgstab"):
+            raise ValueError(f"ensembles solve with cg or bicgstab, not {method!r}")
+        if precond not in _lib.PRECOND:
+            raise NotImplementedError(f"preconditioner {precond!r} is not available (supported: none, jacobi)")
+        prm = _lib.RichardsParams()
+        prm.soil_model = _lib.SOIL[soil_model]
+        prm.method = _lib.METHOD[method]
+        prm.precond = _lib.PRECOND[precond]
+        prm.precision = _lib.PRECISION["fp64"]
+        prm.tol, prm.atol, prm.maxiter, prm.restart = float(tol), float(atol), int(maxiter), 30
+        prm.picard_tol, prm.picard_maxit = float(picard_tol), int(picard_maxit)
+        prm.picard_mode = _lib.PICARD[picard_mode]
+        phi = np.zeros((self.n_inst, 6))
+        for i, inst in enumerate(self.instances):
+            pa = np.asarray(inst.phi, dtype=np.float64)
+            phi[i, : pa.size] = pa
+        phi = np.ascontiguousarray(phi)
+        h0 = _f64(np.concatenate([np.asarray(inst.h0, dtype=np.float64) for inst in self.instances]))
+        has_ks = [inst.Ks_elem is not None for inst in self.instances]
+        if any(has_ks) and not all(has_ks):
+            raise ValueError("Ks_elem must be given for every instance or for none")
+        ks = _f64(np.concatenate([inst.Ks_elem for inst in self.instances])) if all(has_ks) else None
+        self._ck(self.lib.wspm_ens_richards_init(self.h, C.byref(prm), ptr(phi), ptr(ks), ptr(h0)))
+
+    def step(self, dt: float, per_instance: bool = False):
+        st = _lib.StepStats()
+        self._ck(self.lib.wspm_richards_step(self.h, float(dt), C.byref(st)))
+        out = {"picard_iters": st.picard_iters, "krylov_iters": st.krylov_iters, "converged": bool(st.converged),
+               "err": st.err}
+        if per_instance:
+            out.update(self.step_stats())
+        return out
+
+    def step_stats(self):
+        ip = np.zeros(self.n_inst, dtype=np.int32)
+        ik = np.zeros(self.n_inst, dtype=np.int32)
+        ic = np.zeros(self.n_inst, dtype=np.int32)
+        er = np.zeros(self.n_inst)
+        self._ck(self.lib.wspm_ens_step_stats(self.h, ptr(ip), ptr(ik), ptr(ic), ptr(er)))
+        return {"picard_iters_i": ip, "krylov_iters_i": ik, "converged_i": ic.astype(bool), "err_i": er}
+
+    def state(self):
+        """Concatenated (h, theta, theta at the start of the last step) on the GPU."""
+        h = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        th = torch.empty_like(h)
+        thn = torch.empty_like(h)
+        self._ck(self.lib.wspm_richards_state(self.h, ptr(h), ptr(th), ptr(thn)))
+        return h, th, thn
+
+    def split(self, x) -> List[np.ndarray]:
+        """Per-instance pieces of a concatenated nodal array."""
+        x = x.cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+        return [x[self.node_off[i]:self.node_off[i + 1]] for i in range(self.n_inst)]
+
+    def timing(self, on: bool):
+        self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self):
+        """(working SpMV launches, their summed duration in ms, rows they worked on)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)
