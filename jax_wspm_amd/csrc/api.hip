@@ -66,14 +66,16 @@ struct SoilOp {
 // |h - h_m|, |h| and the soil model at the new iterate (used by the next
 // Picard iteration, and as theta(h) once the step has converged).
 // Ensemble mode (ens): an instance is processed once per Picard iteration,
-// after its Krylov solve is done, with its own soil parameters; the finish
-// counts its Picard/Krylov iterations and marks it converged (or capped), as
-// the loop condition of src/solvers/richards_solver_v2.py:141-168.
+// after its Krylov solve is done, with its own soil parameters; h is copied
+// into hm (instances finish their iterations at different times, so the two
+// buffers are not swapped); the finish counts its Picard/Krylov iterations and
+// marks it converged (or capped), as the loop condition of
+// src/solvers/richards_solver_v2.py:141-168.
 struct PicardOp {
     static constexpr int ND = 2;
     SoilParams sp;
     const double* h;
-    const double* hm;
+    double* hm;                 // ensemble: receives h (the new iterate becomes current)
     double *C, *K, *th;
     double* krec;
     int rel;
@@ -85,6 +87,7 @@ struct PicardOp {
     __device__ void row(int i, double* acc) const {
         const double hi = h[i];
         const double d = hi - hm[i];
+        if (ens) hm[i] = hi;
         acc[0] += d * d;
         acc[1] += hi * hi;
         double c, k, t;
@@ -142,6 +145,35 @@ __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__
         out[2] = cnt[2];
         out[3] = 0;
     }
+}
+
+// Active-entry list of the row table: act[0] = number of entries, act[1..] =
+// the entries of the instances still active, in instance order.  mode 0:
+// Krylov solve running (!pdone && !done); mode 1: Picard iterating (!pdone).
+// Workgroup i places instance i: its offset is the entry count of the active
+// instances before it (one pass over their flags), its entries are written
+// in parallel.
+__device__ inline bool ens_on(const KState& s, int mode) { return !s.pdone && (mode == 1 || !s.done); }
+
+__global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict__ st, int n_inst,
+                                                       const int* __restrict__ iblk, int* act, int mode) {
+    const int i = blockIdx.x;
+    int c = 0;
+    for (int j = threadIdx.x; j < i; j += BLOCK)
+        if (ens_on(st[j], mode)) c += iblk[j + 1] - iblk[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    __shared__ int ws[NWAVE];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int pos = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) pos += ws[w];
+    const bool on = ens_on(st[i], mode);
+    const int b0 = iblk[i], nb = iblk[i + 1] - b0;
+    if (on)
+        for (int b = threadIdx.x; b < nb; b += BLOCK) act[1 + pos + b] = b0 + b;
+    if (i == n_inst - 1 && threadIdx.x == 0) act[0] = pos + (on ? nb : 0);
 }
 
 // Per-instance statistics of the last time step.
@@ -395,6 +427,9 @@ struct wspm_ctx {
     int* ens_status = nullptr;      // device int[4] (k_ens_status)
     int* hstatus = nullptr;         // pinned mirror
     int* ens_out = nullptr;         // k_ens_collect output: 3 int + 1 double per instance
+    int* kact = nullptr;            // [1 + n_tab] entries of instances whose Krylov solve runs
+    int* pact = nullptr;            // [1 + n_tab] entries of instances still iterating Picard
+    int ens_grid = 0;               // workgroups of an ensemble launch
     std::vector<void*> allocs;
     std::vector<void*> host_allocs;
 };
@@ -437,6 +472,21 @@ Red red_of(wspm_ctx* ctx, int which) {
     return Red{ctx->partials, ctx->prow, states_of(ctx, which)};
 }
 
+// Which table entries an ensemble launch walks.
+enum { LIST_ALL = 0, LIST_K = 1, LIST_P = 2 };
+
+EnsWalk walk_of(wspm_ctx* ctx, int list) {
+    return EnsWalk{ctx->tab, list == LIST_K ? ctx->kact : list == LIST_P ? ctx->pact : nullptr, ctx->n_tab};
+}
+
+// Rebuild an active-entry list (LIST_K or LIST_P) from the instance states.
+void ens_compact(wspm_ctx* ctx, int list) {
+    hipLaunchKernelGGL(k_ens_compact, dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, (const KState*)ctx->states,
+                       ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact,
+                       list == LIST_K ? 0 : 1);
+    HIPCHK(hipGetLastError());
+}
+
 // Finish of a reduction kernel: one workgroup, or one per instance (ensemble).
 template <class Op, bool RESET_PEV = false>
 void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
@@ -453,10 +503,10 @@ void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
 }
 
 template <class Op>
-void launch_vec(wspm_ctx* ctx, const Op& op, int which) {
-    const int grid = ctx->tab ? ctx->n_tab : ctx->vec_grid;
+void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL) {
+    const int grid = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
     hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
-                       (const RowTab*)ctx->tab);
+                       walk_of(ctx, list));
     HIPCHK(hipGetLastError());
     launch_finish(ctx, op, grid, which);
 }
@@ -473,14 +523,14 @@ hipEvent_t next_event(wspm_ctx* ctx) {
 constexpr size_t TIMING_SLOTS = (size_t)1 << 18;
 
 template <class T, class Epi>
-void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false) {
+void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false, int list = LIST_ALL) {
     const bool t = timed && ctx->timing && ctx->ev_used / 2 < TIMING_SLOTS;
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
     const Red red = red_of(ctx, which);
-    const int grid = ctx->tab ? ctx->n_tab : ctx->spmv_grid;
+    const int grid = ctx->tab ? ctx->ens_grid : ctx->spmv_grid;
     const dim3 g(grid), b(BLOCK);
-    const RowTab* tab = ctx->tab;
+    const EnsWalk tab = walk_of(ctx, list);
     switch (ctx->lanes) {
         case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
         case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
@@ -595,18 +645,19 @@ struct KrylovRun {
     void enqueue(int count) {
         for (int c = 0; c < count && enq < maxiter; ++c, ++enq) {
             if (method == WSPM_BICGSTAB) {
-                launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which);
-                launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true);
-                launch_vec(ctx, BicgS<T>{v, 0.0}, which);
-                launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true);
-                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which);
+                launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which, LIST_K);
+                launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true, LIST_K);
+                launch_vec(ctx, BicgS<T>{v, 0.0}, which, LIST_K);
+                launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true, LIST_K);
+                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K);
             } else if (method == WSPM_CG) {
-                launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true);
-                launch_vec(ctx, CgXR<T>{v, 0.0}, which);
-                launch_vec(ctx, CgP<T>{v, 0.0}, which);
+                launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true, LIST_K);
+                launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K);
+                launch_vec(ctx, CgP<T>{v, 0.0}, which, LIST_K);
             } else {
                 gmres_cycle();
             }
+            if (ctx->tab) ens_compact(ctx, LIST_K);   // drop the instances that converged
         }
     }
 
@@ -867,7 +918,7 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
         const SoilOp op{make_soil(model, p), h, C, K, theta, nullptr, 0};
         const int grid = std::min((n + BLOCK - 1) / BLOCK, 1024);
         hipLaunchKernelGGL(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0),
-                           (const RowTab*)nullptr);
+                           EnsWalk{nullptr, nullptr, 0});
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -897,8 +948,10 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.qc = ctx->qc;
     a.tab = ctx->tab;
+    a.act = ctx->pact;
     a.ens_st = states_of(ctx, 0);
     a.n_tab = ctx->n_tab;
+    a.ens_grid = ctx->ens_grid;
     KrylovInit ki{};
     if (mode != ASM_INIT_NONE) {
         KWork<double>& w = ensure_work<double>(ctx, 0);
@@ -932,6 +985,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
             launch_finish<RefResid, true>(ctx, RefResid{b, x, ctx->ref_r}, grid, 2);
             break;
     }
+    if (ctx->tab) ens_compact(ctx, LIST_K);
 }
 
 int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const double* h_m, const double* C,
@@ -1051,7 +1105,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
         ToF32 op{A_vals, ctx->A32};
         const int g = (int)std::min<int64_t>((ctx->nnz + BLOCK - 1) / BLOCK, 4096);
         hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2),
-                           (const RowTab*)nullptr);  // ND = 0
+                           EnsWalk{nullptr, nullptr, 0});  // ND = 0
         HIPCHK(hipGetLastError());
         if (dinv) launch_vec(ctx, ToF32{dinv, ctx->dinv32}, 2);
     }
@@ -1257,8 +1311,10 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int
 // ---------------------------------------------------------------------------
 // Ensemble of independent instances
 // ---------------------------------------------------------------------------
-// Rows per ensemble workgroup: the register assembly runs one thread per row.
+// Rows per ensemble table entry: the register assembly runs one thread per row.
 static constexpr int ENS_ROWS = BLOCK;
+// Workgroups of an ensemble launch (they stride over the listed entries).
+static constexpr int ENS_GRID = 2048;
 
 // Time rep launches of the plain SpMV at each candidate row width and keep
 // the fastest (ties: the narrower).  The values are zeros: timing only.
@@ -1389,6 +1445,15 @@ int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
         HIPCHK(hipMemset(ctx->states, 0, 3 * (size_t)n_inst * sizeof(KState)));
         ctx->ens_status = dalloc<int>(ctx, 4);
         ctx->ens_out = dalloc<int>(ctx, 5 * (size_t)n_inst + 2);  // 3 int + 1 double each
+        ctx->kact = dalloc<int>(ctx, 1 + (size_t)ctx->n_tab);
+        ctx->pact = dalloc<int>(ctx, 1 + (size_t)ctx->n_tab);
+        HIPCHK(hipMemset(ctx->kact, 0, sizeof(int)));
+        HIPCHK(hipMemset(ctx->pact, 0, sizeof(int)));
+        {
+            const char* g = std::getenv("WSPM_ENS_GRID");
+            const int want = g ? std::atoi(g) : ENS_GRID;
+            ctx->ens_grid = std::max(1, std::min(want > 0 ? want : ENS_GRID, ctx->n_tab));
+        }
         void* hp = nullptr;
         HIPCHK(hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault));
         ctx->host_allocs.push_back(hp);
@@ -1433,7 +1498,7 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
     }
     // hs is pageable and local: the stream is synchronised below, before it goes
     HIPCHK(hipMemcpyAsync(states_of(ctx, 0), hs.data(), ni * sizeof(KState), hipMemcpyHostToDevice, ctx->stream));
-    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 1}, 0);
+    launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 1}, 0, LIST_ALL);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nrec_dirty = false;
     ctx->rich_init = true;
@@ -1456,6 +1521,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const int ni = ctx->n_inst;
     hipLaunchKernelGGL(k_ens_step_begin, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni);
     HIPCHK(hipGetLastError());
+    ens_compact(ctx, LIST_P);
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     const int mode = P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
     int it = 0;
@@ -1468,15 +1534,15 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                            P.picard_mode == WSPM_PICARD_REL, 1};
         run.enqueue(std::max(1, ctx->last_iters[P.method]));
-        launch_vec(ctx, pop, 0);
+        launch_vec(ctx, pop, 0, LIST_P);
         poll_ens(ctx, 0);
         while (ctx->hstatus[0] > 0 && run.enq < P.maxiter) {
             run.enqueue(2);
-            launch_vec(ctx, pop, 0);
+            launch_vec(ctx, pop, 0, LIST_P);
             poll_ens(ctx, 0);
         }
+        ens_compact(ctx, LIST_P);
         ctx->last_iters[P.method] = std::max(1, ctx->hstatus[2]);
-        std::swap(ctx->hA, ctx->hB);
         ++it;
         if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
     }

@@ -76,12 +76,16 @@ struct AsmArgs {
     double* dinv;            // nullable
     float* dinv32;           // nullable
     QuadConst qc;
-    // Ensemble mode (register kernel only): workgroup b assembles rows
-    // [tab[b].x, tab[b].y) of instance tab[b].z; instances whose Picard
-    // iteration has converged (ens_st[z].pdone) only carry x0 over to x.
+    // Ensemble mode (register kernel only): a grid of ens_grid workgroups
+    // walks the row-table entries listed in act (act[0] = count; null: all
+    // n_tab entries); entry t = rows [tab[t].x, tab[t].y) of instance
+    // tab[t].z.  Instances whose Picard iteration has converged
+    // (ens_st[z].pdone) are skipped.
     const int4* tab;         // nullable
+    const int* act;
     const KState* ens_st;
     int n_tab;
+    int ens_grid;
 };
 
 // Vectors written by the fused Krylov start (FP64).

@@ -289,111 +289,118 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
 // never leaves registers and all 256 threads of a workgroup stay busy.  Same
 // fixed per-slot summation order as k_assemble.
 template <int D, int MAXR, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+__device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
     constexpr int K = D + 1;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int row_end = a.n;
-    if (a.tab != nullptr) {
-        const int4 tb = a.tab[blockIdx.x];
-        i = tb.x + threadIdx.x;
-        row_end = tb.y;
-        if (a.ens_st[tb.z].pdone) {   // uniform over the workgroup
-            if constexpr (MODE != ASM_INIT_NONE)
-                if (i < row_end) ki.x[i] = ki.x0[i];
-            return;
-        }
-    }
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (i < row_end) {
-        const int rs = a.row_ptr[i];
-        const int rl = a.row_ptr[i + 1] - rs;
-        const int dg = a.diag_pos[i] - rs;
-        double v[MAXR];
+    const int rs = a.row_ptr[i];
+    const int rl = a.row_ptr[i + 1] - rs;
+    const int dg = a.diag_pos[i] - rs;
+    double v[MAXR];
 #pragma unroll
-        for (int s = 0; s < MAXR; ++s) v[s] = 0.0;
-        double bi;
-        if (a.bc_flag != nullptr && a.bc_flag[i]) {
+    for (int s = 0; s < MAXR; ++s) v[s] = 0.0;
+    double bi;
+    if (a.bc_flag != nullptr && a.bc_flag[i]) {
 #pragma unroll
-            for (int s = 0; s < MAXR; ++s) v[s] = (s == dg) ? 1.0 : 0.0;
-            bi = a.bc_val[i];
-        } else {
-            bi = 0.0;
-            double msum = 0.0;
-            const int qe = a.inc_ptr[i + 1];
-            // NB incidences per batch: their loads are issued together (three
-            // dependent round trips per batch: incidence, element, node records)
-            for (int qb = a.inc_ptr[i]; qb < qe; qb += ASM_REG_BATCH) {
-                int2 ic[ASM_REG_BATCH];
-                int4 cn[ASM_REG_BATCH];
-                double4 nr[ASM_REG_BATCH][K];
-                double ks[ASM_REG_BATCH];
+        for (int s = 0; s < MAXR; ++s) v[s] = (s == dg) ? 1.0 : 0.0;
+        bi = a.bc_val[i];
+    } else {
+        bi = 0.0;
+        double msum = 0.0;
+        const int qe = a.inc_ptr[i + 1];
+        // NB incidences per batch: their loads are issued together (three
+        // dependent round trips per batch: incidence, element, node records)
+        for (int qb = a.inc_ptr[i]; qb < qe; qb += ASM_REG_BATCH) {
+            int2 ic[ASM_REG_BATCH];
+            int4 cn[ASM_REG_BATCH];
+            double4 nr[ASM_REG_BATCH][K];
+            double ks[ASM_REG_BATCH];
 #pragma unroll
-                for (int u = 0; u < ASM_REG_BATCH; ++u)
-                    ic[u] = (qb + u < qe) ? a.inc[qb + u] : make_int2(0, 0);
+            for (int u = 0; u < ASM_REG_BATCH; ++u)
+                ic[u] = (qb + u < qe) ? a.inc[qb + u] : make_int2(0, 0);
 #pragma unroll
-                for (int u = 0; u < ASM_REG_BATCH; ++u)
-                    cn[u] = (qb + u < qe) ? a.conn4[ic[u].x >> 2] : make_int4(0, 0, 0, 0);
+            for (int u = 0; u < ASM_REG_BATCH; ++u)
+                cn[u] = (qb + u < qe) ? a.conn4[ic[u].x >> 2] : make_int4(0, 0, 0, 0);
 #pragma unroll
-                for (int u = 0; u < ASM_REG_BATCH; ++u) {
-                    const int nd[4] = {cn[u].x, cn[u].y, cn[u].z, cn[u].w};
+            for (int u = 0; u < ASM_REG_BATCH; ++u) {
+                const int nd[4] = {cn[u].x, cn[u].y, cn[u].z, cn[u].w};
 #pragma unroll
-                    for (int c = 0; c < K; ++c)
-                        nr[u][c] = (qb + u < qe) ? a.nrec[nd[c]] : make_double4(0.0, 0.0, 0.0, 0.0);
-                    ks[u] = (a.Ks_elem != nullptr && qb + u < qe) ? a.Ks_elem[ic[u].x >> 2] : 1.0;
-                }
-#pragma unroll
-                for (int u = 0; u < ASM_REG_BATCH; ++u) {
-                    if (qb + u >= qe) break;
-                    double px[K], py[K], pz[K], kn[K];
-#pragma unroll
-                    for (int c = 0; c < K; ++c) {
-                        px[c] = nr[u][c].x;
-                        py[c] = nr[u][c].y;
-                        pz[c] = (D == 3) ? nr[u][c].z : 0.0;
-                        kn[c] = (D == 3) ? nr[u][c].w : nr[u][c].z;
-                    }
-                    double out[D + 3];
-                    element_compute<D>(a, px, py, pz, kn, ks[u], ic[u].x & 3, out);
-                    const uint32_t w = (uint32_t)ic[u].y;
-#pragma unroll
-                    for (int c = 0; c < K; ++c) {
-                        const uint32_t sl = (w >> (8 * c)) & 0xffu;
-#pragma unroll
-                        for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? out[c] : 0.0;
-                    }
-                    bi += out[K];
-                    msum += out[K + 1];
-                }
+                for (int c = 0; c < K; ++c)
+                    nr[u][c] = (qb + u < qe) ? a.nrec[nd[c]] : make_double4(0.0, 0.0, 0.0, 0.0);
+                ks[u] = (a.Ks_elem != nullptr && qb + u < qe) ? a.Ks_elem[ic[u].x >> 2] : 1.0;
             }
-            const double ci = a.C[i];
-            const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
-            const double madd = a.inv_dt * msum * ci;
 #pragma unroll
-            for (int s = 0; s < MAXR; ++s) v[s] += (s == dg) ? madd : 0.0;
-            bi -= a.inv_dt * msum * rhs;
-        }
-        double dgv = 0.0;
+            for (int u = 0; u < ASM_REG_BATCH; ++u) {
+                if (qb + u >= qe) break;
+                double px[K], py[K], pz[K], kn[K];
 #pragma unroll
-        for (int s = 0; s < MAXR; ++s) {
-            if (s < rl) {
-                a.A[rs + s] = v[s];
-                if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+                for (int c = 0; c < K; ++c) {
+                    px[c] = nr[u][c].x;
+                    py[c] = nr[u][c].y;
+                    pz[c] = (D == 3) ? nr[u][c].z : 0.0;
+                    kn[c] = (D == 3) ? nr[u][c].w : nr[u][c].z;
+                }
+                double out[D + 3];
+                element_compute<D>(a, px, py, pz, kn, ks[u], ic[u].x & 3, out);
+                const uint32_t w = (uint32_t)ic[u].y;
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    const uint32_t sl = (w >> (8 * c)) & 0xffu;
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? out[c] : 0.0;
+                }
+                bi += out[K];
+                msum += out[K + 1];
             }
-            if (s == dg) dgv = v[s];
         }
-        a.b[i] = bi;
-        const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
-        if (a.dinv != nullptr) a.dinv[i] = di;
-        if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
-        if constexpr (MODE != ASM_INIT_NONE) {
-            double ax = 0.0;
+        const double ci = a.C[i];
+        const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+        const double madd = a.inv_dt * msum * ci;
 #pragma unroll
-            for (int s = 0; s < MAXR; ++s)
-                if (s < rl) ax += v[s] * ki.x0[a.cols[rs + s]];
-            krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
-        }
+        for (int s = 0; s < MAXR; ++s) v[s] += (s == dg) ? madd : 0.0;
+        bi -= a.inv_dt * msum * rhs;
     }
-    if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow);
+    double dgv = 0.0;
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) {
+        if (s < rl) {
+            a.A[rs + s] = v[s];
+            if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+        }
+        if (s == dg) dgv = v[s];
+    }
+    a.b[i] = bi;
+    const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
+    if (a.dinv != nullptr) a.dinv[i] = di;
+    if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
+    if constexpr (MODE != ASM_INIT_NONE) {
+        double ax = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s)
+            if (s < rl) ax += v[s] * ki.x0[a.cols[rs + s]];
+        krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
+    }
+}
+
+template <int D, int MAXR, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+    if (a.tab == nullptr) {
+        const int i = blockIdx.x * BLOCK + threadIdx.x;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (i < a.n) assemble_row<D, MAXR, MODE>(a, ki, i, acc);
+        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow);
+        return;
+    }
+    // ensemble: fixed grid over the listed table entries (instances still
+    // iterating Picard), partial sums in the entry's column
+    const int cnt = a.act ? a.act[0] : a.n_tab;
+    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+        const int t = a.act ? a.act[1 + v] : v;
+        const int4 tb = a.tab[t];
+        if (a.ens_st[tb.z].pdone) continue;
+        const int i = tb.x + threadIdx.x;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (i < tb.y) assemble_row<D, MAXR, MODE>(a, ki, i, acc);
+        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
+    }
 }
 
 void compute_quad_const(int dim, QuadConst& qc) {
@@ -474,7 +481,7 @@ int asm_reg_max_row(int dim) { return dim == 2 ? ASM_REG_ROW_2D : ASM_REG_ROW_3D
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg) {
     if (use_reg) {
-        const int grid = a.tab ? a.n_tab : (a.n + BLOCK - 1) / BLOCK;
+        const int grid = a.tab ? a.ens_grid : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
         return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
     }

@@ -35,9 +35,8 @@ struct KVec {
 // tflag (nullable): the benchmark's launch-timing slot; receives the number of
 // rows the launch worked on (skipped instances count none), so timed averages
 // count working launches only and weigh them by their rows.
-// tab (nullable): ensemble mode; workgroup b owns rows [tab[b].x, tab[b].y)
-// and the state red.st[tab[b].z] of their instance.  Otherwise the grid
-// strides over all n rows with the state red.st[0].
+// ew: ensemble walk (ew.tab null: one system, the grid strides over all n
+// rows with the state red.st[0]).
 //
 // SPMV_UNROLL rows per lane group and pass: their row pointers, then their
 // first LANES entries (values + columns), then the x gathers are each issued
@@ -45,38 +44,20 @@ struct KVec {
 // SPMV_UNROLL rows instead of for one.
 constexpr int SPMV_UNROLL = 4;
 
+// Rows r0, r0 + stride, ... below rend (this lane group's rows).
 template <int LANES, class T, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict__ rp,
-                                                 const int32_t* __restrict__ ci,
-                                                 const T* __restrict__ av, Epi epi, Red red, int* tflag,
-                                                 const RowTab* __restrict__ tab) {
-    const RowTab tb = tab ? tab[blockIdx.x] : make_int4(0, n, 0, 0);
-    KState* st = red.st + tb.z;
-    if (epi.skip(st)) return;
-    if (tflag != nullptr && threadIdx.x == 0) {
-        if (!tab) {
-            if (blockIdx.x == 0) *tflag = n;
-        } else if (tb.w > 0) {
-            atomicAdd(tflag, tb.w);    // first workgroup of an instance: its row count
-        }
-    }
-    constexpr int GPB = BLOCK / LANES;
-    constexpr int ND = Epi::ND;
+__device__ inline void spmv_rows(int r0, int rend, int stride, const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci, const T* __restrict__ av, const Epi& epi,
+                                 double* acc) {
     constexpr int U = SPMV_UNROLL;
-    const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
-    double acc[ND > 0 ? ND : 1];
-#pragma unroll
-    for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
-    epi.load(st);
-    const int stride = tab ? GPB : gridDim.x * GPB;
-    n = tb.y;
-    for (int r0 = (tab ? tb.x : blockIdx.x * GPB) + g; r0 < n; r0 += stride * U) {
+    const int l = threadIdx.x % LANES;
+    for (; r0 < rend; r0 += stride * U) {
         int kb[U], ke[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int row = r0 + u * stride;
-            kb[u] = row < n ? rp[row] : 0;
-            ke[u] = row < n ? rp[row + 1] : 0;
+            kb[u] = row < rend ? rp[row] : 0;
+            ke[u] = row < rend ? rp[row + 1] : 0;
         }
         int col[U];
         double val[U];
@@ -97,38 +78,76 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
 #pragma unroll
             for (int o = LANES / 2; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, LANES);
             const int row = r0 + u * stride;
-            if (l == 0 && row < n) epi.row(row, s[u], acc);
+            if (l == 0 && row < rend) epi.row(row, s[u], acc);
         }
     }
-    if constexpr (ND > 0) {
-        double v[ND];
+}
+
+template <int LANES, class T, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const T* __restrict__ av, Epi epi, Red red, int* tflag,
+                                                 EnsWalk ew) {
+    constexpr int GPB = BLOCK / LANES;
+    constexpr int ND = Epi::ND;
+    constexpr int NA = ND > 0 ? ND : 1;
+    const int g = threadIdx.x / LANES;
+    if (ew.tab == nullptr) {
+        if (epi.skip(red.st)) return;
+        if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
+        double acc[NA];
 #pragma unroll
-        for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        store_partials<ND>(v, red.partials, red.prow);
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        epi.load(red.st);
+        spmv_rows<LANES, T>(blockIdx.x * GPB + g, n, gridDim.x * GPB, rp, ci, av, epi, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
+        return;
+    }
+    const int cnt = ew.count();
+    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+        const int t = ew.entry(v);
+        const RowTab tb = ew.tab[t];
+        KState* st = red.st + tb.z;
+        if (epi.skip(st)) continue;
+        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
+        Epi e = epi;
+        e.load(st);
+        double acc[NA];
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        spmv_rows<LANES, T>(tb.x + g, tb.y, GPB, rp, ci, av, e, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow, t);
     }
 }
 
 // Elementwise kernel with fused dot products.
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, const RowTab* __restrict__ tab) {
-    const RowTab tb = tab ? tab[blockIdx.x] : make_int4(0, n, 0, 0);
-    KState* st = red.st + tb.z;
-    if (op.skip(st)) return;
+__global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew) {
     constexpr int ND = Op::ND;
-    double acc[ND > 0 ? ND : 1];
+    constexpr int NA = ND > 0 ? ND : 1;
+    if (ew.tab == nullptr) {
+        if (op.skip(red.st)) return;
+        double acc[NA];
 #pragma unroll
-    for (int j = 0; j < (ND > 0 ? ND : 1); ++j) acc[j] = 0.0;
-    op.load(st);
-    if (tab) {
-        for (int i = tb.x + threadIdx.x; i < tb.y; i += BLOCK) op.row(i, acc);
-    } else {
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        op.load(red.st);
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) op.row(i, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
+        return;
     }
-    if constexpr (ND > 0) {
-        double v[ND];
+    const int cnt = ew.count();
+    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+        const int t = ew.entry(v);
+        const RowTab tb = ew.tab[t];
+        KState* st = red.st + tb.z;
+        if (op.skip(st)) continue;
+        Op o = op;
+        o.load(st);
+        double acc[NA];
 #pragma unroll
-        for (int j = 0; j < ND; ++j) v[j] = acc[j];
-        store_partials<ND>(v, red.partials, red.prow);
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        for (int i = tb.x + threadIdx.x; i < tb.y; i += BLOCK) o.row(i, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow, t);
     }
 }
 

@@ -90,22 +90,34 @@ struct Red {
     KState* st;           // solver state (ensemble: array indexed by instance)
 };
 
-// Ensemble row table: workgroup b owns rows [x, y) of instance z; w is the
-// instance's row count in its first workgroup's entry, 0 in the others.
-// The workgroups of one instance are consecutive; iblk[i] is the first of
-// instance i (iblk[n_inst] = number of workgroups).
+// Ensemble row table: entry t covers rows [x, y) of instance z; w is the
+// instance's row count in its first entry, 0 in the others.  The entries of
+// one instance are consecutive; iblk[i] is the first of instance i
+// (iblk[n_inst] = number of entries).  Partial sums of entry t go to column t.
 using RowTab = int4;
 
-// Store this workgroup's ND block sums in column blockIdx of the partials
-// (NW = wavefronts per workgroup).
+// How an ensemble launch walks the table: a fixed grid strides over the
+// entries listed in act (act[0] = count, act[1..count] = entry indices,
+// rebuilt on the device as instances converge), or over all n_tab entries
+// when act is null.  tab == nullptr: not an ensemble launch.
+struct EnsWalk {
+    const RowTab* tab;
+    const int* act;
+    int n_tab;
+    __device__ int count() const { return act ? act[0] : n_tab; }
+    __device__ int entry(int v) const { return act ? act[1 + v] : v; }
+};
+
+// Store this workgroup's ND block sums in column bid of the partials
+// (default: the workgroup's index; NW = wavefronts per workgroup).
 template <int ND, int NW = NWAVE>
-__device__ inline void store_partials(double (&v)[ND], double* partials, int prow) {
+__device__ inline void store_partials(double (&v)[ND], double* partials, int prow, int bid = -1) {
     __shared__ double sh[ND * NW];
     block_sum<ND, NW>(v, sh);
     if (threadIdx.x == 0) {
-        const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+        const unsigned b = bid >= 0 ? (unsigned)bid : blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
-        for (int j = 0; j < ND; ++j) partials[(size_t)j * prow + bid] = v[j];
+        for (int j = 0; j < ND; ++j) partials[(size_t)j * prow + b] = v[j];
     }
 }
 
