@@ -221,7 +221,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": f"this build runs on one GPU; --gpus {args.gpus} is not supported "
-                                   "(the ensemble/mesh sharding design is described in DESIGN.md)"}))
+                                   "(multi-GPU sharding of meshes and ensembles is a design, not built)"}))
         sys.exit(2)
     if args.workload == "c3":
         run_ensemble(args)

@@ -30,6 +30,16 @@
  *   wspm_richards_*        src/solvers/richards_solver_v2.py:107-168 and
  *                          src/solvers/richards_solver_3D_v2.py:177-222
  *                          (Picard loop of one implicit time step)
+ *   wspm_edge_set,         src/models/boundary.py:108-173 (Neumann, free
+ *   wspm_richards_edges    drainage of water), :217-277 (Cauchy), :336-421
+ *                          (free drainage of nitrogen): edge terms on
+ *                          consecutive boundary-list pairs
+ *   wspm_transport_fields  src/models/transport_utilities.py:119-192
+ *   wspm_assemble_transport src/numerics/assembly_solute.py:10-186,
+ *                          src/numerics/assembly_nitrogen.py:33-607 with the
+ *                          lumping / mass products / M/dt + K of
+ *                          src/solvers/nitrogen_solver.py:36-66,148-304 and
+ *                          src/solvers/coupled_solver_v2.py:154-200
  *   wspm_ens_*             no single counterpart: the reference runs one
  *                          instance per process (src/cli.py:87-150); an
  *                          ensemble context time-steps many independent
@@ -165,6 +175,41 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
  * summed event-to-event duration (ms) and the rows they worked on (rows may
  * be NULL; an ensemble launch skips the rows of converged instances). */
 int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
+
+/* Transport (2D) -------------------------------------------------------------
+ * Boundary node lists for edge terms, in the reference's list order: an edge
+ * joins consecutive entries, its length is x_J - x_I.  Consecutive entries must
+ * be joined by a mesh edge (their matrix entries live in the CSR pattern). */
+enum { WSPM_EDGE_INLET = 0, WSPM_EDGE_OUTLET = 1 };
+int32_t wspm_edge_set(wspm_ctx* ctx, int32_t list, int32_t n, const int32_t* nodes);
+/* Richards steps of this context add -flux * int N on the inlet list
+ * (Neumann) and/or -int K N on the outlet list (free drainage) to b. */
+int32_t wspm_richards_edges(wspm_ctx* ctx, int32_t inlet_neumann, double neumann_flux,
+                            int32_t outlet_freedrain);
+/* Node fields for transport from h (dev arrays [n]): water fluxes qx, qz, |q|,
+ * dispersion Dxx, Dxz, Dzz and theta; van Genuchten phi (host double[6]). */
+int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, double DL,
+                              double DT, double Dm, double* qx, double* qz, double* absq,
+                              double* dxx, double* dxz, double* dzz, double* theta);
+typedef struct {
+    int32_t lumped;           /* 1: row-sum lumped mass (nitrogen), 0: consistent (solute) */
+    int32_t inlet_cauchy;     /* add Cauchy terms on the inlet list                      */
+    int32_t outlet_freedrain; /* add -K/theta free-drainage terms on the outlet list     */
+    int32_t reserved;
+    double dt;
+    double mu;                /* + mu int theta N N^T in the stiffness (reaction)         */
+    double rho_kd;            /* - rho Kd int N N^T in the mass (NH4 sorption)            */
+    double mu_src;            /* source -mu_src int theta c_src N (coupled species)       */
+    double cauchy_flux, c_inlet;
+} wspm_transport_params;
+/* One species system A = M/dt + K (+ edge terms), b = M c_n / dt + source (+
+ * Cauchy), on the context's pattern.  Fields from wspm_transport_fields; K
+ * (dev [n]) only for free drainage; c_src (dev [n]) or NULL; dinv may be NULL. */
+int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp,
+                                const double* theta, const double* qx, const double* qz,
+                                const double* dxx, const double* dxz, const double* dzz,
+                                const double* K, const double* c_n, const double* c_src,
+                                double* A_vals, double* b, double* dinv);
 
 /* Ensembles ------------------------------------------------------------------
  * n_inst independent 2D instances (own mesh, soil parameters, Dirichlet rows)

@@ -44,6 +44,14 @@ class RichardsParams(C.Structure):
                 ("inner_tol", C.c_double)]
 
 
+class TransportParams(C.Structure):
+    _fields_ = [("lumped", C.c_int32), ("inlet_cauchy", C.c_int32), ("outlet_freedrain", C.c_int32),
+                ("reserved", C.c_int32), ("dt", C.c_double), ("mu", C.c_double), ("rho_kd", C.c_double),
+                ("mu_src", C.c_double), ("cauchy_flux", C.c_double), ("c_inlet", C.c_double)]
+
+
+EDGE = {"inlet": 0, "outlet": 1}
+
 _P = C.c_void_p
 _I = C.c_int32
 _D = C.c_double
@@ -69,6 +77,11 @@ PROTOTYPES = {
     "wspm_richards_state": (_I, [_P, _P, _P, _P]),
     "wspm_timing_enable": (_I, [_P, _I]),
     "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "wspm_edge_set": (_I, [_P, _I, _I, _P]),
+    "wspm_richards_edges": (_I, [_P, _I, _D, _I]),
+    "wspm_transport_fields": (_I, [_P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
+    "wspm_assemble_transport": (_I, [_P, C.POINTER(TransportParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _P]),
     "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),

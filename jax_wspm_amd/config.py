@@ -61,6 +61,31 @@ class SoluteParameters:
 
 
 @dataclass
+class NitrogenParameters:
+    """config/settings.py:52-83."""
+    DL: float = 0.04
+    DT: float = 0.0004
+    Dm: float = 0.288e-4
+    rho: float = 1.6e-6
+    Kd: float = 0.34e-6
+    mu1: float = 0.12
+    mu2: float = 0.048
+    mu3: float = 0.012
+    c_init_NH4: float = 0.0
+    c_init_NO2: float = 0.0
+    c_init_NO3: float = 0.0
+    c_inlet_NH4: float = 50.0
+    c_inlet_NO2: float = 0.0
+    c_inlet_NO3: float = 20.0
+
+    def to_array(self) -> np.ndarray:
+        return np.array([self.DL, self.DT, self.Dm, self.rho, self.Kd, self.mu1, self.mu2, self.mu3])
+
+    def to_dict(self) -> Dict:
+        return asdict(self)
+
+
+@dataclass
 class TimeSteppingParameters:
     """config/settings.py:85-97."""
     dt_init: float = 1e-5
@@ -70,7 +95,7 @@ class TimeSteppingParameters:
     W_it: int = 10
     lambda_amp: float = 2.0
     lambda_red: float = 0.5
-    nsteps: Optional[int] = None     # fixed-dt mode (BASELINE-style runs); None = adaptive
+    nsteps: Optional[int] = None     # fixed-dt mode: exactly nsteps steps; None = adaptive to Tmax
 
     def to_dict(self) -> Dict:
         return asdict(self)
@@ -169,6 +194,7 @@ class SimulationConfig:
     exponential: ExponentialParameters = field(default_factory=ExponentialParameters)
     van_genuchten: VanGenuchtenParameters = field(default_factory=VanGenuchtenParameters)
     solute: SoluteParameters = field(default_factory=SoluteParameters)
+    nitrogen: NitrogenParameters = field(default_factory=NitrogenParameters)
     time: TimeSteppingParameters = field(default_factory=TimeSteppingParameters)
     solver: SolverParameters = field(default_factory=SolverParameters)
     test_case: str = "Test1"
@@ -185,6 +211,7 @@ class SimulationConfig:
             "exponential": self.exponential.to_dict(),
             "van_genuchten": self.van_genuchten.to_dict(),
             "solute": self.solute.to_dict(),
+            "nitrogen": self.nitrogen.to_dict(),
             "time": self.time.to_dict(),
             "solver": self.solver.to_dict(),
             "mesh": self.mesh.to_dict(),
@@ -202,6 +229,7 @@ def create_config_from_dict(d: Dict[str, Any]) -> SimulationConfig:
         exponential=ExponentialParameters(**d.get("exponential", {})),
         van_genuchten=VanGenuchtenParameters(**d.get("van_genuchten", {})),
         solute=SoluteParameters(**d.get("solute", {})),
+        nitrogen=NitrogenParameters(**d.get("nitrogen", {})),
         time=TimeSteppingParameters(**d.get("time", {})),
         solver=SolverParameters(**d.get("solver", {})),
         test_case=d.get("test_case", "Test1"),

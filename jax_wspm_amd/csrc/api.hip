@@ -17,6 +17,7 @@
 
 #include "assemble.h"
 #include "krylov_kernels.h"
+#include "transport.h"
 #include "wspm_device.h"
 #include "wspm_internal.h"
 
@@ -430,6 +431,14 @@ struct wspm_ctx {
     int* kact = nullptr;            // [1 + n_tab] entries of instances whose Krylov solve runs
     int* pact = nullptr;            // [1 + n_tab] entries of instances still iterating Picard
     int ens_grid = 0;               // workgroups of an ensemble launch
+    // boundary node lists for edge terms: [0] inlet (top), [1] outlet (bottom)
+    struct EdgeList {
+        int cnt = 0;
+        int32_t *nodes = nullptr, *slot_prev = nullptr, *slot_next = nullptr;
+    } edges[2];
+    bool rich_neumann = false, rich_freedrain = false;
+    double rich_flux = 0.0;
+    TrQuad tq{};
     std::vector<void*> allocs;
     std::vector<void*> host_allocs;
 };
@@ -847,6 +856,7 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         ctx->H = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
         ctx->H32 = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
         compute_quad_const(dim, ctx->qc);
+        compute_tr_quad(ctx->tq);
         HIPCHK(hipDeviceSynchronize());
     } catch (const Err&) {
         g_create_error = ctx->err;
@@ -924,6 +934,30 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
+}
+
+// Edge terms of one boundary list on (A, b) (see transport.h).
+static void launch_edge_list(wspm_ctx* ctx, int list, int kind, double flux, double c_in, const double* K,
+                             int kstride, const double* th, double* A, double* b, double* dinv) {
+    const auto& L = ctx->edges[list];
+    if (L.cnt < 2) return;
+    EdgeArgs e{};
+    e.cnt = L.cnt;
+    e.nodes = L.nodes;
+    e.slot_prev = L.slot_prev;
+    e.slot_next = L.slot_next;
+    e.diag_pos = ctx->diag_pos;
+    e.nrec = ctx->nrec;
+    e.kind = kind;
+    e.flux = flux;
+    e.c_in = c_in;
+    e.K = K;
+    e.kstride = kstride;
+    e.th = th;
+    e.A = A;
+    e.b = b;
+    e.dinv = dinv;
+    HIPCHK(launch_edges(e, ctx->stream));
 }
 
 // Assembly (+ Dirichlet rows + Jacobi diagonal); with mode != ASM_INIT_NONE
@@ -1201,8 +1235,11 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     int it = 0, kry = 0;
     double err = INFINITY;
+    // edge terms change b after the assembly, so the solver starts on its own
+    const bool edges = ctx->rich_neumann || ctx->rich_freedrain;
     while (err >= P.picard_tol && it < P.picard_maxit) {
-        const int mode = mixed ? ASM_INIT_RESID
+        const int mode = edges ? ASM_INIT_NONE
+                         : mixed ? ASM_INIT_RESID
                          : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG
                          : P.method == WSPM_CG ? ASM_INIT_CG : ASM_INIT_GMRES;
         if (mixed) {
@@ -1212,12 +1249,18 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
                     mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
+        if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
+            launch_edge_list(ctx, 0, EDGE_NEUMANN, ctx->rich_flux, 0.0, nullptr, 0, nullptr, nullptr, ctx->bv,
+                             nullptr);
+        if (ctx->rich_freedrain)   // nitrogen_solver.py:121-125, K(h_m) from the node records
+            launch_edge_list(ctx, 1, EDGE_FREEDRAIN_WATER, 0.0, 0.0, krec_of(ctx), 4, nullptr, nullptr, ctx->bv,
+                             nullptr);
         wspm_krylov_stats st{};
         const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                            P.picard_mode == WSPM_PICARD_REL};
         if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
-                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, true);
+                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !edges);
             launch_vec(ctx, pop, 0);
             poll(ctx, 0);
         } else {
@@ -1225,7 +1268,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             // error behind them, then read both back with one synchronisation;
             // if the solve needed more iterations, finish it and redo the error.
             KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
-                                  P.tol, P.atol, P.maxiter, P.restart, 0, true);
+                                  P.tol, P.atol, P.maxiter, P.restart, 0, !edges);
             run.enqueue(run.first_chunk());
             launch_vec(ctx, pop, 0);
             poll(ctx, 0);
@@ -1307,6 +1350,119 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int
     API_END
 }
 
+
+// ---------------------------------------------------------------------------
+// Transport: edge lists, flux/dispersion fields, species systems
+// ---------------------------------------------------------------------------
+int32_t wspm_edge_set(wspm_ctx* ctx, int32_t list, int32_t n, const int32_t* nodes) {
+    if (!ctx) return WSPM_EINVAL;
+    if (list != WSPM_EDGE_INLET && list != WSPM_EDGE_OUTLET) return fail(ctx, WSPM_EINVAL, "unknown edge list");
+    if (n < 0 || (n > 0 && !nodes)) return fail(ctx, WSPM_EINVAL, "bad edge node array");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "edge terms are 2D");
+    const HostPattern& P = ctx->hp;
+    auto pos = [&](int i, int j) -> int {
+        const auto b = P.cols.begin() + P.row_ptr[i], e = P.cols.begin() + P.row_ptr[i + 1];
+        const auto it = std::lower_bound(b, e, j);
+        return (it != e && *it == j) ? (int)(it - P.cols.begin()) : -1;
+    };
+    std::vector<int32_t> nd(nodes, nodes + n), sp(n, -1), sn(n, -1);
+    std::vector<uint8_t> seen(ctx->n, 0);
+    for (int k = 0; k < n; ++k) {
+        if (nd[k] < 0 || nd[k] >= ctx->n) return fail(ctx, WSPM_EINVAL, "edge node out of range");
+        if (seen[nd[k]]) return fail(ctx, WSPM_EINVAL, "edge node listed twice");
+        seen[nd[k]] = 1;
+    }
+    for (int k = 0; k < n; ++k) {
+        if (k > 0) sp[k] = pos(nd[k], nd[k - 1]);
+        if (k + 1 < n) sn[k] = pos(nd[k], nd[k + 1]);
+        if ((k > 0 && sp[k] < 0) || (k + 1 < n && sn[k] < 0))
+            return fail(ctx, WSPM_EINVAL, "consecutive edge nodes " + std::to_string(nd[k]) +
+                                              " are not joined by a mesh edge");
+    }
+    API_BEGIN
+    auto& L = ctx->edges[list];
+    if (n > L.cnt || !L.nodes) {
+        L.nodes = dalloc<int32_t>(ctx, std::max(n, 1));
+        L.slot_prev = dalloc<int32_t>(ctx, std::max(n, 1));
+        L.slot_next = dalloc<int32_t>(ctx, std::max(n, 1));
+    }
+    if (n > 0) {
+        HIPCHK(hipMemcpy(L.nodes, nd.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.slot_prev, sp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.slot_next, sn.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    L.cnt = n;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_richards_edges(wspm_ctx* ctx, int32_t inlet_neumann, double neumann_flux, int32_t outlet_freedrain) {
+    if (!ctx) return WSPM_EINVAL;
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    ctx->rich_neumann = inlet_neumann != 0;
+    ctx->rich_flux = neumann_flux;
+    ctx->rich_freedrain = outlet_freedrain != 0;
+    return WSPM_OK;
+}
+
+int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, double DL, double DT, double Dm,
+                              double* qx, double* qz, double* absq, double* dxx, double* dxz, double* dzz,
+                              double* theta) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!phi || !h || !qx || !qz || !absq || !dxx || !dxz || !dzz || !theta)
+        return fail(ctx, WSPM_EINVAL, "null argument");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "transport is 2D");
+    API_BEGIN
+    FieldArgs a{};
+    a.n = ctx->n;
+    a.inc_ptr = ctx->inc_ptr;
+    a.inc = ctx->inc;
+    a.conn4 = ctx->conn4;
+    a.nrec = ctx->nrec;
+    a.h = h;
+    a.sp = make_soil(WSPM_SOIL_VAN_GENUCHTEN, phi);
+    a.DL = DL; a.DT = DT; a.Dm = Dm;
+    a.qx = qx; a.qz = qz; a.aq = absq; a.dxx = dxx; a.dxz = dxz; a.dzz = dzz; a.th = theta;
+    HIPCHK(launch_transport_fields(a, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp, const double* theta,
+                                const double* qx, const double* qz, const double* dxx, const double* dxz,
+                                const double* dzz, const double* K, const double* c_n, const double* c_src,
+                                double* A_vals, double* b, double* dinv) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!tp || !theta || !qx || !qz || !dxx || !dxz || !dzz || !c_n || !A_vals || !b)
+        return fail(ctx, WSPM_EINVAL, "null argument");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "transport is 2D");
+    if (!(tp->dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
+    if (ctx->max_row > 16) return fail(ctx, WSPM_EINVAL, "rows longer than 16 entries");
+    if (tp->outlet_freedrain && !K) return fail(ctx, WSPM_EINVAL, "free drainage needs K");
+    API_BEGIN
+    TrArgs a{};
+    a.n = ctx->n;
+    a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos;
+    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
+    a.th = theta; a.qx = qx; a.qz = qz; a.dxx = dxx; a.dxz = dxz; a.dzz = dzz;
+    a.c_n = c_n;
+    a.c_src = c_src;
+    a.mu = tp->mu; a.rho_kd = tp->rho_kd; a.mu_src = tp->mu_src;
+    a.inv_dt = 1.0 / tp->dt;
+    a.lumped = tp->lumped != 0;
+    a.tq = ctx->tq;
+    a.A = A_vals; a.b = b; a.dinv = dinv;
+    HIPCHK(launch_assemble_transport(a, ctx->max_row, ctx->stream));
+    if (tp->inlet_cauchy)   // src/models/boundary.py:217-277
+        launch_edge_list(ctx, WSPM_EDGE_INLET, EDGE_CAUCHY, tp->cauchy_flux, tp->c_inlet, nullptr, 0, nullptr,
+                         A_vals, b, dinv);
+    if (tp->outlet_freedrain)   // boundary.py:336-421
+        launch_edge_list(ctx, WSPM_EDGE_OUTLET, EDGE_FREEDRAIN_N, 0.0, 0.0, K, 1, theta, A_vals, nullptr, dinv);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
 
 // ---------------------------------------------------------------------------
 // Ensemble of independent instances

@@ -171,6 +171,53 @@ class Context:
         self._ck(self.lib.wspm_richards_state(self.h, ptr(h), ptr(th), ptr(thn)))
         return h, th, thn
 
+    # -- transport ---------------------------------------------------------------------
+    def edge_set(self, which: str, nodes):
+        """Boundary node list ("inlet" / "outlet") for edge terms, in list order."""
+        nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32).ravel())
+        self._ck(self.lib.wspm_edge_set(self.h, _lib.EDGE[which], int(nd.size), ptr(nd)))
+
+    def richards_edges(self, neumann_flux: Optional[float] = None, freedrain: bool = False):
+        """Neumann flux on the inlet list and/or free drainage on the outlet list
+        in every following Richards step."""
+        self._ck(self.lib.wspm_richards_edges(self.h, 0 if neumann_flux is None else 1,
+                                              float(neumann_flux or 0.0), 1 if freedrain else 0))
+
+    def transport_fields(self, phi, h, DL, DT, Dm):
+        """(qx, qz, |q|, Dxx, Dxz, Dzz, theta) at the nodes (device tensors)."""
+        h = _f64(h)
+        p = np.zeros(6)
+        pa = np.asarray(phi, dtype=np.float64)
+        p[: pa.size] = pa
+        out = [torch.empty_like(h) for _ in range(7)]
+        self._ck(self.lib.wspm_transport_fields(self.h, ptr(p), ptr(h), float(DL), float(DT), float(Dm),
+                                                *(ptr(o) for o in out)))
+        return tuple(out)
+
+    def assemble_transport(self, fields, c_n, dt, *, lumped, mu=0.0, rho_kd=0.0, mu_src=0.0, c_src=None,
+                           cauchy=None, K=None, want_dinv=True):
+        """One species system on the context's pattern.  fields: the 7-tuple of
+        transport_fields; cauchy: (flux, c_inlet) on the inlet list; K: enables
+        free drainage on the outlet list."""
+        qx, qz, _, dxx, dxz, dzz, th = (_f64(f) for f in fields)
+        tp = _lib.TransportParams()
+        tp.lumped = 1 if lumped else 0
+        tp.inlet_cauchy = 1 if cauchy is not None else 0
+        tp.outlet_freedrain = 1 if K is not None else 0
+        tp.dt, tp.mu, tp.rho_kd, tp.mu_src = float(dt), float(mu), float(rho_kd), float(mu_src)
+        if cauchy is not None:
+            tp.cauchy_flux, tp.c_inlet = float(cauchy[0]), float(cauchy[1])
+        c_n = _f64(c_n)
+        cs = _f64(c_src) if c_src is not None else None
+        Kd = _f64(K) if K is not None else None
+        A = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+        b = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        dinv = torch.empty(self.n, dtype=torch.float64, device=self.device) if want_dinv else None
+        self._ck(self.lib.wspm_assemble_transport(self.h, C.byref(tp), ptr(th), ptr(qx), ptr(qz), ptr(dxx),
+                                                  ptr(dxz), ptr(dzz), ptr(Kd), ptr(c_n), ptr(cs), ptr(A), ptr(b),
+                                                  ptr(dinv)))
+        return A, b, dinv
+
     # -- timing hooks ---------------------------------------------------------------
     def timing(self, on: bool):
         self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
