@@ -40,6 +40,9 @@
  *                          lumping / mass products / M/dt + K of
  *                          src/solvers/nitrogen_solver.py:36-66,148-304 and
  *                          src/solvers/coupled_solver_v2.py:154-200
+ *   wspm_adj_step          no counterpart (the reference has no gradient code):
+ *                          one backward step of the discrete adjoint of
+ *                          wspm_richards_step
  *   wspm_ens_*             no single counterpart: the reference runs one
  *                          instance per process (src/cli.py:87-150); an
  *                          ensemble context time-steps many independent
@@ -210,6 +213,28 @@ int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp,
                                 const double* dxx, const double* dxz, const double* dzz,
                                 const double* K, const double* c_n, const double* c_src,
                                 double* A_vals, double* b, double* dinv);
+
+/* Discrete adjoint (2D, van Genuchten, per-element Ks) -------------------------
+ * Step k solved F_k(h_k, h_{k-1}; p) = 0 (the converged Picard fixed point of
+ * wspm_richards_step; m = 1 - 1/n).  One call does, for a loss L:
+ *   J_k^T lam_k = -dL/dh_k + ML dtheta/dh(h_k) lam_{k+1} / dt   (Jacobi-BiCGSTAB)
+ *   grad_logks[e] += lam_k . dF_k/dlogKs_e,  grad_an[0..1] += lam_k . dF_k/d(alpha, n)
+ * plus the observation terms of L = l(theta(h_k)[obs]): dL/dh_k and the direct
+ * dtheta/d(alpha, n) parts, from dldy = dl/dy (y = theta(h_k)[obs]).
+ * Device pointers: Ks_elem [E], h = h_k, h_prev = h_{k-1}, obs [n_obs] int32,
+ * dldy [n_obs] (both NULL when n_obs = 0), lam_next [n] (NULL for the last
+ * step), lam [n] out, grad_logks [E] and grad_an [2] accumulated. */
+typedef struct {
+    double dt;
+    double phi[6];        /* alpha, theta_s, theta_r, n, (m = 1 - 1/n), (Ks: Ks_elem) */
+    double tol;           /* relative tolerance of the transpose solve           */
+    int32_t maxiter;
+    int32_t n_obs;
+} wspm_adj_params;
+int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* Ks_elem,
+                      const double* h, const double* h_prev, const int32_t* obs,
+                      const double* dldy, const double* lam_next, double* lam,
+                      double* grad_logks, double* grad_an, wspm_krylov_stats* stats);
 
 /* Ensembles ------------------------------------------------------------------
  * n_inst independent 2D instances (own mesh, soil parameters, Dirichlet rows)

@@ -50,6 +50,11 @@ class TransportParams(C.Structure):
                 ("mu_src", C.c_double), ("cauchy_flux", C.c_double), ("c_inlet", C.c_double)]
 
 
+class AdjParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("phi", C.c_double * 6), ("tol", C.c_double), ("maxiter", C.c_int32),
+                ("n_obs", C.c_int32)]
+
+
 EDGE = {"inlet": 0, "outlet": 1}
 
 _P = C.c_void_p
@@ -82,6 +87,7 @@ PROTOTYPES = {
     "wspm_transport_fields": (_I, [_P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "wspm_assemble_transport": (_I, [_P, C.POINTER(TransportParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      _P]),
+    "wspm_adj_step": (_I, [_P, C.POINTER(AdjParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(KrylovStats)]),
     "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),

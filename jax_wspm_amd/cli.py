@@ -94,9 +94,6 @@ def _history(v):
 
 def run(args):
     cfg = build_config(args)
-    if args.test_case in ("SoluteTest", "NitrogenTest"):
-        raise NotImplementedError(f"{args.test_case}: the coupled solute/nitrogen solvers are not on the "
-                                  "MI355X path yet")
     if args.solver == "direct":
         raise NotImplementedError("solver 'direct' (dense LU) is not on the MI355X path")
     if args.preconditioner not in ("none", "jacobi"):
@@ -111,7 +108,11 @@ def run(args):
     print("Starting simulation with configuration:")
     print(f"Test case: {args.test_case}")
     print(f"Solver: {args.solver} with {args.preconditioner} preconditioner")
-    cls = RichardsSolver3D if args.test_case == "Test3D" else RichardsSolver
+    if args.test_case in ("SoluteTest", "NitrogenTest"):
+        from .transport import CoupledSolver, NitrogenSolver
+        cls = CoupledSolver if args.test_case == "SoluteTest" else NitrogenSolver
+    else:
+        cls = RichardsSolver3D if args.test_case == "Test3D" else RichardsSolver
     solver = cls(cfg, mesh_dir=args.mesh_dir, history=_history(args.history))
     results = solver.solve()
     solver.ctx.close()
@@ -134,6 +135,13 @@ def run(args):
         "elements": results["tetrahedra"] if "tetrahedra" in results else results["triangles"],
         "krylov_iterations": results["krylov_iterations"],
     }
+    # src/cli.py:167-179; the reference's "solute": None entries are left out
+    # (an object array would need allow_pickle to load)
+    if args.test_case == "SoluteTest":
+        save["solute"] = results["solute"]
+    elif args.test_case == "NitrogenTest":
+        for k in ("NH4_concentration", "NO2_concentration", "NO3_concentration"):
+            save[k] = results[k]
     if args.test_case in ("Test1", "Test2", "Test3"):
         for k in ("l2_pressure", "linf_pressure", "l2_saturation", "linf_saturation", "l2_relative_pressure",
                   "l2_relative_saturation"):

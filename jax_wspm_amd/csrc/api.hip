@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "assemble.h"
+#include "adjoint.h"
 #include "krylov_kernels.h"
 #include "transport.h"
 #include "wspm_device.h"
@@ -360,6 +361,24 @@ __global__ void k_nan_check(KState* s, const double* x, int n) {
     }
 }
 
+// Adjoint parameter reduction: out[0..1] += (sum a[0:len], sum a[len:2 len]).
+struct AdjSum {
+    static constexpr int ND = 2;
+    const double* a;
+    int len;
+    double* out;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        acc[0] += a[i];
+        acc[1] += a[len + i];
+    }
+    __device__ void fin(const double* t, KState*) const {
+        out[0] += t[0];
+        out[1] += t[1];
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -436,6 +455,12 @@ struct wspm_ctx {
         int cnt = 0;
         int32_t *nodes = nullptr, *slot_prev = nullptr, *slot_next = nullptr;
     } edges[2];
+    // discrete adjoint workspace (allocated on first use)
+    struct {
+        double *nodal = nullptr, *ml = nullptr, *J = nullptr, *JT = nullptr, *dinv = nullptr, *rhs = nullptr,
+               *dldh = nullptr, *ge = nullptr, *gn = nullptr;
+        int32_t* tpos = nullptr;
+    } adj;
     bool rich_neumann = false, rich_freedrain = false;
     double rich_flux = 0.0;
     TrQuad tq{};
@@ -1460,6 +1485,99 @@ int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp, 
     if (tp->outlet_freedrain)   // boundary.py:336-421
         launch_edge_list(ctx, WSPM_EDGE_OUTLET, EDGE_FREEDRAIN_N, 0.0, 0.0, K, 1, theta, A_vals, nullptr, dinv);
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the implicit step
+// ---------------------------------------------------------------------------
+static void adj_reduce(wspm_ctx* ctx, const double* a, int len, double* out) {
+    const AdjSum op{a, len, out};
+    const int g = std::max(1, std::min((len + BLOCK - 1) / BLOCK, 1024));
+    hipLaunchKernelGGL(k_vec<AdjSum>, dim3(g), dim3(BLOCK), 0, ctx->stream, len, op, red_of(ctx, 0),
+                       EnsWalk{nullptr, nullptr, 0});
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_finish<2, AdjSum>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials, ctx->prow, g,
+                       ctx->states);
+    HIPCHK(hipGetLastError());
+}
+
+int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* Ks_elem, const double* h,
+                      const double* h_prev, const int32_t* obs, const double* dldy, const double* lam_next,
+                      double* lam, double* grad_logks, double* grad_an, wspm_krylov_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!prm || !Ks_elem || !h || !h_prev || !lam || !grad_logks || !grad_an)
+        return fail(ctx, WSPM_EINVAL, "null argument");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "the adjoint is 2D");
+    if (ctx->max_row > 16) return fail(ctx, WSPM_EINVAL, "rows longer than 16 entries");
+    if (!(prm->dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
+    if (prm->n_obs < 0 || (prm->n_obs > 0 && (!obs || !dldy))) return fail(ctx, WSPM_EINVAL, "bad observations");
+    if (prm->maxiter < 1) return fail(ctx, WSPM_EINVAL, "maxiter must be >= 1");
+    API_BEGIN
+    const int n = ctx->n, E = ctx->E;
+    auto& W = ctx->adj;
+    if (!W.nodal) {
+        W.nodal = dalloc<double>(ctx, 9 * (size_t)n);
+        W.ml = dalloc<double>(ctx, n);
+        W.J = dalloc<double>(ctx, ctx->nnz);
+        W.JT = dalloc<double>(ctx, ctx->nnz);
+        W.dinv = dalloc<double>(ctx, n);
+        W.rhs = dalloc<double>(ctx, n);
+        W.dldh = dalloc<double>(ctx, n);
+        W.ge = dalloc<double>(ctx, 2 * (size_t)E);
+        W.gn = dalloc<double>(ctx, 2 * (size_t)n);
+        // position of the transposed entry of every CSR slot (symmetric pattern)
+        const HostPattern& P = ctx->hp;
+        std::vector<int32_t> tp(P.cols.size());
+        for (int i = 0; i < n; ++i)
+            for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) {
+                const int j = P.cols[k];
+                const auto b = P.cols.begin() + P.row_ptr[j], e = P.cols.begin() + P.row_ptr[j + 1];
+                tp[k] = (int32_t)(std::lower_bound(b, e, i) - P.cols.begin());
+            }
+        W.tpos = dupload(ctx, tp.data(), tp.size());
+    }
+    AdjArgs a{};
+    a.n = n;
+    a.E = E;
+    a.row_ptr = ctx->row_ptr;
+    a.diag_pos = ctx->diag_pos;
+    a.inc_ptr = ctx->inc_ptr;
+    a.inc = ctx->inc;
+    a.conn4 = ctx->conn4;
+    a.nrec = ctx->nrec;
+    a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
+    a.ks_e = Ks_elem;
+    a.h = h;
+    a.hp = h_prev;
+    double p[6];
+    for (int k = 0; k < 6; ++k) p[k] = prm->phi[k];
+    p[5] = 1.0;
+    a.sp = make_soil(WSPM_SOIL_VAN_GENUCHTEN, p);
+    adj_tie_m(a.sp);
+    a.inv_dt = 1.0 / prm->dt;
+    a.qc = ctx->qc;
+    double* nd = W.nodal;
+    a.nd = AdjNodal{nd, nd + n, nd + 2 * (size_t)n, nd + 3 * (size_t)n, nd + 4 * (size_t)n, nd + 5 * (size_t)n,
+                    nd + 6 * (size_t)n, nd + 7 * (size_t)n, nd + 8 * (size_t)n};
+    a.ml = W.ml;
+    HIPCHK(launch_adj_mass(a, W.ml, ctx->stream));
+    HIPCHK(launch_adj_nodal(a, ctx->stream));
+    HIPCHK(launch_adj_jacobian(a, ctx->max_row, W.J, W.JT, W.dinv, W.tpos, ctx->nnz, ctx->stream));
+    HIPCHK(hipMemsetAsync(W.dldh, 0, n * sizeof(double), ctx->stream));
+    if (prm->n_obs > 0) HIPCHK(launch_adj_obs(a, prm->n_obs, obs, dldy, W.dldh, grad_an, ctx->stream));
+    HIPCHK(launch_adj_rhs(a, W.dldh, lam_next, W.rhs, ctx->stream));
+    if (lam_next) HIPCHK(hipMemcpyAsync(lam, lam_next, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    else HIPCHK(hipMemsetAsync(lam, 0, n * sizeof(double), ctx->stream));
+    wspm_krylov_stats st{};
+    krylov_solve<double>(ctx, WSPM_BICGSTAB, W.JT, W.dinv, W.rhs, lam, lam, prm->tol, 0.0, prm->maxiter, 30, 0, &st);
+    HIPCHK(launch_adj_params(a, lam, grad_logks, W.ge, W.gn, ctx->stream));
+    adj_reduce(ctx, W.ge, E, grad_an);
+    adj_reduce(ctx, W.gn, n, grad_an);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats) *stats = st;
     return WSPM_OK;
     API_END
 }

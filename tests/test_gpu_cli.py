@@ -59,3 +59,30 @@ def test_solve_system_mirror_matches_oracle():
     with pytest.raises(NotImplementedError):
         solve_system(A, b, b, SolverParameters(solver_type="cg", precond_type="ilu"))
     ctx.close()
+
+
+def test_cli_nitrogen_case_writes_species_and_matches_oracle(tmp_path):
+    """--test-case NitrogenTest (generated 25 x 25 unit square, the reference's
+    data/mesh/nitrogen/p_25 layout): NPZ with the three species, equal to the
+    oracle's nitrogen loop."""
+    from jax_wspm_amd.cli import parse_args, run
+    from oracle import mesh as omesh, transport as otr
+    print("[case] cli NitrogenTest 25", flush=True)
+    args = parse_args(["--test-case", "NitrogenTest", "--mesh-size", "25", "--solver", "bicgstab",
+                       "--preconditioner", "jacobi", "--dt", "1e-3", "--tmax", "0.0025", "--output-dir", str(tmp_path)])
+    run(args)
+    z = np.load(tmp_path / "numerical_results_NitrogenTest.npz")
+    for k in ("pressure_head", "theta", "times", "points", "iterations", "errors", "dt_values", "elements",
+              "NH4_concentration", "NO2_concentration", "NO3_concentration"):
+        assert k in z.files, k
+    steps = len(z["times"])
+    assert steps == 3          # t = 0, 1e-3, 2.2e-3 (dt grows by 1.2)
+    pts, tri = omesh.structured_tri(25, 1.0)
+    phi = np.array([3.6, 0.43, 0.078, 1.56, 1 - 1 / 1.56, 0.2496])
+    o = otr.nitrogen_run(pts, np.asarray(tri), phi, steps, 1e-3, solver="bicgstab", precond="jacobi")
+    assert list(z["iterations"]) == list(o["iterations"])
+    for k in range(steps):
+        for key, okey in (("pressure_head", "pressure_head"), ("NH4_concentration", "NH4"),
+                          ("NO3_concentration", "NO3")):
+            a, b = z[key][k], o[okey][k]
+            assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), (key, k)
