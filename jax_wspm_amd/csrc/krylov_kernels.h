@@ -35,8 +35,8 @@ struct KVec {
 // tflag (nullable): the benchmark's launch-timing slot; receives the number of
 // rows the launch worked on (skipped instances count none), so timed averages
 // count working launches only and weigh them by their rows.
-// ew: ensemble walk (ew.tab null: one system, the grid strides over all n
-// rows with the state red.st[0]).
+// ew: ensemble walk (ew.tab null: one system; workgroup b owns the contiguous
+// rows of logical block xcd_block(b), with the state red.st[0]).
 //
 // SPMV_UNROLL rows per lane group and pass: their row pointers, then their
 // first LANES entries (values + columns), then the x gathers are each issued
@@ -99,7 +99,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
         epi.load(red.st);
-        spmv_rows<LANES, T>(blockIdx.x * GPB + g, n, gridDim.x * GPB, rp, ci, av, epi, acc);
+        const int chunk = (n + gridDim.x - 1) / gridDim.x;
+        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
+        spmv_rows<LANES, T>(rb + g, min(n, rb + chunk), GPB, rp, ci, av, epi, acc);
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
@@ -131,7 +133,10 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
         op.load(red.st);
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) op.row(i, acc);
+        // contiguous chunk per workgroup, a multiple of BLOCK rows
+        const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk, re = min(n, rb + chunk);
+        for (int i = rb + threadIdx.x; i < re; i += BLOCK) op.row(i, acc);
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }

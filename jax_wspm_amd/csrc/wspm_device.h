@@ -47,6 +47,22 @@ struct KState {
     double y[MAX_RESTART];                   // GMRES least-squares solution
 };
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by index,
+// and each XCD has its own L2.  xcd_block maps a workgroup's index to a
+// logical index such that every XCD receives one contiguous range of logical
+// blocks: neighbouring rows (which gather the same node and element records)
+// then share one L2 instead of being fetched by several.  A bijection of
+// [0, nb) for any nb.  Used by the one-system SpMV and vector kernels (it
+// brings the SpMV's fetched bytes down to its algorithmic bytes); measured
+// slower for the ensemble walks and neutral for the assembly, which keep
+// the plain order.
+constexpr int NXCD = 8;
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb / NXCD, r = nb % NXCD;
+    const int x = b % NXCD, k = b / NXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic reductions.  Wavefront butterfly (fixed xor order), then the
 // NWAVE wave sums added in wave order by thread 0.  Grid level: every
