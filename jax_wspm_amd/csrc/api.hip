@@ -401,6 +401,7 @@ struct wspm_ctx {
     int asm_win = 0, asm_inc = 0, asm_grid = 0;   // assembly LDS window / incidence capacity
     size_t asm_lds = 0;
     bool asm_reg = false;                         // rows fit the register assembly kernel
+    int asm_kernel = ASM_K_LDS;                   // ASM_K_*
     int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
     int spmv_grid = 1, vec_grid = 1;
@@ -831,6 +832,11 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
             const char* ak = std::getenv("WSPM_ASM_KERNEL");
             ctx->asm_reg = P.max_row <= asm_reg_max_row(dim) && !(ak && std::string(ak) == "lds");
             if (ctx->asm_reg) ctx->asm_grid = (n_nodes + BLOCK - 1) / BLOCK;
+            int max_row_inc = 0;
+            for (int i = 0; i < n_nodes; ++i) max_row_inc = std::max(max_row_inc, P.inc_ptr[i + 1] - P.inc_ptr[i]);
+            // WSPM_ASM_KERNEL=inc keeps the incidence-gather register kernel
+            const bool nbr = dim == 2 && max_row_inc <= asm_reg_max_row(dim) - 1 && !(ak && std::string(ak) == "inc");
+            ctx->asm_kernel = !ctx->asm_reg ? ASM_K_LDS : nbr ? ASM_K_NBR : ASM_K_REG;
             ctx->prow = std::max(MAX_BLOCKS, std::max(ctx->asm_grid, (n_nodes + R - 1) / R));
         }
         if (!ctx->asm_reg && ctx->asm_lds > 64 * 1024) {
@@ -1023,7 +1029,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     }
     int grid = 0;
     HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream,
-                           ctx->asm_reg));
+                           ctx->asm_kernel));
     if (mode == ASM_INIT_NONE) return;
     KWork<double>& w = ensure_work<double>(ctx, 0);
     KVec<double> v = make_kvec<double>(ctx, w, dinv, b, h_m, x);

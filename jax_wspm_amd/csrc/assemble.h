@@ -102,7 +102,11 @@ size_t asm_lds_bytes(int dim, int max_block_nnz, int max_block_inc);
 // Launch the assembly; grid_out receives the number of workgroups (partial
 // rows written when mode != ASM_INIT_NONE).
 int asm_reg_max_row(int dim);
+// Assembly kernels: LDS-staged (any rows), register with incidence gathers
+// (2D rows <= 8 entries), register with the row's neighbourhood gathered once
+// (2D rows <= 8 entries and <= 7 incidences).
+enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2 };
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
-                           int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg);
+                           int prow, size_t lds, int* grid_out, hipStream_t s, int kernel);
 
 }  // namespace wspm

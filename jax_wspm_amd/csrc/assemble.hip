@@ -380,12 +380,130 @@ __device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int 
     }
 }
 
-template <int D, int MAXR, int MODE>
+// Neighbourhood variant (2D).  A row's CSR columns are exactly the nodes its
+// incident elements touch, and every incidence's slot word already maps the
+// element's vertices onto those columns.  So the row gathers the node records
+// and x0 values of its columns once; each incidence then only adds its own
+// word and Ks_e.  Three dependent memory round trips per row (row pointers;
+// columns + incidences; records + x0 + Ks) instead of three per batch of
+// incidences, and rl node-record loads instead of 3 per incidence.  The
+// element arithmetic and the order of every sum are those of assemble_row,
+// so the results are identical.  Needs at most MAXR - 1 incidences per row.
+template <int MAXR, int MODE>
+__device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
+    constexpr int MAXI = MAXR - 1;
+    const int rs = a.row_ptr[i];
+    const int rl = a.row_ptr[i + 1] - rs;
+    const int dg = a.diag_pos[i] - rs;
+    const int q0 = a.inc_ptr[i], nq = a.inc_ptr[i + 1] - q0;
+    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
+    int col[MAXR];
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) col[s] = s < rl ? a.cols[rs + s] : i;
+    int2 ic[MAXI];
+#pragma unroll
+    for (int u = 0; u < MAXI; ++u) ic[u] = (!dir && u < nq) ? a.inc[q0 + u] : make_int2(0, 0);
+    double4 nr[MAXR];
+    double xv[MAXR];
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) {
+        nr[s] = (!dir && s < rl) ? a.nrec[col[s]] : make_double4(0.0, 0.0, 0.0, 0.0);
+        if constexpr (MODE != ASM_INIT_NONE) xv[s] = s < rl ? ki.x0[col[s]] : 0.0;
+    }
+    double ks[MAXI];
+#pragma unroll
+    for (int u = 0; u < MAXI; ++u) ks[u] = (a.Ks_elem != nullptr && !dir && u < nq) ? a.Ks_elem[ic[u].x >> 2] : 1.0;
+    double v[MAXR];
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) v[s] = 0.0;
+    double bi;
+    if (dir) {
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) v[s] = (s == dg) ? 1.0 : 0.0;
+        bi = a.bc_val[i];
+    } else {
+        bi = 0.0;
+        double msum = 0.0;
+        // every iteration predicated (no early exit), so the loop unrolls fully
+        // and ic / ks stay in registers
+#pragma unroll
+        for (int u = 0; u < MAXI; ++u) {
+            if (u < nq) {
+            const uint32_t w = (uint32_t)ic[u].y;
+            double px[3], py[3], pz[3], kn[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                // one-hot blend of the row's records (exact: one weight is 1,
+                // the others 0 against finite values).  A select chain here is
+                // turned by the compiler into an indexed load from a private
+                // array, i.e. scratch memory.
+                const uint32_t sl = (w >> (8 * c)) & 0xffu;
+                double rx = 0.0, ry = 0.0, rk = 0.0;
+#pragma unroll
+                for (int s = 0; s < MAXR; ++s) {
+                    const double m = (sl == (uint32_t)s) ? 1.0 : 0.0;
+                    rx += m * nr[s].x;
+                    ry += m * nr[s].y;
+                    rk += m * nr[s].z;
+                }
+                px[c] = rx;
+                py[c] = ry;
+                pz[c] = 0.0;
+                kn[c] = rk;
+            }
+            double out[5];
+            element_compute<2>(a, px, py, pz, kn, ks[u], ic[u].x & 3, out);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t sl = (w >> (8 * c)) & 0xffu;
+#pragma unroll
+                for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? out[c] : 0.0;
+            }
+            bi += out[3];
+            msum += out[4];
+            }
+        }
+        const double ci = a.C[i];
+        const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+        const double madd = a.inv_dt * msum * ci;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) v[s] += (s == dg) ? madd : 0.0;
+        bi -= a.inv_dt * msum * rhs;
+    }
+    double dgv = 0.0;
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) {
+        if (s < rl) {
+            a.A[rs + s] = v[s];
+            if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+        }
+        if (s == dg) dgv = v[s];
+    }
+    a.b[i] = bi;
+    const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
+    if (a.dinv != nullptr) a.dinv[i] = di;
+    if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
+    if constexpr (MODE != ASM_INIT_NONE) {
+        double ax = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s)
+            if (s < rl) ax += v[s] * xv[s];
+        krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
+    }
+}
+
+template <int D, int MAXR, int MODE, bool NBR>
+__device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
+    if constexpr (NBR && D == 2) assemble_row_nbr<MAXR, MODE>(a, ki, i, acc);
+    else assemble_row<D, MAXR, MODE>(a, ki, i, acc);
+}
+
+template <int D, int MAXR, int MODE, bool NBR>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     if (a.tab == nullptr) {
         const int i = blockIdx.x * BLOCK + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
-        if (i < a.n) assemble_row<D, MAXR, MODE>(a, ki, i, acc);
+        if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
         if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow);
         return;
     }
@@ -398,7 +516,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
         if (a.ens_st[tb.z].pdone) continue;
         const int i = tb.x + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
-        if (i < tb.y) assemble_row<D, MAXR, MODE>(a, ki, i, acc);
+        if (i < tb.y) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
         if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
     }
 }
@@ -458,20 +576,20 @@ static hipError_t launch_mode(int mode, dim3 g, size_t lds, hipStream_t s, const
     return hipGetLastError();
 }
 
-template <int D, int MAXR>
+template <int D, int MAXR, bool NBR>
 static hipError_t launch_reg(int mode, dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki,
                              double* partials, int prow) {
     switch (mode) {
         case ASM_INIT_BICG:
-            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_BICG>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_BICG, NBR>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
         case ASM_INIT_CG:
-            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_CG>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_CG, NBR>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
         case ASM_INIT_GMRES:
-            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_GMRES>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_GMRES, NBR>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
         case ASM_INIT_RESID:
-            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_RESID>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_RESID, NBR>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
         default:
-            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_NONE>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
+            hipLaunchKernelGGL((k_assemble_reg<D, MAXR, ASM_INIT_NONE, NBR>), g, dim3(BLOCK), 0, s, a, ki, partials, prow); break;
     }
     return hipGetLastError();
 }
@@ -479,11 +597,12 @@ static hipError_t launch_reg(int mode, dim3 g, hipStream_t s, const AsmArgs& a, 
 int asm_reg_max_row(int dim) { return dim == 2 ? ASM_REG_ROW_2D : ASM_REG_ROW_3D; }
 
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
-                           int prow, size_t lds, int* grid_out, hipStream_t s, bool use_reg) {
-    if (use_reg) {
+                           int prow, size_t lds, int* grid_out, hipStream_t s, int kernel) {
+    if (kernel != ASM_K_LDS) {
         const int grid = a.tab ? a.ens_grid : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
-        return launch_reg<2, ASM_REG_ROW_2D>(mode, dim3(grid), s, a, ki, partials, prow);
+        if (kernel == ASM_K_NBR) return launch_reg<2, ASM_REG_ROW_2D, true>(mode, dim3(grid), s, a, ki, partials, prow);
+        return launch_reg<2, ASM_REG_ROW_2D, false>(mode, dim3(grid), s, a, ki, partials, prow);
     }
     const int R = asm_rows_per_block(dim);
     const int grid = (a.n + R - 1) / R;

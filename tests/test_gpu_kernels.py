@@ -43,17 +43,19 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernel", ["auto", "lds"])
+@pytest.mark.parametrize("kernel", ["auto", "inc", "lds"])
 @pytest.mark.parametrize("name,mk,model,phi,use_ks", CASES, ids=[c[0] for c in CASES])
 def test_pattern_soil_assembly(name, mk, model, phi, use_ks, kernel, monkeypatch):
-    """Both assembly kernels: the register one (rows that fit) and the
-    LDS-staged one (forced with WSPM_ASM_KERNEL=lds, used for long rows)."""
+    """Every assembly kernel: the default (2D: register kernel gathering the
+    row's neighbourhood once), the register kernel with per-incidence gathers
+    (WSPM_ASM_KERNEL=inc) and the LDS-staged one (WSPM_ASM_KERNEL=lds, used
+    for long rows)."""
     import torch
     pts, el = mk()
     n = len(pts)
     print(f"[case] {name} n={n} E={len(el)} kernel={kernel}", flush=True)
-    if kernel == "lds":
-        monkeypatch.setenv("WSPM_ASM_KERNEL", "lds")
+    if kernel != "auto":
+        monkeypatch.setenv("WSPM_ASM_KERNEL", kernel)
     ctx = _ctx(pts, el)
     pat = oasm.Pattern(el, n)
     rp, ci = ctx.pattern()
@@ -98,6 +100,35 @@ def test_pattern_soil_assembly(name, mk, model, phi, use_ks, kernel, monkeypatch
     scale = np.abs(pat.to_scipy(np.abs(vals)) @ np.abs(x))
     assert np.max(np.abs(y - yr) / np.maximum(scale, 1e-300)) <= 1e-14
     ctx.close()
+
+
+def test_register_kernels_agree_bitwise(monkeypatch):
+    """The neighbourhood kernel does the incidence kernel's arithmetic in the
+    same order: A, b and the Jacobi diagonal are bitwise equal (jittered mesh,
+    per-element Ks, Dirichlet rows)."""
+    import torch
+    pts, el = omesh.structured_tri(17, 1.0)
+    rng = np.random.default_rng(8)
+    inner = (pts[:, 0] > 0) & (pts[:, 0] < 1) & (pts[:, 1] > 0) & (pts[:, 1] < 1)
+    pts = pts.copy()
+    pts[inner] += 0.3 / 17 * rng.uniform(-1, 1, size=(inner.sum(), 2))
+    n = len(pts)
+    h_m, h_n = _state(n, 4)
+    Cm, Km, thm = osoil.van_genuchten(h_m, VG)
+    _, _, thn = osoil.van_genuchten(h_n, VG)
+    ks = np.exp(np.log(1e-3) + 0.5 * rng.standard_normal(len(el)))
+    top = np.nonzero(pts[:, 1] == 1.0)[0]
+    out = []
+    for kernel in ("auto", "inc"):
+        print(f"[case] bitwise register kernels {kernel}", flush=True)
+        monkeypatch.setenv("WSPM_ASM_KERNEL", kernel)
+        ctx = _ctx(pts, el)
+        ctx.set_dirichlet([(top, -75.0)])
+        A, b, d = ctx.assemble_richards(h_m, Cm, Km, thm, thn, 1e-4, Ks_elem=ks)
+        out.append([t.cpu().numpy() for t in (A, b, d)])
+        ctx.close()
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
 
 
 def _system(cells=12, dim=2, seed=0, dt=1e-4):
