@@ -14,6 +14,11 @@ HIP events on the solver's stream; their mean duration gives the roofline
 line.  The CPU baseline runs the oracle (NumPy/SciPy FP64 restatement of the
 reference, one thread) on the first steps of the same run.
 
+Workload c4: the discrete adjoint of the flow at 256 x 256 (200 steps of dt
+1e-3, theta misfit at 64 nodes, checkpoint every 20 steps); value = time
+steps of forward + backward per second; the gradient is checked on 16 random
+directions by central differences.
+
 Workload c3: the batched ensemble -- 256 independent 2D instances, side N
 drawn from {32, 48, ..., 256}, per-instance van Genuchten parameters
 (jax_wspm_amd.ensemble.vg_ensemble, seed 0), 100 steps of dt 1e-4,
@@ -49,8 +54,12 @@ WORKLOADS = {
     "c3": ("batched ensemble: 256 independent 2D van Genuchten instances, side N in {32,48,...,256}, "
            "per-instance alpha, n, Ks, theta_r, theta_s, fixed dt 1e-4, Jacobi-BiCGSTAB, FP64",
            None, 2, "bicgstab", "fp64", None, 1e-4),
+    "c4": ("discrete adjoint of 2D Richards flow: 256x256 triangles (131,072 elements), log-normal Ks, "
+           "200 steps dt 1e-3, theta misfit at 64 seeded nodes, gradient w.r.t. per-element log Ks, alpha, n, "
+           "checkpoint every 20 steps, FP64", 256, 2, "bicgstab", "fp64", {"sigma": 0.5, "length": 0.05, "seed": 0},
+           1e-3),
 }
-DEFAULT_STEPS = {"c3": 100}
+DEFAULT_STEPS = {"c3": 100, "c4": 200}
 
 # HBM-side bytes per working launch of the dominant SpMV (k_spmv<8,double,BicgQ>),
 # from two rocprofv3 --pmc runs of this script with --calibrate-copy: FETCH_SIZE
@@ -71,6 +80,7 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: steps per instance of the plain single-instance path (0 = skip)")
     args = ap.parse_args()
@@ -227,6 +237,90 @@ def run_ensemble(args):
     print(json.dumps(out))
 
 
+def run_adjoint(args):
+    """c4: forward + checkpointed adjoint backward of the theta misfit; the
+    gradient is checked on 16 random directions by central differences of
+    the same forward (tight Picard / Krylov tolerances so the differences
+    resolve the converged-step model)."""
+    import numpy as np
+    import torch
+
+    from jax_wspm_amd import mesh as M
+    from jax_wspm_amd.adjoint import RichardsAdjointModel
+
+    desc, cells, _, _, _, ksf, dt = WORKLOADS["c4"]
+    steps = args.steps
+    pts, tri = M.unit_square(cells, 1.0)
+    top = np.nonzero(pts[:, 1] == 1.0)[0]
+    log_ks = np.log(M.correlated_log_ks(pts, tri, cells, mean_ks=1e-2, sigma=ksf["sigma"], length=ksf["length"],
+                                        seed=ksf["seed"]))
+    rng = np.random.default_rng(0)
+    upper = np.nonzero((pts[:, 1] >= 0.75) & (pts[:, 1] < 1.0))[0]
+    obs = np.sort(rng.choice(upper, size=64, replace=False))
+    obs_steps = list(range(20, steps + 1, 20)) or [steps]
+    model = RichardsAdjointModel(pts, tri, [(top, np.full(top.shape, -5.0))], np.full(len(pts), -100.0), dt, steps,
+                                 obs, obs_steps, 0.40, 0.06, checkpoint=20, tol=1e-13, maxiter=400, picard_tol=1e-12,
+                                 picard_maxit=300, adj_tol=1e-12)
+    lk = torch.tensor(log_ks, device="cuda")
+    alpha, n = 0.1, 2.0
+    y_obs = None
+    print(f"[bench] c4: n={model.n} E={model.E} steps={steps} obs={len(obs)}x{len(obs_steps)}", file=sys.stderr,
+          flush=True)
+    model.forward(lk, alpha, n)                          # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y = model.forward(lk, alpha, n)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    y_obs = (y * 0.98).detach()                          # synthetic data: a 2 % drier theta
+    dldy = 2.0 * (y - y_obs)
+    t0 = time.perf_counter()
+    g_ks, g_a, g_n = model.backward(dldy)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    print(f"[bench] c4 forward {t_fwd:.2f} s backward {t_bwd:.2f} s", file=sys.stderr, flush=True)
+    g = np.concatenate([g_ks.cpu().numpy(), [float(g_a), float(g_n)]])
+
+    def loss(p):
+        with torch.no_grad():
+            yy = model.forward(torch.tensor(p[:-2], device="cuda"), float(p[-2]), float(p[-1]))
+            return float(((yy - y_obs) ** 2).sum())
+
+    p0 = np.concatenate([log_ks, [alpha, n]])
+    scale = np.concatenate([np.ones(len(log_ks)), [alpha, n]])
+    errs, nerrs = [], []
+    drng = np.random.default_rng(1)
+    for k in range(args.fd_dirs):
+        d = drng.normal(size=p0.size) * scale
+        # central differences, eps 1e-4: truncation ~eps^2, and the forward's
+        # solver noise is divided by eps
+        eps = 1e-4
+        fd = (loss(p0 + eps * d) - loss(p0 - eps * d)) / (2 * eps)
+        errs.append(abs(g @ d - fd) / max(abs(fd), 1e-300))
+        nerrs.append(abs(g @ d - fd) / (np.linalg.norm(g) * np.linalg.norm(d)))
+        print(f"[bench] c4 direction {k + 1}/{args.fd_dirs}: g.d {g @ d:.6e} fd {fd:.6e} rel err {errs[-1]:.2e}",
+              file=sys.stderr, flush=True)
+    model.close()
+    out = {
+        "metric": METRIC, "value": round(steps / (t_fwd + t_bwd), 3), "unit": "time steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": 1, "ms_per_step": round((t_fwd + t_bwd) / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (structured mesh, seeded Ks field and observation nodes, synthetic observations)",
+        "config": {"workload": desc, "nodes": model.n, "elements": model.E,
+                   "value_counts": "time steps of forward + adjoint backward (one gradient) per second",
+                   "forward_s": round(t_fwd, 3), "backward_s": round(t_bwd, 3),
+                   "backward_over_forward": round(t_bwd / t_fwd, 2),
+                   "adjoint_bicgstab_iters_per_step": round(float(np.mean(model.adj_iters)), 2),
+                   "fd_directions": args.fd_dirs,
+                   "fd_max_rel_err": float(max(errs)) if errs else None,
+                   "fd_median_rel_err": float(np.median(errs)) if errs else None,
+                   "fd_max_err_over_norms": float(max(nerrs)) if nerrs else None,
+                   "tolerances": "Picard rel 1e-12, Krylov 1e-13, adjoint solve 1e-12",
+                   "not_included": "the NO3- part of the loss (nitrogen-transport adjoint not built)"},
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,6 +337,9 @@ def main():
         del src, dst
     if args.workload == "c3":
         run_ensemble(args)
+        return
+    if args.workload == "c4":
+        run_adjoint(args)
         return
     import numpy as np
     import torch
