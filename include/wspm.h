@@ -69,7 +69,11 @@ enum {
 
 enum { WSPM_SOIL_GARDNER = 0, WSPM_SOIL_VAN_GENUCHTEN = 1 };
 enum { WSPM_CG = 0, WSPM_BICGSTAB = 1, WSPM_GMRES = 2 };
-enum { WSPM_PRECOND_NONE = 0, WSPM_PRECOND_JACOBI = 1 };
+/* Jacobi is fused into the Krylov kernels; the others (src/solvers/
+ * jax_linear_solver.py:39-252) are applied between them: 2x2 block Jacobi,
+ * SSOR (omega = 1) and the reference's "ilu0" (I + L D^-1)(D + U). */
+enum { WSPM_PRECOND_NONE = 0, WSPM_PRECOND_JACOBI = 1, WSPM_PRECOND_BLOCK_JACOBI = 2,
+       WSPM_PRECOND_SSOR = 3, WSPM_PRECOND_ILU = 4 };
 enum { WSPM_FP64 = 0, WSPM_MIXED = 1 };
 enum { WSPM_PICARD_REL = 0, WSPM_PICARD_ABS = 1 };
 
@@ -93,7 +97,7 @@ typedef struct {
 typedef struct {
     int32_t soil_model;   /* WSPM_SOIL_*                                      */
     int32_t method;       /* WSPM_CG / WSPM_BICGSTAB / WSPM_GMRES             */
-    int32_t precond;      /* WSPM_PRECOND_*                                   */
+    int32_t precond;      /* WSPM_PRECOND_* (general ones: FP64, one mesh)    */
     int32_t precision;    /* WSPM_FP64 or WSPM_MIXED                          */
     double phi[6];        /* gardner: alpha0, theta_s, theta_r, Ks
                              van Genuchten: alpha, theta_s, theta_r, n, m, Ks
@@ -151,6 +155,11 @@ int32_t wspm_spmv(wspm_ctx* ctx, const double* A_vals, const double* x, double* 
 int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv,
                     const double* b, const double* x0, double* x, double tol, double atol,
                     int32_t maxiter, int32_t restart, wspm_krylov_stats* stats);
+
+/* Krylov solve with any WSPM_PRECOND_* built from A_vals (dev pointers). */
+int32_t wspm_krylov_pc(wspm_ctx* ctx, int32_t method, int32_t precond, const double* A_vals,
+                       const double* b, const double* x0, double* x, double tol, double atol,
+                       int32_t maxiter, int32_t restart, wspm_krylov_stats* stats);
 
 /* Mixed precision: FP32 Krylov inner solves inside FP64 iterative refinement
  * on the true residual; stops when |b - A x| <= max(tol |b|, atol) (FP64). */

@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwspm.so"
 WSPM_OK, WSPM_EINVAL, WSPM_EHIP, WSPM_ENOMEM, WSPM_ESTATE = 0, -1, -2, -3, -4
 SOIL = {"gardner": 0, "exponential": 0, "vg": 1, "van_genuchten": 1}
 METHOD = {"cg": 0, "bicgstab": 1, "gmres": 2}
-PRECOND = {"none": 0, None: 0, "jacobi": 1}
+PRECOND = {"none": 0, None: 0, "jacobi": 1, "block_jacobi": 2, "ssor": 3, "ilu": 4}
 PRECISION = {"fp64": 0, "mixed": 1}
 PICARD = {"rel": 0, "abs": 1}
 
@@ -75,6 +75,7 @@ PROTOTYPES = {
     "wspm_assemble_richards": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
     "wspm_spmv": (_I, [_P, _P, _P, _P]),
     "wspm_krylov": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, C.POINTER(KrylovStats)]),
+    "wspm_krylov_pc": (_I, [_P, _I, _I, _P, _P, _P, _P, _D, _D, _I, _I, C.POINTER(KrylovStats)]),
     "wspm_krylov_mixed": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, _D, _I,
                                C.POINTER(KrylovStats)]),
     "wspm_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P]),

@@ -13,11 +13,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "assemble.h"
 #include "adjoint.h"
 #include "krylov_kernels.h"
+#include "precond.h"
 #include "transport.h"
 #include "wspm_device.h"
 #include "wspm_internal.h"
@@ -462,6 +464,12 @@ struct wspm_ctx {
                *dldh = nullptr, *ge = nullptr, *gn = nullptr;
         int32_t* tpos = nullptr;
     } adj;
+    // general preconditioners (allocated on first use)
+    struct {
+        Levels lo, up;
+        bool levels = false;
+        double *binv = nullptr, *dg = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+    } pc;
     bool rich_neumann = false, rich_freedrain = false;
     double rich_flux = 0.0;
     TrQuad tq{};
@@ -620,18 +628,66 @@ KWork<T>& ensure_work(wspm_ctx* ctx, int vcols) {
 }
 
 template <class T>
-KVec<T> make_kvec(wspm_ctx* ctx, KWork<T>& w, const T* dinv, const T* b, const T* x0, T* x) {
+KVec<T> make_kvec(wspm_ctx* ctx, KWork<T>& w, const T* dinv, const T* b, const T* x0, T* x, bool gpc = false) {
     KVec<T> v{};
     v.n = ctx->n;
     v.b = b; v.x = x; v.x0 = x0;
     v.r = w.r; v.rh = w.rh; v.q = w.q; v.t = w.t; v.w = w.w;
     v.p = w.p;
-    v.ph = dinv ? w.ph : w.p;     // M = identity: p_hat is p
-    v.sh = dinv ? w.sh : w.r;     // s lives in r
-    v.z = dinv ? w.z : w.r;
+    const bool sep = dinv != nullptr || gpc;
+    v.ph = sep ? w.ph : w.p;      // M = identity: p_hat is p
+    v.sh = sep ? w.sh : w.r;      // s lives in r
+    v.z = sep ? w.z : w.r;
+    v.gpc = gpc ? 1 : 0;
     v.V = w.V; v.ldv = ctx->n;
     v.dinv = dinv;
     return v;
+}
+
+// General preconditioners: workspace, level sets (once per mesh), setup from A.
+void pc_ensure(wspm_ctx* ctx, bool levels) {
+    auto& P = ctx->pc;
+    if (!P.dg) {
+        const int n = ctx->n;
+        P.dg = dalloc<double>(ctx, n);
+        P.binv = dalloc<double>(ctx, 4 * (size_t)((n + 1) / 2));
+        P.t1 = dalloc<double>(ctx, n);
+        P.t2 = dalloc<double>(ctx, n);
+        P.t3 = dalloc<double>(ctx, n);
+    }
+    if (levels && !P.levels) {
+        const HostPattern& H = ctx->hp;
+        const int n = ctx->n;
+        auto build = [&](bool lower, Levels& L) {
+            std::vector<int> lev(n, 0);
+            int nl = 0;
+            for (int t = 0; t < n; ++t) {
+                const int i = lower ? t : n - 1 - t;
+                int m = -1;
+                for (int k = H.row_ptr[i]; k < H.row_ptr[i + 1]; ++k) {
+                    const int j = H.cols[k];
+                    if (lower ? j < i : j > i) m = std::max(m, lev[j]);
+                }
+                lev[i] = m + 1;
+                nl = std::max(nl, lev[i] + 1);
+            }
+            std::vector<int32_t> ptr(nl + 1, 0), rows(n);
+            for (int i = 0; i < n; ++i) ++ptr[lev[i] + 1];
+            for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+            std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+            for (int i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+            L.n_lev = nl;
+            L.lvl_ptr = dupload(ctx, ptr.data(), ptr.size());
+            L.lvl_rows = dupload(ctx, rows.data(), rows.size());
+        };
+        build(true, P.lo);
+        build(false, P.up);
+        P.levels = true;
+    }
+}
+
+PcArgs pc_args(wspm_ctx* ctx, const double* A) {
+    return PcArgs{ctx->n, ctx->row_ptr, ctx->cols, ctx->diag_pos, A, ctx->pc.binv, ctx->pc.dg};
 }
 
 // One Krylov solve, split so callers can enqueue other work between the
@@ -651,28 +707,56 @@ struct KrylovRun {
     KState* ds = nullptr;
     int nbx = 0;
     int enq = 0;        // iterations (CG/BiCGSTAB) or cycles (GMRES) enqueued
+    int pc = 0;         // general preconditioner kind (WSPM_PRECOND_BLOCK_JACOBI..ILU) or 0
+
+    // dst = M src for a general preconditioner (FP64 only)
+    void apply(const T* src, T* dst, int skipmode) {
+        if constexpr (std::is_same<T, double>::value) {
+            HIPCHK(launch_pc_apply(pc, pc_args(ctx, A), ctx->pc.lo, ctx->pc.up, src, dst, ctx->pc.t3, ds, skipmode,
+                                   ctx->stream));
+        }
+    }
 
     KrylovRun(wspm_ctx* c, int m, const T* A_, const T* dinv_, const T* b, const T* x0, T* x, double tol,
-              double atol, int maxit, int restart, int wh, bool started)
-        : ctx(c), method(m), A(A_), dinv(dinv_), which(wh), maxiter(maxit) {
+              double atol, int maxit, int restart, int wh, bool started, int pc_kind = 0)
+        : ctx(c), method(m), A(A_), dinv(dinv_), which(wh), maxiter(maxit),
+          pc(pc_kind >= WSPM_PRECOND_BLOCK_JACOBI ? pc_kind : 0) {
         const int n = ctx->n;
         ds = states_of(ctx, which);
+        const bool gpc = pc != 0;
+        if (gpc) {
+            if constexpr (std::is_same<T, double>::value) {
+                pc_ensure(ctx, pc == WSPM_PRECOND_SSOR || pc == WSPM_PRECOND_ILU);
+                HIPCHK(launch_pc_setup(pc, pc_args(ctx, A), ctx->stream));
+            }
+        }
         if (method == WSPM_GMRES) {
             R = std::min(std::min(restart, n), MAX_RESTART);
             KWork<T>& w = ensure_work<T>(ctx, R + 1);
-            v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+            v = make_kvec<T>(ctx, w, dinv, b, x0, x, gpc);
             H = (which == 1) ? ctx->H32 : ctx->H;
             nbx = std::min((n + BLOCK - 1) / BLOCK, 256);
             if (!started) launch_spmv<T>(ctx, A, GmInit<T>{v, tol, atol, maxiter, R}, which, true);
+            if (gpc) {
+                T* t1 = (T*)ctx->pc.t1;
+                T* t2 = (T*)ctx->pc.t2;
+                apply(v.w, t1, 0);
+                apply(b, t2, 0);
+                launch_vec(ctx, GmPcStart<T>{v, t1, t2}, which);
+            }
             launch_vec(ctx, GmScale<T>{v, 0.0}, which);
         } else {
             KWork<T>& w = ensure_work<T>(ctx, 0);
-            v = make_kvec<T>(ctx, w, dinv, b, x0, x);
+            v = make_kvec<T>(ctx, w, dinv, b, x0, x, gpc);
             if (started) {
             } else if (method == WSPM_BICGSTAB) {
                 launch_spmv<T>(ctx, A, BicgInit<T>{v, tol, atol, maxiter}, which, true);
             } else {
                 launch_spmv<T>(ctx, A, CgInit<T>{v, tol, atol, maxiter}, which, true);
+                if (gpc) {
+                    apply(v.r, v.z, 1);
+                    launch_vec(ctx, CgPcStart<T>{v}, which);
+                }
             }
         }
     }
@@ -681,13 +765,19 @@ struct KrylovRun {
         for (int c = 0; c < count && enq < maxiter; ++c, ++enq) {
             if (method == WSPM_BICGSTAB) {
                 launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which, LIST_K);
+                if (pc) apply(v.p, v.ph, 1);
                 launch_spmv<T>(ctx, A, BicgQ<T>{v}, which, true, LIST_K);
                 launch_vec(ctx, BicgS<T>{v, 0.0}, which, LIST_K);
+                if (pc) apply(v.r, v.sh, 1);      // s lives in r
                 launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true, LIST_K);
                 launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K);
             } else if (method == WSPM_CG) {
                 launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true, LIST_K);
                 launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K);
+                if (pc) {
+                    apply(v.r, v.z, 1);
+                    launch_vec(ctx, CgPcRZ<T>{v}, which, LIST_K);
+                }
                 launch_vec(ctx, CgP<T>{v, 0.0}, which, LIST_K);
             } else {
                 gmres_cycle();
@@ -701,6 +791,10 @@ struct KrylovRun {
         HIPCHK(hipGetLastError());
         for (int j = 0; j < R; ++j) {
             launch_spmv<T>(ctx, A, GmArnoldi<T>{v, j}, which, true);
+            if (pc) {
+                apply(v.w, (T*)ctx->pc.t1, 2);
+                launch_vec(ctx, GmPcNorm<T>{v, (T*)ctx->pc.t1}, which);
+            }
             hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
                                red_of(ctx, which));
             HIPCHK(hipGetLastError());
@@ -716,6 +810,10 @@ struct KrylovRun {
         HIPCHK(hipGetLastError());
         launch_vec(ctx, GmUpdX<T>{v, R, state_field<double>(ds, offsetof(KState, y))}, which);
         launch_spmv<T>(ctx, A, GmResid<T>{v}, which, true);
+        if (pc) {
+            apply(v.w, (T*)ctx->pc.t1, 1);
+            launch_vec(ctx, GmPcResid<T>{v, (T*)ctx->pc.t1}, which);
+        }
         launch_vec(ctx, GmScale<T>{v, 0.0}, which);
     }
 
@@ -750,8 +848,8 @@ struct KrylovRun {
 template <class T>
 void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T* b, const T* x0, T* x,
                   double tol, double atol, int maxiter, int restart, int which, wspm_krylov_stats* st,
-                  bool started = false) {
-    KrylovRun<T> run(ctx, method, A, dinv, b, x0, x, tol, atol, maxiter, restart, which, started);
+                  bool started = false, int pc = 0) {
+    KrylovRun<T> run(ctx, method, A, dinv, b, x0, x, tol, atol, maxiter, restart, which, started, pc);
     run.enqueue(run.first_chunk());
     poll(ctx, which);
     while (run.more()) {
@@ -1110,6 +1208,35 @@ int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const d
     API_END
 }
 
+int32_t wspm_krylov_pc(wspm_ctx* ctx, int32_t method, int32_t precond, const double* A_vals, const double* b,
+                       const double* x0, double* x, double tol, double atol, int32_t maxiter, int32_t restart,
+                       wspm_krylov_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!A_vals || !b || !x0 || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    if (precond < WSPM_PRECOND_NONE || precond > WSPM_PRECOND_ILU) return fail(ctx, WSPM_EINVAL, "unknown preconditioner");
+    int rc = check_krylov_args(ctx, method, maxiter, restart);
+    if (rc) return rc;
+    API_BEGIN
+    const double* dinv = nullptr;
+    if (precond == WSPM_PRECOND_JACOBI) {
+        pc_ensure(ctx, false);
+        HIPCHK(launch_pc_setup(WSPM_PRECOND_JACOBI, pc_args(ctx, A_vals), ctx->stream));
+        dinv = ctx->pc.dg;
+    }
+    wspm_krylov_stats st{};
+    krylov_solve<double>(ctx, method, A_vals, dinv, b, x0, x, tol, atol, maxiter, restart, 0, &st, false, precond);
+    if (method == WSPM_GMRES) {
+        hipLaunchKernelGGL(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
+        HIPCHK(hipGetLastError());
+        poll(ctx, 0);
+        st.code = ctx->hstate[0].nan_flag ? -1 : 0;
+    }
+    if (stats) *stats = st;
+    return WSPM_OK;
+    API_END
+}
+
 static void ensure_mixed(wspm_ctx* ctx) {
     if (!ctx->ref_r) {
         ctx->ref_r = dalloc<double>(ctx, ctx->n);
@@ -1190,10 +1317,12 @@ static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm)
         return fail(ctx, WSPM_EINVAL, "unknown soil model");
     int rc = check_krylov_args(ctx, prm->method, prm->maxiter, prm->restart);
     if (rc) return rc;
-    if (prm->precond != WSPM_PRECOND_NONE && prm->precond != WSPM_PRECOND_JACOBI)
+    if (prm->precond < WSPM_PRECOND_NONE || prm->precond > WSPM_PRECOND_ILU)
         return fail(ctx, WSPM_EINVAL, "unknown preconditioner");
     if (prm->precision != WSPM_FP64 && prm->precision != WSPM_MIXED)
         return fail(ctx, WSPM_EINVAL, "unknown precision");
+    if (prm->precision == WSPM_MIXED && prm->precond > WSPM_PRECOND_JACOBI)
+        return fail(ctx, WSPM_EINVAL, "mixed precision takes no preconditioner or Jacobi");
     if (prm->picard_maxit < 1) return fail(ctx, WSPM_EINVAL, "picard_maxit must be >= 1");
     return WSPM_OK;
 }
@@ -1267,10 +1396,12 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     int it = 0, kry = 0;
     double err = INFINITY;
-    // edge terms change b after the assembly, so the solver starts on its own
-    const bool edges = ctx->rich_neumann || ctx->rich_freedrain;
+    // edge terms change b after the assembly, and a general preconditioner
+    // needs M r0: in both cases the solver starts on its own
+    const bool gpc = P.precond > WSPM_PRECOND_JACOBI;
+    const bool own_start = ctx->rich_neumann || ctx->rich_freedrain || gpc;
     while (err >= P.picard_tol && it < P.picard_maxit) {
-        const int mode = edges ? ASM_INIT_NONE
+        const int mode = own_start ? ASM_INIT_NONE
                          : mixed ? ASM_INIT_RESID
                          : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG
                          : P.method == WSPM_CG ? ASM_INIT_CG : ASM_INIT_GMRES;
@@ -1292,7 +1423,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                            P.picard_mode == WSPM_PICARD_REL};
         if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
-                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !edges);
+                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !own_start);
             launch_vec(ctx, pop, 0);
             poll(ctx, 0);
         } else {
@@ -1300,7 +1431,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             // error behind them, then read both back with one synchronisation;
             // if the solve needed more iterations, finish it and redo the error.
             KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
-                                  P.tol, P.atol, P.maxiter, P.restart, 0, !edges);
+                                  P.tol, P.atol, P.maxiter, P.restart, 0, !own_start, P.precond);
             run.enqueue(run.first_chunk());
             launch_vec(ctx, pop, 0);
             poll(ctx, 0);
@@ -1759,6 +1890,7 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
     int rc = check_richards_params(ctx, prm);
     if (rc) return rc;
     if (prm->method == WSPM_GMRES) return fail(ctx, WSPM_EINVAL, "ensembles solve with CG or BiCGSTAB");
+    if (prm->precond > WSPM_PRECOND_JACOBI) return fail(ctx, WSPM_EINVAL, "ensembles take no preconditioner or Jacobi");
     if (prm->precision != WSPM_FP64) return fail(ctx, WSPM_EINVAL, "ensembles are FP64");
     API_BEGIN
     richards_alloc(ctx, prm, Ks_elem, h0);

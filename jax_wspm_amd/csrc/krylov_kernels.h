@@ -24,6 +24,8 @@ struct KVec {
     T* V;         // GMRES basis, column-major, leading dimension ldv
     long ldv;
     const T* dinv;  // Jacobi inverse diagonal or nullptr
+    int gpc;        // a general preconditioner is applied between the kernels
+                    // (dinv is then null; ph, sh, z are separate vectors)
 };
 
 // Kernels with ND > 0 leave one partial row per workgroup; the host follows
@@ -387,9 +389,11 @@ struct CgXR {  // x += alpha p; r -= alpha q; z = M r; gamma', |r|^2
         acc[1] += rd * rd;
     }
     __device__ void fin(const double* t, KState* s) const {
-        s->beta = t[0] / s->gamma;
-        s->gamma = t[0];
-        s->rs = v.dinv ? t[1] : t[0];
+        if (!v.gpc) {   // with a general M, gamma = r.z follows in CgPcRZ
+            s->beta = t[0] / s->gamma;
+            s->gamma = t[0];
+        }
+        s->rs = (v.dinv || v.gpc) ? t[1] : t[0];
         s->k += 1;
         s->steps = s->k;
         s->done = !(s->rs > s->atol2 && s->k < s->maxiter);
@@ -404,7 +408,7 @@ struct CgP {  // p = z + beta p (the new p of JAX's body, computed after the tes
     __device__ bool skip(const KState* s) const { return s->done; }
     __device__ void load(const KState* s) { beta = s->beta; }
     __device__ void row(int i, double*) const {
-        const double zi = v.dinv ? (double)v.z[i] : (double)v.r[i];
+        const double zi = (v.dinv || v.gpc) ? (double)v.z[i] : (double)v.r[i];
         v.p[i] = (T)(zi + beta * (double)v.p[i]);
     }
     __device__ void fin(const double*, KState*) const {}
@@ -462,9 +466,10 @@ struct GmResid {  // after a cycle: w = M (b - A x); |w|; loop control
         acc[0] += (double)wT * (double)wT;
     }
     __device__ void fin(const double* t, KState* s) const {
-        s->rnorm = safe_norm(t[0], 2.220446049250313e-16);
         s->k += 1;
         s->cycles = s->k;
+        if (v.gpc) return;   // |M r| and the loop test follow in GmPcResid
+        s->rnorm = safe_norm(t[0], 2.220446049250313e-16);
         s->done = !(s->k < s->maxiter && s->rnorm > s->ptol);
     }
 };
@@ -558,6 +563,92 @@ struct GmUpdX {  // x += V[:, :R] y
         v.x[i] = (T)((double)v.x[i] + d);
     }
     __device__ void fin(const double*, KState*) const {}
+};
+
+// ------------------- general preconditioner steps (gpc) ---------------------
+// With a general M (block Jacobi, SSOR, ILU) the host applies M between the
+// kernels; these finish the dot products that involve M's output.
+template <class T>
+struct CgPcStart {  // after z = M r0: p = z; gamma = r.z
+    static constexpr int ND = 1;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        const T zi = v.z[i];
+        v.p[i] = zi;
+        acc[0] += (double)v.r[i] * (double)zi;
+    }
+    __device__ void fin(const double* t, KState* s) const { s->gamma = t[0]; }
+};
+
+template <class T>
+struct CgPcRZ {  // after z = M r: beta = r.z / gamma; gamma = r.z
+    static constexpr int ND = 1;
+    KVec<T> v;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const { acc[0] += (double)v.r[i] * (double)v.z[i]; }
+    __device__ void fin(const double* t, KState* s) const {
+        s->beta = t[0] / s->gamma;
+        s->gamma = t[0];
+    }
+};
+
+template <class T>
+struct GmPcStart {  // w = M r0 (in t), M b (in t2): |M b|, |M r0|, loop test
+    static constexpr int ND = 2;
+    KVec<T> v;
+    const T* mr;
+    const T* mb;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        const T wi = mr[i];
+        v.w[i] = wi;
+        acc[0] += (double)mb[i] * (double)mb[i];
+        acc[1] += (double)wi * (double)wi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        const double atol_eff = fmax(s->tol * s->bnorm, s->atol);
+        s->mbnorm = sqrt(t[0]);
+        s->ptol = s->mbnorm * fmin(1.0, atol_eff / s->bnorm);
+        s->rnorm = safe_norm(t[1], 2.220446049250313e-16);
+        s->done = !(0 < s->maxiter && s->rnorm > s->ptol);
+    }
+};
+
+template <class T>
+struct GmPcNorm {  // Arnoldi: w = M (A v) (in t); v0 = |w|
+    static constexpr int ND = 1;
+    KVec<T> v;
+    const T* mw;
+    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        const T wi = mw[i];
+        v.w[i] = wi;
+        acc[0] += (double)wi * (double)wi;
+    }
+    __device__ void fin(const double* t, KState* s) const { s->v0 = safe_norm(t[0], 2.220446049250313e-16); }
+};
+
+template <class T>
+struct GmPcResid {  // after a cycle: w = M (b - A x) (in t); |w|; loop test
+    static constexpr int ND = 1;
+    KVec<T> v;
+    const T* mw;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState*) {}
+    __device__ void row(int i, double* acc) const {
+        const T wi = mw[i];
+        v.w[i] = wi;
+        acc[0] += (double)wi * (double)wi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->rnorm = safe_norm(t[0], 2.220446049250313e-16);
+        s->done = !(s->k < s->maxiter && s->rnorm > s->ptol);
+    }
 };
 
 }  // namespace wspm

@@ -115,6 +115,22 @@ class Context:
         self._ck(self.lib.wspm_spmv(self.h, ptr(A), ptr(x), ptr(y)))
         return y
 
+    def krylov_pc(self, method: str, precond: Optional[str], A, b, x0, tol=1e-10, atol=0.0, maxiter=100,
+                  restart=30):
+        """Krylov solve with any preconditioner of the reference built from A:
+        none, jacobi, block_jacobi, ssor, ilu."""
+        if method not in _lib.METHOD:
+            raise ValueError(f"Unsupported solver type: {method}")
+        if precond not in _lib.PRECOND:
+            raise ValueError(f"Unsupported preconditioner: {precond}")
+        b, x0 = _f64(b), _f64(x0)
+        x = torch.empty_like(x0)
+        st = _lib.KrylovStats()
+        self._ck(self.lib.wspm_krylov_pc(self.h, _lib.METHOD[method], _lib.PRECOND[precond], ptr(A), ptr(b), ptr(x0),
+                                         ptr(x), float(tol), float(atol), int(maxiter), int(restart), C.byref(st)))
+        return x, {"iters": st.iters, "arnoldi": st.arnoldi, "code": st.code, "refinements": st.refinements,
+                   "res_norm": st.res_norm, "b_norm": st.b_norm}
+
     def krylov(self, method: str, A, b, x0, dinv=None, tol=1e-10, atol=0.0, maxiter=100, restart=30,
                precision="fp64", inner_tol=1e-6, max_refine=50):
         b, x0 = _f64(b), _f64(x0)
@@ -141,7 +157,7 @@ class Context:
             raise ValueError(f"Unsupported solver type: {method}")
         if precond not in _lib.PRECOND:
             raise NotImplementedError(f"preconditioner {precond!r} is not available on the MI355X path "
-                                      "(supported: none, jacobi)")
+                                      "(supported: none, jacobi, block_jacobi, ssor, ilu)")
         prm = _lib.RichardsParams()
         prm.soil_model = _lib.SOIL[soil_model]
         prm.method = _lib.METHOD[method]

@@ -49,8 +49,12 @@ def solve_system(matrix: CSRMatrix, rhs, x0, solver_config):
         raise ValueError(f"Unsupported solver type: {st}")
     if st == "direct":
         raise NotImplementedError("solver 'direct' (dense LU) is not on the MI355X path")
+    if pt in ("block_jacobi", "ssor", "ilu"):
+        x, info = matrix.ctx.krylov_pc(st, pt, matrix.values, rhs, x0, tol=solver_config.tol,
+                                       maxiter=solver_config.maxiter, restart=solver_config.restart)
+        return x, (info["code"] if st == "gmres" else None)
     if pt not in (None, "none", "jacobi"):
-        raise NotImplementedError(f"preconditioner {pt!r} is not on the MI355X path (supported: none, jacobi)")
+        raise ValueError(f"Unsupported preconditioner: {pt}")
     dinv = matrix.dinv if pt == "jacobi" else None
     if pt == "jacobi" and dinv is None:
         raise ValueError("Jacobi needs the assembled inverse diagonal (matrix.dinv)")

@@ -16,8 +16,9 @@ the published algorithms of that module:
   ``restart`` Arnoldi steps unless breakdown; classical Gram-Schmidt (one
   pass, see ``_cgs``); least squares by normal equations + Cholesky; outer loop while k < maxiter and |M(b - A x)| > ptol.
 
-Each function takes ``matvec`` (x -> A x) and ``dinv`` (Jacobi inverse diagonal
-or None) and returns (x, info) where info carries iteration counts.
+Each function takes ``matvec`` (x -> A x) and ``dinv`` (Jacobi inverse diagonal,
+a callable preconditioner, or None) and returns (x, info) where info carries
+iteration counts.
 """
 from __future__ import annotations
 
@@ -28,8 +29,12 @@ EPS = np.finfo(np.float64).eps
 
 
 def _M(dinv):
+    """Preconditioner: None (identity), a Jacobi inverse diagonal, or any
+    callable x -> M x (oracle.precond)."""
     if dinv is None:
         return lambda v: v
+    if callable(dinv):
+        return dinv
     return lambda v: dinv * v
 
 

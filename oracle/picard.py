@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import assembly, krylov, soil
+from . import assembly, precond, krylov, soil
 
 
 @dataclass
@@ -76,7 +76,12 @@ class Stepper:
         vals, b = assembly.assemble(self.pattern, pb.points, th, theta_n, h_m, K, C, dt,
                                     pb.bc_lists, pb.Ks_elem)
         A = self.pattern.to_scipy(vals)
-        dinv = assembly.jacobi_inverse(self.pattern, vals) if pb.precond == "jacobi" else None
+        if pb.precond == "jacobi":
+            dinv = assembly.jacobi_inverse(self.pattern, vals)
+        elif pb.precond in ("block_jacobi", "ssor", "ilu"):
+            dinv = precond.make(pb.precond, A)
+        else:
+            dinv = None
         h, info = krylov.solve(pb.solver, A.dot, b, h_m, dinv, pb.tol, 0.0, pb.maxiter, pb.restart)
         self.krylov_iters.append(info["iters"])
         return h

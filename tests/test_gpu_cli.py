@@ -57,7 +57,17 @@ def test_solve_system_mirror_matches_oracle():
     with pytest.raises(ValueError):
         solve_system(A, b, b, SolverParameters(solver_type="lu"))
     with pytest.raises(NotImplementedError):
-        solve_system(A, b, b, SolverParameters(solver_type="cg", precond_type="ilu"))
+        solve_system(A, b, b, SolverParameters(solver_type="direct"))
+    # the general preconditioners go through wspm_krylov_pc
+    from oracle import precond as opc
+    x, _ = solve_system(A, b, torch.as_tensor(h, device="cuda"),
+                        SolverParameters(solver_type="bicgstab", precond_type="ilu"))
+    pat = oasm.Pattern(el, n)
+    Co, Ko, tho = osoil.van_genuchten(h, vg)
+    vals, bo = oasm.assemble(pat, pts, tho, tho, h, Ko, Co, 1e-3, [(top, np.full(top.size, -75.0))])
+    S = pat.to_scipy(vals)
+    xo, _ = okr.solve("bicgstab", S.dot, bo, h, opc.make("ilu", S))
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-10
     ctx.close()
 
 

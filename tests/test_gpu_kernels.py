@@ -236,3 +236,24 @@ def test_deterministic_repeat():
     for a, c in zip(outs[0], outs[1]):
         assert np.array_equal(a, c)
     ctx.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab", "gmres"])
+@pytest.mark.parametrize("precond", ["block_jacobi", "ssor", "ilu"])
+def test_krylov_general_preconditioners_match_oracle(method, precond):
+    """Block Jacobi, SSOR and ILU applied between the Krylov kernels (level-
+    scheduled triangular sweeps) against the oracle's factorisations."""
+    import torch
+    from oracle import precond as opc
+    print(f"[case] krylov {method} {precond}", flush=True)
+    pts, el, pat, vals, b, h_m, _ = _gardner_system(14, 2, 0.1)
+    A = pat.to_scipy(vals)
+    x_o, info = okr.solve(method, A.dot, b, h_m, opc.make(precond, A), 1e-10, 0.0, 100, 30)
+    ctx = _ctx(pts, el)
+    x_g, st = ctx.krylov_pc(method, precond, torch.as_tensor(vals, device="cuda"), torch.as_tensor(b, device="cuda"),
+                            torch.as_tensor(h_m, device="cuda"), tol=1e-10, maxiter=100, restart=30)
+    ctx.close()
+    assert st["iters"] == info["iters"], (st, info)
+    if method == "gmres":
+        assert st["arnoldi"] == info["arnoldi"]
+    assert rel_l2(x_g.cpu().numpy(), x_o) <= 1e-10

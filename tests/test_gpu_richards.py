@@ -127,3 +127,20 @@ def test_mixed_precision_vs_fp64(dim, solver):
         err_t = rel_l2(gmx["theta"][k], g64["theta"][k])
         print(f"  step {k}: rel L2 h {err_h:.2e} theta {err_t:.2e}")
         assert err_h <= 1e-8 and err_t <= 1e-8
+
+
+@pytest.mark.parametrize("solver,precond", [("cg", "ilu"), ("bicgstab", "ssor"), ("gmres", "block_jacobi")])
+def test_tracy_general_preconditioners(solver, precond, golden):
+    """Test1 mesh 25 (published rule) with the preconditioners applied between
+    the Krylov kernels: the oracle trajectory, and the converged published
+    metrics (53 Picard iterations; resultsRE2D/benchmark_results_cpu_Test1_WA.csv
+    holds the same figures for its *,ilu rows)."""
+    from jax_wspm_amd.problems import from_config
+    print(f"[case] Test1 mesh 25 {solver}+{precond}", flush=True)
+    pb = from_config(_cfg("Test1", solver=solver, precond=precond))
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    _compare(g, o)
+    assert int(np.sum(g["iterations"])) == 53
+    e = tracy.final_errors(pb.points, g["pressure_head"][-1], g["theta"][-1], g["times"][-1], 1)
+    assert f"{e['l2_pressure']:.4e}" == "1.7613e-01" and f"{e['l2_saturation']:.4e}" == "5.0311e-03"
