@@ -274,6 +274,30 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm,
 int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylov_iters,
                             int32_t* converged, double* err);
 
+/* ---- Diagnostics (2D, single-mesh contexts) ---- */
+
+/* Domain integral of the nodal field f (dev [n]) with the 3-point rule of
+ * integrate_2d_domain (src/solvers/.ipynb_checkpoints/
+ * richards_mass_balance-checkpoint.py:25-65): sum over triangles of
+ * |det J| * sum_q (1/3) f(x_q), i.e. twice the integral, as the reference.
+ * out: host double.  The mass-balance driver of that file (:210-335)
+ * evaluates it on theta and on the exact theta every step. */
+int32_t wspm_domain_integral(wspm_ctx* ctx, const double* f, double* out);
+/* calculate_element_size (src/models/transport_utilities.py:196-204): the
+ * smallest edge length of every triangle; size dev [E]. */
+int32_t wspm_element_size(wspm_ctx* ctx, double* size);
+/* calculate_peclet_courant (transport_utilities.py:206-242) on the node
+ * fields of wspm_transport_fields (all dev [n]).  Per element, with h its
+ * smallest edge and v = q / theta at its nodes: Pe = max over the nodes of
+ * |v_x| h / (2 Dxx) and |v_z| h / (2 Dzz); Cr = max of |v| dt / h.
+ * out (host [2]): all_elements = 0 reproduces the reference's result
+ * (max(Pe_0, Cr_0), max(Pe_1, Cr_1)), which its tuple iteration at :237-240
+ * yields; all_elements = 1 gives (max_e Pe_e, max_e Cr_e).  NaN propagates
+ * as in jnp.max.  pe_elem, cr_elem: dev [E] per-element values, or NULL. */
+int32_t wspm_peclet_courant(wspm_ctx* ctx, const double* qx, const double* qz, const double* theta,
+                            const double* dxx, const double* dzz, double dt, int32_t all_elements,
+                            double* pe_elem, double* cr_elem, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,9 @@ PROTOTYPES = {
     "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),
+    "wspm_domain_integral": (_I, [_P, _P, C.POINTER(_D)]),
+    "wspm_element_size": (_I, [_P, _P]),
+    "wspm_peclet_courant": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _P, _P, C.POINTER(_D)]),
 }
 
 _lib = None

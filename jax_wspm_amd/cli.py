@@ -62,6 +62,10 @@ def parse_args(argv=None):
     p.add_argument("--ks-length", type=float, default=0.05)
     p.add_argument("--ks-seed", type=int, default=0)
     p.add_argument("--history", default="all", help="'all', 'last' or k (every k-th step) kept on the host")
+    p.add_argument("--mass-balance", action="store_true",
+                   help="Test1-3: record the domain integrals of theta and of the exact theta every step")
+    p.add_argument("--diagnostics", action="store_true",
+                   help="SoluteTest/NitrogenTest: record the largest element Peclet and Courant numbers every step")
     return p.parse_args(argv)
 
 
@@ -112,7 +116,16 @@ def run(args):
         cls = CoupledSolver if args.test_case == "SoluteTest" else NitrogenSolver
     else:
         cls = RichardsSolver3D if args.test_case == "Test3D" else RichardsSolver
-    solver = cls(cfg, mesh_dir=args.mesh_dir, history=_history(args.history))
+    extra = {}
+    if args.mass_balance:
+        if args.test_case not in ("Test1", "Test2", "Test3"):
+            raise ValueError("--mass-balance needs Test1, Test2 or Test3")
+        extra["mass_balance"] = True
+    if args.diagnostics:
+        if args.test_case not in ("SoluteTest", "NitrogenTest"):
+            raise ValueError("--diagnostics needs SoluteTest or NitrogenTest")
+        extra["diagnostics"] = True
+    solver = cls(cfg, mesh_dir=args.mesh_dir, history=_history(args.history), **extra)
     results = solver.solve()
     solver.ctx.close()
 
@@ -145,6 +158,9 @@ def run(args):
         for k in ("l2_pressure", "linf_pressure", "l2_saturation", "linf_saturation", "l2_relative_pressure",
                   "l2_relative_saturation"):
             save[k] = float(results[k])
+    for k in ("mass_balance_numerical", "mass_balance_exact", "mass_balance_error", "peclet_max", "courant_max"):
+        if k in results:
+            save[k] = np.asarray(results[k])
     res_file = out_dir / f"numerical_results_{args.test_case}.npz"
     np.savez(res_file, **save)
     steps = len(results["times"])

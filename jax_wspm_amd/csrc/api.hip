@@ -18,6 +18,7 @@
 
 #include "assemble.h"
 #include "adjoint.h"
+#include "diagnostics.h"
 #include "krylov_kernels.h"
 #include "precond.h"
 #include "transport.h"
@@ -464,6 +465,7 @@ struct wspm_ctx {
                *dldh = nullptr, *ge = nullptr, *gn = nullptr;
         int32_t* tpos = nullptr;
     } adj;
+    double* diag_out = nullptr;     // device scalars of the diagnostics reductions
     // general preconditioners (allocated on first use)
     struct {
         Levels lo, up;
@@ -2006,6 +2008,67 @@ int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylo
     if (krylov_iters) HIPCHK(hipMemcpyAsync(krylov_iters, ik, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (converged) HIPCHK(hipMemcpyAsync(converged, ic, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (err) HIPCHK(hipMemcpyAsync(err, er, ni * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics
+// ---------------------------------------------------------------------------
+static int diag_check(wspm_ctx* ctx) {
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "2D only, as the reference");
+    if (ctx->E < 1) return fail(ctx, WSPM_EINVAL, "empty mesh");
+    return WSPM_OK;
+}
+
+static DiagArgs diag_args(wspm_ctx* ctx) {
+    DiagArgs a{};
+    a.E = ctx->E;
+    a.conn4 = ctx->conn4;
+    a.nrec = ctx->nrec;
+    a.partials = ctx->partials;
+    a.prow = ctx->prow;
+    a.grid = std::max(1, std::min((ctx->E + BLOCK - 1) / BLOCK, ctx->prow));
+    return a;
+}
+
+int32_t wspm_domain_integral(wspm_ctx* ctx, const double* f, double* out) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!f || !out) return fail(ctx, WSPM_EINVAL, "null argument");
+    if (int rc = diag_check(ctx)) return rc;
+    API_BEGIN
+    if (!ctx->diag_out) ctx->diag_out = dalloc<double>(ctx, 4);
+    HIPCHK(launch_domain_integral(diag_args(ctx), f, ctx->diag_out, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->diag_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_element_size(wspm_ctx* ctx, double* size) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!size) return fail(ctx, WSPM_EINVAL, "null argument");
+    if (int rc = diag_check(ctx)) return rc;
+    API_BEGIN
+    HIPCHK(launch_element_size(diag_args(ctx), size, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_peclet_courant(wspm_ctx* ctx, const double* qx, const double* qz, const double* theta,
+                            const double* dxx, const double* dzz, double dt, int32_t all_elements,
+                            double* pe_elem, double* cr_elem, double* out) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!qx || !qz || !theta || !dxx || !dzz || !out) return fail(ctx, WSPM_EINVAL, "null argument");
+    if (int rc = diag_check(ctx)) return rc;
+    API_BEGIN
+    if (!ctx->diag_out) ctx->diag_out = dalloc<double>(ctx, 4);
+    const PeCrArgs p{qx, qz, theta, dxx, dzz, dt, pe_elem, cr_elem, all_elements != 0};
+    HIPCHK(launch_peclet_courant(diag_args(ctx), p, ctx->diag_out, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->diag_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END

@@ -210,6 +210,32 @@ class Context:
                                                 *(ptr(o) for o in out)))
         return tuple(out)
 
+    def domain_integral(self, f) -> float:
+        """sum_e |det J_e| * mean_q f over the 3-point rule (twice the integral,
+        as richards_mass_balance-checkpoint.py:25-65)."""
+        f = _f64(f)
+        out = C.c_double()
+        self._ck(self.lib.wspm_domain_integral(self.h, ptr(f), C.byref(out)))
+        return out.value
+
+    def element_size(self) -> torch.Tensor:
+        out = torch.empty(self.n_elems, dtype=torch.float64, device=self.device)
+        self._ck(self.lib.wspm_element_size(self.h, ptr(out)))
+        return out
+
+    def peclet_courant(self, qx, qz, theta, dxx, dzz, dt, *, all_elements=False, per_element=False):
+        """(Pe, Cr) of transport_utilities.py:206-242; see wspm_peclet_courant.
+        per_element: also return the per-element arrays."""
+        args = [_f64(v) for v in (qx, qz, theta, dxx, dzz)]
+        pe = cr = None
+        if per_element:
+            pe = torch.empty(self.n_elems, dtype=torch.float64, device=self.device)
+            cr = torch.empty_like(pe)
+        out = (C.c_double * 2)()
+        self._ck(self.lib.wspm_peclet_courant(self.h, *(ptr(a) for a in args), float(dt), 1 if all_elements else 0,
+                                              ptr(pe), ptr(cr), out))
+        return (out[0], out[1], pe, cr) if per_element else (out[0], out[1])
+
     def assemble_transport(self, fields, c_n, dt, *, lumped, mu=0.0, rho_kd=0.0, mu_src=0.0, c_src=None,
                            cauchy=None, K=None, want_dinv=True):
         """One species system on the context's pattern.  fields: the 7-tuple of

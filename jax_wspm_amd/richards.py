@@ -8,6 +8,12 @@ keeps the host time loop of the reference: dt adaptation by iteration count
 before ``t += dt_new`` (richards_solver_v2.py:316,323), and theta stored at the
 end (2D v2) or at the start (3D, and the earlier 2D solver that produced the
 published CSVs) of each step.
+
+``mass_balance=True`` (Test1-3) adds the records of the mass-balance solver
+(src/solvers/.ipynb_checkpoints/richards_mass_balance-checkpoint.py:210-335):
+per step the domain integral of theta(h) at the end of the step and of the
+exact Tracy theta at the step's recorded time (``wspm_domain_integral``), and
+|1 - numerical / exact| * 100.
 """
 from __future__ import annotations
 
@@ -38,7 +44,7 @@ class RichardsSolver:
     step plus the last."""
 
     def __init__(self, config: Union[SimulationConfig, Problem], mesh_dir: Optional[str] = None,
-                 history: Union[str, int] = "all"):
+                 history: Union[str, int] = "all", mass_balance: bool = False):
         self.problem = config if isinstance(config, Problem) else from_config(config, mesh_dir)
         self.config = config
         self.history = history
@@ -47,6 +53,21 @@ class RichardsSolver:
         self.ctx.set_dirichlet(pb.bc_lists)
         self.points = pb.points
         self.mesh_info = pb.mesh_info
+        self.mass_balance = bool(mass_balance)
+        if self.mass_balance and not (pb.name in ("Test1", "Test2", "Test3") and pb.soil_model == "gardner"):
+            raise ValueError("mass balance needs a Tracy test case (Test1, Test2, Test3)")
+
+    def _exact_theta(self, t: float) -> torch.Tensor:
+        """theta of Tracy's solution at time t (exact_solutions.py, as the
+        mass-balance solver evaluates it at :272-276)."""
+        from .tracy import exact_solution
+        pb = self.problem
+        alpha0, ts, tr, ks = (float(v) for v in pb.phi[:4])
+        hd = -15.24
+        eps0 = np.exp(alpha0 * hd)
+        d = alpha0 * (ts - tr) / ks
+        _, S = exact_solution(pb.points[:, 0], pb.points[:, 1], t, int(pb.name[-1]), -hd, alpha0, eps0, d)
+        return torch.as_tensor((ts - tr) * S + tr, dtype=torch.float64, device=self.ctx.device)
 
     def init_state(self):
         pb = self.problem
@@ -71,6 +92,8 @@ class RichardsSolver:
         dt = float(pb.dt0)
         rec = {k: [] for k in ("pressure_head", "theta", "times", "iterations", "errors", "dt_values",
                                "krylov_iterations")}
+        if self.mass_balance:
+            rec.update(mass_balance_numerical=[], mass_balance_exact=[])
         k = 0
         last = None
         while True:
@@ -89,6 +112,10 @@ class RichardsSolver:
             rec["errors"].append(st["err"])
             rec["dt_values"].append(dt_new)
             rec["krylov_iterations"].append(st["krylov_iters"])
+            if self.mass_balance:
+                th_end = self.ctx.richards_state()[1]
+                rec["mass_balance_numerical"].append(self.ctx.domain_integral(th_end))
+                rec["mass_balance_exact"].append(self.ctx.domain_integral(self._exact_theta(t)))
             if self._keep(k):
                 h, th, thn = self.ctx.richards_state()
                 rec["pressure_head"].append(h.cpu().numpy())
@@ -113,6 +140,9 @@ class RichardsSolver:
         })
         if out["theta"].size:
             out["final_theta"] = out["theta"][-1]
+        if self.mass_balance:
+            from .diagnostics import mass_balance_error
+            out["mass_balance_error"] = mass_balance_error(out["mass_balance_numerical"], out["mass_balance_exact"])
         return out
 
 

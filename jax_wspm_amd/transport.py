@@ -73,9 +73,10 @@ class _TransportBase:
     freedrain = True
 
     def __init__(self, config: SimulationConfig, mesh_dir: Optional[str] = None, points=None, triangles=None,
-                 history: Any = "all"):
+                 history: Any = "all", diagnostics: bool = False):
         self.config = config
         self.history = history
+        self.diagnostics = bool(diagnostics)
         self.points, self.triangles, self.mesh_info = _mesh(config, mesh_dir, points, triangles)
         self.inlet, self.outlet = transport_boundary_nodes(self.points, config.test_case)
         self.ctx = Context(self.points, self.triangles)
@@ -100,6 +101,15 @@ class _TransportBase:
         if self.history == "last":
             return last
         return last or k % int(self.history) == 0
+
+    def _diag(self, out, fields, dt):
+        """Whole-mesh maxima of the element Peclet / Courant numbers of this
+        step's fields (transport_utilities.py:206-242 over every element)."""
+        if self.diagnostics:
+            pe, cr = self.ctx.peclet_courant(fields[0], fields[1], fields[6], fields[3], fields[5], dt,
+                                             all_elements=True)
+            out.setdefault("peclet_max", []).append(pe)
+            out.setdefault("courant_max", []).append(cr)
 
     def _done(self, t, k):
         ts = self.config.time
@@ -148,6 +158,7 @@ class NitrogenSolver(_TransportBase):
             out["errors"].append(st["err"])
             out["dt_values"].append(dt)
             out["krylov_iterations"].append(st["krylov_iters"])
+            self._diag(out, fields, dt)
             c = new
             t += dt
             dt = dt_new
@@ -192,6 +203,7 @@ class CoupledSolver(_TransportBase):
             out["errors"].append(st["err"])
             out["dt_values"].append(dt_new)
             out["krylov_iterations"].append(st["krylov_iters"])
+            self._diag(out, fields, dt)
             c = c_new
             t += dt
             dt = dt_new
