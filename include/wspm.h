@@ -43,6 +43,11 @@
  *   wspm_adj_step          no counterpart (the reference has no gradient code):
  *                          one backward step of the discrete adjoint of
  *                          wspm_richards_step
+ *   wspm_adj_transport_step  no counterpart: one backward step of the adjoint
+ *                          of the nitrogen species steps
+ *   wspm_domain_integral,  richards_mass_balance-checkpoint.py:25-65,
+ *   wspm_element_size,     src/models/transport_utilities.py:196-242
+ *   wspm_peclet_courant    (diagnostics)
  *   wspm_ens_*             no single counterpart: the reference runs one
  *                          instance per process (src/cli.py:87-150); an
  *                          ensemble context time-steps many independent
@@ -199,9 +204,12 @@ int32_t wspm_edge_set(wspm_ctx* ctx, int32_t list, int32_t n, const int32_t* nod
 int32_t wspm_richards_edges(wspm_ctx* ctx, int32_t inlet_neumann, double neumann_flux,
                             int32_t outlet_freedrain);
 /* Node fields for transport from h (dev arrays [n]): water fluxes qx, qz, |q|,
- * dispersion Dxx, Dxz, Dzz and theta; van Genuchten phi (host double[6]). */
-int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, double DL,
-                              double DT, double Dm, double* qx, double* qz, double* absq,
+ * dispersion Dxx, Dxz, Dzz and theta; van Genuchten phi (host double[6]).
+ * Ks_elem (dev [E]) or NULL: with a per-element Ks field (an extension; the
+ * reference's Ks is phi[5]) a node's K is the soil model's K times the mean
+ * Ks_elem of its incident elements. */
+int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, const double* Ks_elem,
+                              double DL, double DT, double Dm, double* qx, double* qz, double* absq,
                               double* dxx, double* dxz, double* dzz, double* theta);
 typedef struct {
     int32_t lumped;           /* 1: row-sum lumped mass (nitrogen), 0: consistent (solute) */
@@ -232,7 +240,9 @@ int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp,
  * dtheta/d(alpha, n) parts, from dldy = dl/dy (y = theta(h_k)[obs]).
  * Device pointers: Ks_elem [E], h = h_k, h_prev = h_{k-1}, obs [n_obs] int32,
  * dldy [n_obs] (both NULL when n_obs = 0), lam_next [n] (NULL for the last
- * step), lam [n] out, grad_logks [E] and grad_an [2] accumulated. */
+ * step), dldh_in [n] (NULL, or further dL/dh_k terms, e.g. from
+ * wspm_adj_transport_step), lam [n] out, grad_logks [E] and grad_an [2]
+ * accumulated. */
 typedef struct {
     double dt;
     double phi[6];        /* alpha, theta_s, theta_r, n, (m = 1 - 1/n), (Ks: Ks_elem) */
@@ -242,8 +252,41 @@ typedef struct {
 } wspm_adj_params;
 int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* Ks_elem,
                       const double* h, const double* h_prev, const int32_t* obs,
-                      const double* dldy, const double* lam_next, double* lam,
-                      double* grad_logks, double* grad_an, wspm_krylov_stats* stats);
+                      const double* dldy, const double* lam_next, const double* dldh_in,
+                      double* lam, double* grad_logks, double* grad_an, wspm_krylov_stats* stats);
+
+/* Adjoint of the nitrogen species steps NH4 -> NO2 -> NO3 that follow the
+ * Richards step k (no counterpart: the reference's nitrogen_solver.py:148-238
+ * is never differentiated).  Forward model at fixed dt, lumped mass, Cauchy
+ * inlet (the inlet list of wspm_edge_set), fields of wspm_transport_fields
+ * with Ks_elem:
+ *   A_s(f_k) c_s^k = M_s(f_k) c_s^{k-1} / dt - mu_{s-1} int theta c_{s-1}^k N.
+ * One call solves the three transposed systems (NO3 first, Jacobi-BiCGSTAB):
+ *   A_s^T mu_s^k = dL/dc_s^k + M_s(f_{k+1}) mu_s^{k+1} / dt
+ *                  - (dR_{s+1}/dc_s)^T mu_{s+1}^k
+ * and carries -sum_s mu_s^T dR_s/df_k through the fields to dldh (out: the
+ * dL/dh_k to pass to wspm_adj_step's dldh_in), grad_logks and grad_an.
+ * Device pointers: Ks_elem [E]; h = h_k; h_next = h_{k+1} (NULL at the last
+ * step); c, c_prev [3n] species at steps k and k-1 (NH4, NO2, NO3 blocks);
+ * obs [n_obs], dldy [n_obs] = dL/dNO3^k[obs]; lam_next [3n] (NULL at the last
+ * step); lam [3n] out; dldh [n] out; grad_logks [E], grad_an [2] accumulated.
+ * stats: the three solves' iterations summed, the largest residual. */
+typedef struct {
+    double dt;
+    double phi[6];        /* van Genuchten as wspm_adj_params (Ks: Ks_elem)     */
+    double DL, DT, Dm;    /* dispersivities and molecular diffusion              */
+    double mu[3];         /* first-order rates of NH4, NO2, NO3                  */
+    double rho_kd;        /* rho * Kd of NH4 (sorption)                          */
+    double cauchy_flux;   /* Cauchy flux on the inlet list (0: no inlet term)    */
+    double tol;           /* relative tolerance of the transpose solves          */
+    int32_t maxiter;
+    int32_t n_obs;
+} wspm_adj_tr_params;
+int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, const double* Ks_elem,
+                                const double* h, const double* h_next, const double* c,
+                                const double* c_prev, const int32_t* obs, const double* dldy,
+                                const double* lam_next, double* lam, double* dldh,
+                                double* grad_logks, double* grad_an, wspm_krylov_stats* stats);
 
 /* Ensembles ------------------------------------------------------------------
  * n_inst independent 2D instances (own mesh, soil parameters, Dirichlet rows)

@@ -55,6 +55,12 @@ class AdjParams(C.Structure):
                 ("n_obs", C.c_int32)]
 
 
+class AdjTrParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("phi", C.c_double * 6), ("DL", C.c_double), ("DT", C.c_double),
+                ("Dm", C.c_double), ("mu", C.c_double * 3), ("rho_kd", C.c_double), ("cauchy_flux", C.c_double),
+                ("tol", C.c_double), ("maxiter", C.c_int32), ("n_obs", C.c_int32)]
+
+
 EDGE = {"inlet": 0, "outlet": 1}
 
 _P = C.c_void_p
@@ -85,10 +91,13 @@ PROTOTYPES = {
     "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "wspm_edge_set": (_I, [_P, _I, _I, _P]),
     "wspm_richards_edges": (_I, [_P, _I, _D, _I]),
-    "wspm_transport_fields": (_I, [_P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
+    "wspm_transport_fields": (_I, [_P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "wspm_assemble_transport": (_I, [_P, C.POINTER(TransportParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      _P]),
-    "wspm_adj_step": (_I, [_P, C.POINTER(AdjParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(KrylovStats)]),
+    "wspm_adj_step": (_I, [_P, C.POINTER(AdjParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                           C.POINTER(KrylovStats)]),
+    "wspm_adj_transport_step": (_I, [_P, C.POINTER(AdjTrParams), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     C.POINTER(KrylovStats)]),
     "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),

@@ -14,6 +14,7 @@
 #include <cstdint>
 
 #include "assemble.h"
+#include "transport.h"
 #include "wspm_device.h"
 
 namespace wspm {
@@ -60,5 +61,57 @@ hipError_t launch_adj_params(const AdjArgs& a, const double* lam, double* grad_l
 // Observations: dldh[obs[j]] += dldy[j] dtheta/dh; gan[0..1] += sum dldy dtheta/d(alpha, n).
 hipError_t launch_adj_obs(const AdjArgs& a, int n_obs, const int32_t* obs, const double* dldy, double* dldh,
                           double* gan, hipStream_t s);
+
+// JT[k] = J[tpos[k]] for nnz CSR slots.
+hipError_t launch_transpose_values(const double* J, const int32_t* tpos, double* JT, int64_t nnz, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Adjoint of the nitrogen species steps (NH4 -> NO2 -> NO3) that follow the
+// Richards step k (src/solvers/nitrogen_solver.py:148-238 at fixed dt, lumped
+// mass, Cauchy inlet).  Species s solves R_s = A_s(f_k) c_s^k - b_s = 0 with
+//   A_s = M_s / dt + K_s(f_k) (+ Cauchy edge terms, constant),
+//   b_s = M_s c_s^{k-1} / dt - mu_{s-1} int theta c_{s-1}^k N (s > 0),
+// where f_k = (theta, qx, qz, Dxx, Dxz, Dzz) are the node fields of h_k
+// (k_transport_fields) and M_s the row-sum lumped -int (theta + rho Kd_s) N N^T.
+// The adjoints mu_s^k solve A_s^T mu_s^k = dL/dc_s^k + M_s^{k+1} mu_s^{k+1} / dt
+// - (dR_{s+1}^k / dc_s^k)^T mu_{s+1}^k, NO3 first; the fields then receive
+// g_f = -sum_s mu_s^T dR_s/df, which is carried to h_k (dL/dh_k, handed to
+// the Richards adjoint of the same step), to alpha, n and to log Ks_e.
+// ---------------------------------------------------------------------------
+struct TrAdjArgs {
+    int n, E;
+    const int32_t* inc_ptr;
+    const int2* inc;
+    const int4* conn4;
+    const double4* nrec;
+    TrQuad tq;
+    const double* ks_e;         // [E]
+    const double* h;            // h_k
+    SoilParams sp;              // van Genuchten, Ks = 1, m tied to n
+    double DL, DT, Dm;
+    double inv_dt;
+    double mu[3];               // first-order rates of the species
+    double rho_kd;              // sorption of species 0
+    const double *th, *qx, *qz; // fields of h_k
+    const double* c;            // [3n] species at step k
+    const double* cp;           // [3n] species at step k-1
+    const double* lam;          // [3n] species adjoints at step k
+};
+
+// Lumped species mass of step k+1 from its theta: ml_s = mth + rho_kd_s * mT.
+hipError_t launch_tr_adj_mass(const TrAdjArgs& a, const double* th_next, double* mth, double* mT, hipStream_t s);
+// rhs of species s: M_s^{k+1} lam_next / dt (lam_next nullable) plus the
+// coupling term of species s + 1 (lam_up = its adjoint, nullable).
+hipError_t launch_tr_adj_rhs(const TrAdjArgs& a, int species, const double* mth, const double* mT,
+                             const double* lam_next, const double* lam_up, double* rhs, hipStream_t s);
+// rhs[obs[j]] += dldy[j]
+hipError_t launch_tr_adj_obs(int n_obs, const int32_t* obs, const double* dldy, double* rhs, hipStream_t s);
+// Field gradients gf[6n] = (theta, qx, qz, Dxx, Dxz, Dzz) of -sum_s mu_s^T R_s.
+hipError_t launch_tr_adj_fields(const TrAdjArgs& a, double* gf, hipStream_t s);
+// Fields -> h_k, alpha, n, log Ks: dldh[n] (written), gn[2n] (alpha / n
+// terms per node, summed by the caller), grad_logks[E] (added to); scratch
+// w[3n].
+hipError_t launch_tr_adj_chain(const TrAdjArgs& a, const double* gf, double* w, double* dldh, double* gn,
+                               double* grad_logks, hipStream_t s);
 
 }  // namespace wspm

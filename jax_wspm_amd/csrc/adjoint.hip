@@ -104,7 +104,8 @@ __device__ inline double adj_geometry(const double (&px)[3], const double (&py)[
     return det;
 }
 
-__device__ inline void load_element(const AdjArgs& a, int e, int (&nd)[3], double (&px)[3], double (&py)[3]) {
+template <class Args>
+__device__ inline void load_element(const Args& a, int e, int (&nd)[3], double (&px)[3], double (&py)[3]) {
     const int4 cn = a.conn4[e];
     nd[0] = cn.x;
     nd[1] = cn.y;
@@ -270,6 +271,239 @@ __global__ void k_adj_obs(AdjArgs a, int n_obs, const int32_t* obs, const double
     gan[1] += sn;
 }
 
+// ---------------------------------------------------------------------------
+// Transport adjoint (nitrogen species chain)
+// ---------------------------------------------------------------------------
+
+// Lumped species mass at step k+1: per incidence (e, a) of node i,
+// sum_c m_ac = det (-sum_{c,u} U[a][c][u] theta_u - rho_kd sum_c T[a][c])
+// (the forward's m, k_assemble_transport).
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_mass(TrAdjArgs a, const double* th, double* mth, double* mT) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const TrQuad& Q = a.tq;
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = a.inc_ptr[i]; q < a.inc_ptr[i + 1]; ++q) {
+        const int2 ic = a.inc[q];
+        const int la = ic.x & 3;
+        int nd[3];
+        double px[3], py[3], Gx[3], Gz[3];
+        load_element(a, ic.x >> 2, nd, px, py);
+        const double det = adj_geometry(px, py, Gx, Gz);
+        double u = 0.0, t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            t += Q.T[la][c];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) u += Q.U[la][c][v] * th[nd[v]];
+        }
+        s0 += -u * det;
+        s1 += -t * det;
+    }
+    mth[i] = s0;
+    mT[i] = s1;
+}
+
+// rhs_i = (mth_i + rho_kd_s mT_i) lam_next_i / dt
+//         - mu_s sum_{(e,b) at i} det sum_{a,c} lam_up_a U[a][c][b] theta_c
+// The second term is -(dR_{s+1}/dc_s)^T lam_up: R_{s+1} carries
+// +mu_s det sum_{c,u} U[a][c][u] theta_c c_s,u (minus the source of b).
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_rhs(TrAdjArgs a, int sp, const double* mth, const double* mT,
+                                                      const double* lam_next, const double* lam_up, double* rhs) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    double r = 0.0;
+    if (lam_next) r = a.inv_dt * (mth[i] + (sp == 0 ? a.rho_kd : 0.0) * mT[i]) * lam_next[i];
+    if (lam_up) {
+        const TrQuad& Q = a.tq;
+        double s = 0.0;
+        for (int q = a.inc_ptr[i]; q < a.inc_ptr[i + 1]; ++q) {
+            const int2 ic = a.inc[q];
+            const int lb = ic.x & 3;
+            int nd[3];
+            double px[3], py[3], Gx[3], Gz[3];
+            load_element(a, ic.x >> 2, nd, px, py);
+            const double det = adj_geometry(px, py, Gx, Gz);
+            double t = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) t += lam_up[nd[aa]] * Q.U[aa][c][lb] * a.th[nd[c]];
+            s += det * t;
+        }
+        r += -a.mu[sp] * s;
+    }
+    rhs[i] = r;
+}
+
+__global__ void k_tr_adj_obs(int n_obs, const int32_t* obs, const double* dldy, double* rhs) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int j = 0; j < n_obs; ++j) rhs[obs[j]] += dldy[j];   // serial: repeated nodes add up
+}
+
+// Field gradients at node j: g_f[j] = -sum_s sum_{(e,v) at j} d(phi_s,e)/d f_v,
+// phi_s,e = sum_a mu_a R_a^e.  Per element (G, det as the forward):
+//   d/dDxx_v = -det ck_v (mu.Gx)(Gx.c),  d/dDzz_v = -det ck_v (mu.Gz)(Gz.c),
+//   d/dDxz_v = -det ck_v ((mu.Gx)(Gz.c) + (mu.Gz)(Gx.c)),
+//   d/dqx_v  = -det (sum_a mu_a T[a][v]) (Gx.c),  d/dqz_v likewise with Gz,
+//   d/dtheta_v = det ( mu_s sum_{a,c} mu_a c_c U[a][c][v]
+//                     - sum_a mu_a (c_a - cp_a) sum_c U[a][c][v] / dt
+//                     + mu_{s-1} sum_{a,u} mu_a U[a][v][u] c_{s-1,u} ).
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_fields(TrAdjArgs a, double* gf) {
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= a.n) return;
+    const TrQuad& Q = a.tq;
+    const size_t n = (size_t)a.n;
+    double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = a.inc_ptr[j]; q < a.inc_ptr[j + 1]; ++q) {
+        const int2 ic = a.inc[q];
+        const int v = ic.x & 3;
+        int nd[3];
+        double px[3], py[3], Gx[3], Gz[3];
+        load_element(a, ic.x >> 2, nd, px, py);
+        const double det = adj_geometry(px, py, Gx, Gz);
+        for (int sp = 0; sp < 3; ++sp) {
+            double m[3], c[3], cp[3], cs[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[k] = a.lam[sp * n + nd[k]];
+                c[k] = a.c[sp * n + nd[k]];
+                cp[k] = a.cp[sp * n + nd[k]];
+                cs[k] = sp > 0 ? a.c[(sp - 1) * n + nd[k]] : 0.0;
+            }
+            double mGx = 0.0, mGz = 0.0, cGx = 0.0, cGz = 0.0, mT = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                mGx += m[k] * Gx[k];
+                mGz += m[k] * Gz[k];
+                cGx += c[k] * Gx[k];
+                cGz += c[k] * Gz[k];
+                mT += m[k] * Q.T[k][v];
+            }
+            double rx = 0.0, dm = 0.0, sr = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < 3; ++aa) {
+                double uc = 0.0, us = 0.0, u1 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    uc += Q.U[aa][k][v] * c[k];
+                    us += Q.U[aa][v][k] * cs[k];
+                    u1 += Q.U[aa][k][v];
+                }
+                rx += m[aa] * uc;
+                dm += m[aa] * (c[aa] - cp[aa]) * u1;
+                sr += m[aa] * us;
+            }
+            const double ck = Q.ck[v];
+            g[0] -= det * (a.mu[sp] * rx - a.inv_dt * dm + (sp > 0 ? a.mu[sp - 1] : 0.0) * sr);
+            g[1] -= -det * mT * cGx;
+            g[2] -= -det * mT * cGz;
+            g[3] -= -det * ck * mGx * cGx;
+            g[4] -= -det * ck * (mGx * cGz + mGz * cGx);
+            g[5] -= -det * ck * mGz * cGz;
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) gf[f * n + j] = g[f];
+}
+
+// Node gradient of h averaged over the incident elements (b/c form), the
+// mean incident Ks, and the incidence count (as k_transport_fields).
+__device__ inline void tr_node_grad(const TrAdjArgs& a, int i, double& gx, double& gz, double& ksn, double& cnt) {
+    double sx = 0.0, sz = 0.0, ks = 0.0;
+    const int q0 = a.inc_ptr[i], q1 = a.inc_ptr[i + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int e = a.inc[q].x >> 2;
+        ks += a.ks_e[e];
+        int nd[3];
+        double x[3], y[3];
+        load_element(a, e, nd, x, y);
+        const double b1 = y[1] - y[2], b2 = y[2] - y[0], b3 = y[0] - y[1];
+        const double c1 = x[2] - x[1], c2 = x[0] - x[2], c3 = x[1] - x[0];
+        const double aire = fabs(c3 * b2 - c2 * b3) / 2;
+        sx += (b1 * a.h[nd[0]] + b2 * a.h[nd[1]] + b3 * a.h[nd[2]]) / (2 * aire);
+        sz += (c1 * a.h[nd[0]] + c2 * a.h[nd[1]] + c3 * a.h[nd[2]]) / (2 * aire);
+    }
+    cnt = (double)(q1 - q0);
+    gx = sx / cnt;
+    gz = sz / cnt;
+    ksn = ks / cnt;
+}
+
+// Per node: carry g_f through D(q, theta) (dual numbers over (qx, qz, theta)
+// on the forward's expressions), then q = -K grad h, K = Kr(h) Ks_node and
+// theta(h) (soil_dual over (h, alpha, n)).  Writes the local part of dL/dh,
+// the alpha / n terms, and for the neighbour pass w = (G_gx / cnt, G_gz / cnt,
+// G_K Kr / cnt).
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_chain(TrAdjArgs a, const double* gf, double* w, double* dldh,
+                                                        double* gn) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const size_t n = (size_t)a.n;
+    double gx, gz, ksn, cnt;
+    tr_node_grad(a, i, gx, gz, ksn, cnt);
+    const D3 qx{a.qx[i], {1.0, 0.0, 0.0}}, qz{a.qz[i], {0.0, 1.0, 0.0}}, th{a.th[i], {0.0, 0.0, 1.0}};
+    const D3 aq = dsqrt(qx * qx + qz * qz);
+    const double ts = a.sp.p[1];
+    const D3 tau = dpow(th, dconst(7.0 / 3.0)) / dconst(ts * ts);
+    const D3 DLd = dconst(a.DL), DTd = dconst(a.DT), Dmd = dconst(a.Dm);
+    const D3 dxx = (DLd * qx * qx + DTd * qz * qz) / aq + th * Dmd * tau;
+    const D3 dzz = (DLd * qz * qz + DTd * qx * qx) / aq + th * Dmd * tau;
+    const D3 dxz = dconst(a.DL - a.DT) * qx * qz / aq;
+    const double gth = gf[i], gqx = gf[n + i], gqz = gf[2 * n + i];
+    const double gxx = gf[3 * n + i], gxz = gf[4 * n + i], gzz = gf[5 * n + i];
+    const double Gqx = gqx + gxx * dxx.d[0] + gxz * dxz.d[0] + gzz * dzz.d[0];
+    const double Gqz = gqz + gxx * dxx.d[1] + gxz * dxz.d[1] + gzz * dzz.d[1];
+    const double Gth = gth + gxx * dxx.d[2] + gxz * dxz.d[2] + gzz * dzz.d[2];
+    D3 sth, skr;
+    soil_dual(a.sp, a.h[i], sth, skr);
+    double C, Kr, thv;
+    soil_eval(a.sp, a.h[i], C, Kr, thv);
+    const double K = Kr * ksn;
+    const double GK = -Gqx * gx - Gqz * (gz + 1.0);
+    dldh[i] = Gth * sth.d[0] + GK * ksn * skr.d[0];
+    gn[i] = Gth * sth.d[1] + GK * ksn * skr.d[1];
+    gn[n + i] = Gth * sth.d[2] + GK * ksn * skr.d[2];
+    w[i] = -K * Gqx / cnt;
+    w[n + i] = -K * Gqz / cnt;
+    w[2 * n + i] = GK * Kr / cnt;
+}
+
+// dldh_j += sum_{(e,b) at j} sum_{i in e} (w_gx,i b_b + w_gz,i c_b) / (2 |area_e|)
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_dh(TrAdjArgs a, const double* w, double* dldh) {
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= a.n) return;
+    const size_t n = (size_t)a.n;
+    double s = 0.0;
+    for (int q = a.inc_ptr[j]; q < a.inc_ptr[j + 1]; ++q) {
+        const int2 ic = a.inc[q];
+        const int lb = ic.x & 3;
+        int nd[3];
+        double x[3], y[3];
+        load_element(a, ic.x >> 2, nd, x, y);
+        const double b[3] = {y[1] - y[2], y[2] - y[0], y[0] - y[1]};
+        const double c[3] = {x[2] - x[1], x[0] - x[2], x[1] - x[0]};
+        const double aire = fabs(c[2] * b[1] - c[1] * b[2]) / 2;
+        double wx = 0.0, wz = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            wx += w[nd[k]];
+            wz += w[n + nd[k]];
+        }
+        s += (wx * b[lb] + wz * c[lb]) / (2 * aire);
+    }
+    dldh[j] += s;
+}
+
+// grad_logks[e] += Ks_e sum_{i in e} G_K,i Kr_i / cnt_i
+__global__ __launch_bounds__(BLOCK) void k_tr_adj_ks(TrAdjArgs a, const double* w, double* grad_logks) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= a.E) return;
+    const int4 cn = a.conn4[e];
+    const double* wk = w + 2 * (size_t)a.n;
+    grad_logks[e] += a.ks_e[e] * (wk[cn.x] + wk[cn.y] + wk[cn.z]);
+}
+
 static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
 hipError_t launch_adj_nodal(const AdjArgs& a, hipStream_t s) {
@@ -311,6 +545,45 @@ hipError_t launch_adj_params(const AdjArgs& a, const double* lam, double* grad_l
 hipError_t launch_adj_obs(const AdjArgs& a, int n_obs, const int32_t* obs, const double* dldy, double* dldh,
                           double* gan, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_obs, dim3(1), dim3(64), 0, s, a, n_obs, obs, dldy, dldh, gan);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_values(const double* J, const int32_t* tpos, double* JT, int64_t nnz, hipStream_t s) {
+    const unsigned g = (unsigned)std::min<int64_t>((nnz + BLOCK - 1) / BLOCK, 8192);
+    hipLaunchKernelGGL(k_adj_transpose, dim3(g), dim3(BLOCK), 0, s, J, tpos, JT, nnz);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_adj_mass(const TrAdjArgs& a, const double* th_next, double* mth, double* mT, hipStream_t s) {
+    hipLaunchKernelGGL(k_tr_adj_mass, grid_of(a.n), dim3(BLOCK), 0, s, a, th_next, mth, mT);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_adj_rhs(const TrAdjArgs& a, int species, const double* mth, const double* mT,
+                             const double* lam_next, const double* lam_up, double* rhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_tr_adj_rhs, grid_of(a.n), dim3(BLOCK), 0, s, a, species, mth, mT, lam_next, lam_up, rhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_adj_obs(int n_obs, const int32_t* obs, const double* dldy, double* rhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_tr_adj_obs, dim3(1), dim3(64), 0, s, n_obs, obs, dldy, rhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_adj_fields(const TrAdjArgs& a, double* gf, hipStream_t s) {
+    hipLaunchKernelGGL(k_tr_adj_fields, grid_of(a.n), dim3(BLOCK), 0, s, a, gf);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_adj_chain(const TrAdjArgs& a, const double* gf, double* w, double* dldh, double* gn,
+                               double* grad_logks, hipStream_t s) {
+    hipLaunchKernelGGL(k_tr_adj_chain, grid_of(a.n), dim3(BLOCK), 0, s, a, gf, w, dldh, gn);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tr_adj_dh, grid_of(a.n), dim3(BLOCK), 0, s, a, w, dldh);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tr_adj_ks, grid_of(a.E), dim3(BLOCK), 0, s, a, w, grad_logks);
     return hipGetLastError();
 }
 

@@ -465,6 +465,11 @@ struct wspm_ctx {
                *dldh = nullptr, *ge = nullptr, *gn = nullptr;
         int32_t* tpos = nullptr;
     } adj;
+    // species adjoint workspace (allocated on first use)
+    struct {
+        double *f = nullptr, *fnext = nullptr, *mth = nullptr, *mT = nullptr, *A = nullptr, *AT = nullptr,
+               *dinv = nullptr, *b = nullptr, *rhs = nullptr, *gf = nullptr, *w = nullptr, *gn = nullptr;
+    } adjtr;
     double* diag_out = nullptr;     // device scalars of the diagnostics reductions
     // general preconditioners (allocated on first use)
     struct {
@@ -1570,9 +1575,9 @@ int32_t wspm_richards_edges(wspm_ctx* ctx, int32_t inlet_neumann, double neumann
     return WSPM_OK;
 }
 
-int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, double DL, double DT, double Dm,
-                              double* qx, double* qz, double* absq, double* dxx, double* dxz, double* dzz,
-                              double* theta) {
+int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h, const double* Ks_elem, double DL,
+                              double DT, double Dm, double* qx, double* qz, double* absq, double* dxx, double* dxz,
+                              double* dzz, double* theta) {
     if (!ctx) return WSPM_EINVAL;
     if (!phi || !h || !qx || !qz || !absq || !dxx || !dxz || !dzz || !theta)
         return fail(ctx, WSPM_EINVAL, "null argument");
@@ -1585,6 +1590,7 @@ int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h,
     a.conn4 = ctx->conn4;
     a.nrec = ctx->nrec;
     a.h = h;
+    a.ks_e = Ks_elem;
     a.sp = make_soil(WSPM_SOIL_VAN_GENUCHTEN, phi);
     a.DL = DL; a.DT = DT; a.Dm = Dm;
     a.qx = qx; a.qz = qz; a.aq = absq; a.dxx = dxx; a.dxz = dxz; a.dzz = dzz; a.th = theta;
@@ -1632,6 +1638,21 @@ int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp, 
 // ---------------------------------------------------------------------------
 // Discrete adjoint of the implicit step
 // ---------------------------------------------------------------------------
+// Position of the transposed entry of every CSR slot (the pattern is
+// structurally symmetric), built once per context.
+static void ensure_tpos(wspm_ctx* ctx) {
+    if (ctx->adj.tpos) return;
+    const HostPattern& P = ctx->hp;
+    std::vector<int32_t> tp(P.cols.size());
+    for (int i = 0; i < ctx->n; ++i)
+        for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) {
+            const int j = P.cols[k];
+            const auto b = P.cols.begin() + P.row_ptr[j], e = P.cols.begin() + P.row_ptr[j + 1];
+            tp[k] = (int32_t)(std::lower_bound(b, e, i) - P.cols.begin());
+        }
+    ctx->adj.tpos = dupload(ctx, tp.data(), tp.size());
+}
+
 static void adj_reduce(wspm_ctx* ctx, const double* a, int len, double* out) {
     const AdjSum op{a, len, out};
     const int g = std::max(1, std::min((len + BLOCK - 1) / BLOCK, 1024));
@@ -1645,7 +1666,8 @@ static void adj_reduce(wspm_ctx* ctx, const double* a, int len, double* out) {
 
 int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* Ks_elem, const double* h,
                       const double* h_prev, const int32_t* obs, const double* dldy, const double* lam_next,
-                      double* lam, double* grad_logks, double* grad_an, wspm_krylov_stats* stats) {
+                      const double* dldh_in, double* lam, double* grad_logks, double* grad_an,
+                      wspm_krylov_stats* stats) {
     if (!ctx) return WSPM_EINVAL;
     if (!prm || !Ks_elem || !h || !h_prev || !lam || !grad_logks || !grad_an)
         return fail(ctx, WSPM_EINVAL, "null argument");
@@ -1668,17 +1690,8 @@ int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* K
         W.dldh = dalloc<double>(ctx, n);
         W.ge = dalloc<double>(ctx, 2 * (size_t)E);
         W.gn = dalloc<double>(ctx, 2 * (size_t)n);
-        // position of the transposed entry of every CSR slot (symmetric pattern)
-        const HostPattern& P = ctx->hp;
-        std::vector<int32_t> tp(P.cols.size());
-        for (int i = 0; i < n; ++i)
-            for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) {
-                const int j = P.cols[k];
-                const auto b = P.cols.begin() + P.row_ptr[j], e = P.cols.begin() + P.row_ptr[j + 1];
-                tp[k] = (int32_t)(std::lower_bound(b, e, i) - P.cols.begin());
-            }
-        W.tpos = dupload(ctx, tp.data(), tp.size());
     }
+    ensure_tpos(ctx);
     AdjArgs a{};
     a.n = n;
     a.E = E;
@@ -1706,7 +1719,8 @@ int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* K
     HIPCHK(launch_adj_mass(a, W.ml, ctx->stream));
     HIPCHK(launch_adj_nodal(a, ctx->stream));
     HIPCHK(launch_adj_jacobian(a, ctx->max_row, W.J, W.JT, W.dinv, W.tpos, ctx->nnz, ctx->stream));
-    HIPCHK(hipMemsetAsync(W.dldh, 0, n * sizeof(double), ctx->stream));
+    if (dldh_in) HIPCHK(hipMemcpyAsync(W.dldh, dldh_in, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    else HIPCHK(hipMemsetAsync(W.dldh, 0, n * sizeof(double), ctx->stream));
     if (prm->n_obs > 0) HIPCHK(launch_adj_obs(a, prm->n_obs, obs, dldy, W.dldh, grad_an, ctx->stream));
     HIPCHK(launch_adj_rhs(a, W.dldh, lam_next, W.rhs, ctx->stream));
     if (lam_next) HIPCHK(hipMemcpyAsync(lam, lam_next, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1718,6 +1732,123 @@ int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* K
     adj_reduce(ctx, W.gn, n, grad_an);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (stats) *stats = st;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, const double* Ks_elem,
+                                const double* h, const double* h_next, const double* c, const double* c_prev,
+                                const int32_t* obs, const double* dldy, const double* lam_next, double* lam,
+                                double* dldh, double* grad_logks, double* grad_an, wspm_krylov_stats* stats) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!prm || !Ks_elem || !h || !c || !c_prev || !lam || !dldh || !grad_logks || !grad_an)
+        return fail(ctx, WSPM_EINVAL, "null argument");
+    if ((h_next == nullptr) != (lam_next == nullptr))
+        return fail(ctx, WSPM_EINVAL, "h_next and lam_next go together (both NULL at the last step)");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    if (ctx->dim != 2) return fail(ctx, WSPM_EINVAL, "the adjoint is 2D");
+    if (ctx->max_row > 16) return fail(ctx, WSPM_EINVAL, "rows longer than 16 entries");
+    if (!(prm->dt > 0.0)) return fail(ctx, WSPM_EINVAL, "dt must be positive");
+    if (prm->n_obs < 0 || (prm->n_obs > 0 && (!obs || !dldy))) return fail(ctx, WSPM_EINVAL, "bad observations");
+    if (prm->maxiter < 1) return fail(ctx, WSPM_EINVAL, "maxiter must be >= 1");
+    if (prm->cauchy_flux != 0.0 && ctx->edges[WSPM_EDGE_INLET].cnt < 2)
+        return fail(ctx, WSPM_EINVAL, "a Cauchy flux needs the inlet edge list (wspm_edge_set)");
+    API_BEGIN
+    const int n = ctx->n, E = ctx->E;
+    const size_t N = (size_t)n;
+    auto& W = ctx->adjtr;
+    if (!W.f) {
+        W.f = dalloc<double>(ctx, 7 * N);
+        W.fnext = dalloc<double>(ctx, 7 * N);
+        W.mth = dalloc<double>(ctx, N);
+        W.mT = dalloc<double>(ctx, N);
+        W.A = dalloc<double>(ctx, ctx->nnz);
+        W.AT = dalloc<double>(ctx, ctx->nnz);
+        W.dinv = dalloc<double>(ctx, N);
+        W.b = dalloc<double>(ctx, N);
+        W.rhs = dalloc<double>(ctx, N);
+        W.gf = dalloc<double>(ctx, 6 * N);
+        W.w = dalloc<double>(ctx, 3 * N);
+        W.gn = dalloc<double>(ctx, 2 * N);
+    }
+    ensure_tpos(ctx);
+    double p[6];
+    for (int k = 0; k < 6; ++k) p[k] = prm->phi[k];
+    p[5] = 1.0;                                   // Ks enters through Ks_elem
+    SoilParams sp = make_soil(WSPM_SOIL_VAN_GENUCHTEN, p);
+    adj_tie_m(sp);
+    // node fields of h_k (and of h_{k+1} for the next step's species mass)
+    auto fields = [&](const double* hh, double* out) {
+        FieldArgs f{};
+        f.n = n;
+        f.inc_ptr = ctx->inc_ptr; f.inc = ctx->inc; f.conn4 = ctx->conn4; f.nrec = ctx->nrec;
+        f.h = hh; f.ks_e = Ks_elem; f.sp = sp;
+        f.DL = prm->DL; f.DT = prm->DT; f.Dm = prm->Dm;
+        f.qx = out; f.qz = out + N; f.aq = out + 2 * N; f.dxx = out + 3 * N; f.dxz = out + 4 * N;
+        f.dzz = out + 5 * N; f.th = out + 6 * N;
+        HIPCHK(launch_transport_fields(f, ctx->stream));
+    };
+    fields(h, W.f);
+    const double* F = W.f;
+    TrAdjArgs a{};
+    a.n = n; a.E = E;
+    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
+    a.tq = ctx->tq;
+    a.ks_e = Ks_elem;
+    a.h = h;
+    a.sp = sp;
+    a.DL = prm->DL; a.DT = prm->DT; a.Dm = prm->Dm;
+    a.inv_dt = 1.0 / prm->dt;
+    for (int k = 0; k < 3; ++k) a.mu[k] = prm->mu[k];
+    a.rho_kd = prm->rho_kd;
+    a.qx = F; a.qz = F + N; a.th = F + 6 * N;
+    a.c = c; a.cp = c_prev; a.lam = lam;
+    if (h_next) {
+        fields(h_next, W.fnext);
+        HIPCHK(launch_tr_adj_mass(a, W.fnext + 6 * N, W.mth, W.mT, ctx->stream));
+    }
+    wspm_krylov_stats tot{};
+    for (int s = 2; s >= 0; --s) {
+        double* ls = lam + s * N;
+        HIPCHK(launch_tr_adj_rhs(a, s, W.mth, W.mT, lam_next ? lam_next + s * N : nullptr,
+                                 s < 2 ? lam + (s + 1) * N : nullptr, W.rhs, ctx->stream));
+        if (s == 2 && prm->n_obs > 0) HIPCHK(launch_tr_adj_obs(prm->n_obs, obs, dldy, W.rhs, ctx->stream));
+        // A_s of the forward step (k_assemble_transport + the Cauchy inlet)
+        TrArgs t{};
+        t.n = n;
+        t.row_ptr = ctx->row_ptr; t.cols = ctx->cols; t.diag_pos = ctx->diag_pos;
+        t.inc_ptr = ctx->inc_ptr; t.inc = ctx->inc; t.conn4 = ctx->conn4; t.nrec = ctx->nrec;
+        t.th = F + 6 * N; t.qx = F; t.qz = F + N; t.dxx = F + 3 * N; t.dxz = F + 4 * N; t.dzz = F + 5 * N;
+        t.c_n = c_prev + s * N;
+        t.c_src = s > 0 ? c + (s - 1) * N : nullptr;
+        t.mu = prm->mu[s];
+        t.rho_kd = s == 0 ? prm->rho_kd : 0.0;
+        t.mu_src = s > 0 ? prm->mu[s - 1] : 0.0;
+        t.inv_dt = a.inv_dt;
+        t.lumped = 1;
+        t.tq = ctx->tq;
+        t.A = W.A; t.b = W.b; t.dinv = W.dinv;
+        HIPCHK(launch_assemble_transport(t, ctx->max_row, ctx->stream));
+        if (prm->cauchy_flux != 0.0)
+            launch_edge_list(ctx, WSPM_EDGE_INLET, EDGE_CAUCHY, prm->cauchy_flux, 0.0, nullptr, 0, nullptr, W.A,
+                             W.b, W.dinv);
+        HIPCHK(launch_transpose_values(W.A, ctx->adj.tpos, W.AT, ctx->nnz, ctx->stream));
+        if (lam_next) HIPCHK(hipMemcpyAsync(ls, lam_next + s * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                                            ctx->stream));
+        else HIPCHK(hipMemsetAsync(ls, 0, N * sizeof(double), ctx->stream));
+        wspm_krylov_stats st{};
+        krylov_solve<double>(ctx, WSPM_BICGSTAB, W.AT, W.dinv, W.rhs, ls, ls, prm->tol, 0.0, prm->maxiter, 30, 0,
+                             &st);
+        tot.iters += st.iters;
+        if (st.code != 0) tot.code = st.code;
+        tot.res_norm = std::max(tot.res_norm, st.res_norm);
+        tot.b_norm = std::max(tot.b_norm, st.b_norm);
+    }
+    HIPCHK(launch_tr_adj_fields(a, W.gf, ctx->stream));
+    HIPCHK(launch_tr_adj_chain(a, W.gf, W.w, dldh, W.gn, grad_logks, ctx->stream));
+    adj_reduce(ctx, W.gn, n, grad_an);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats) *stats = tot;
     return WSPM_OK;
     API_END
 }

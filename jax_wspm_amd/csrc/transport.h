@@ -32,6 +32,7 @@ struct FieldArgs {
     const int4* conn4;
     const double4* nrec;
     const double* h;
+    const double* ks_e;       // nullable: per-element Ks; K = soil K(h) * mean of the incident Ks_e
     SoilParams sp;
     double DL, DT, Dm;
     double *qx, *qz, *aq, *dxx, *dxz, *dzz, *th;

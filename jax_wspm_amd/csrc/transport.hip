@@ -56,15 +56,19 @@ __device__ inline double tri_geometry(const double (&px)[3], const double (&py)[
 // ---------------------------------------------------------------------------
 // Flux / dispersion at the nodes: mean of the incident elements' gradients
 // (b/c form over |area|, transport_utilities.py:150-171), then q = -K grad h
-// (+ gravity in z) and the dispersion tensor (:173-189).
+// (+ gravity in z) and the dispersion tensor (:173-189).  With a per-element
+// Ks field (not in the reference, whose Ks is one number) the node's K is the
+// soil model's K times the mean Ks of its incident elements.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_transport_fields(FieldArgs a) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= a.n) return;
-    double sx = 0.0, sz = 0.0;
+    double sx = 0.0, sz = 0.0, ks = 0.0;
     const int q0 = a.inc_ptr[i], q1 = a.inc_ptr[i + 1];
     for (int q = q0; q < q1; ++q) {
-        const int4 cn = a.conn4[a.inc[q].x >> 2];
+        const int e = a.inc[q].x >> 2;
+        if (a.ks_e) ks += a.ks_e[e];
+        const int4 cn = a.conn4[e];
         const int nd[3] = {cn.x, cn.y, cn.z};
         double x[3], y[3], hv[3];
 #pragma unroll
@@ -84,6 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_transport_fields(FieldArgs a) {
     const double gx = sx / cnt, gz = sz / cnt;
     double C, K, th;
     soil_eval(a.sp, a.h[i], C, K, th);
+    if (a.ks_e) K *= ks / cnt;
     const double qx = -K * gx;
     const double qz = -K * (gz + 1.0);
     const double aq = sqrt(qx * qx + qz * qz);

@@ -199,14 +199,17 @@ class Context:
         self._ck(self.lib.wspm_richards_edges(self.h, 0 if neumann_flux is None else 1,
                                               float(neumann_flux or 0.0), 1 if freedrain else 0))
 
-    def transport_fields(self, phi, h, DL, DT, Dm):
-        """(qx, qz, |q|, Dxx, Dxz, Dzz, theta) at the nodes (device tensors)."""
+    def transport_fields(self, phi, h, DL, DT, Dm, Ks_elem=None):
+        """(qx, qz, |q|, Dxx, Dxz, Dzz, theta) at the nodes (device tensors).
+        Ks_elem: optional per-element Ks; K at a node is then the soil K times
+        the mean Ks of the node's elements."""
         h = _f64(h)
+        ks = _f64(Ks_elem) if Ks_elem is not None else None
         p = np.zeros(6)
         pa = np.asarray(phi, dtype=np.float64)
         p[: pa.size] = pa
         out = [torch.empty_like(h) for _ in range(7)]
-        self._ck(self.lib.wspm_transport_fields(self.h, ptr(p), ptr(h), float(DL), float(DT), float(Dm),
+        self._ck(self.lib.wspm_transport_fields(self.h, ptr(p), ptr(h), ptr(ks), float(DL), float(DT), float(Dm),
                                                 *(ptr(o) for o in out)))
         return tuple(out)
 

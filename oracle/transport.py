@@ -58,21 +58,27 @@ def element_gradients(points, tris, h):
     return dhx, dhz
 
 
-def fluxes_dispersion(points, tris, h, phi, DL, DT, Dm):
-    """(qx, qz, |q|, Dxx, Dxz, Dzz, theta) at every node."""
+def fluxes_dispersion(points, tris, h, phi, DL, DT, Dm, ks_elem=None):
+    """(qx, qz, |q|, Dxx, Dxz, Dzz, theta) at every node.  ks_elem: optional
+    per-element Ks (an extension): K = K(h) * mean Ks of the node's elements."""
     n = points.shape[0]
     dhx, dhz = element_gradients(points, tris, h)
     sx = np.zeros(n)
     sz = np.zeros(n)
+    sk = np.zeros(n)
     cnt = np.zeros(n)
     for e in range(tris.shape[0]):          # element order, as the masked sum
         for a in range(3):
             i = tris[e, a]
             sx[i] += dhx[e]
             sz[i] += dhz[e]
+            if ks_elem is not None:
+                sk[i] += ks_elem[e]
             cnt[i] += 1.0
     gx, gz = sx / cnt, sz / cnt
     _, K, theta = van_genuchten(h, phi)
+    if ks_elem is not None:
+        K = K * (sk / cnt)
     qx = -K * gx
     qz = -K * (gz + 1.0)
     aq = np.sqrt(qx ** 2 + qz ** 2)
