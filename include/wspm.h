@@ -74,6 +74,9 @@ enum {
 
 enum { WSPM_SOIL_GARDNER = 0, WSPM_SOIL_VAN_GENUCHTEN = 1 };
 enum { WSPM_CG = 0, WSPM_BICGSTAB = 1, WSPM_GMRES = 2 };
+/* Picard steps only (wspm_richards_params.method): each linear system solved
+ * by wspm_direct's banded LU instead of a Krylov method. */
+enum { WSPM_DIRECT = 3 };
 /* Jacobi is fused into the Krylov kernels; the others (src/solvers/
  * jax_linear_solver.py:39-252) are applied between them: 2x2 block Jacobi,
  * SSOR (omega = 1) and the reference's "ilu0" (I + L D^-1)(D + U). */
@@ -101,7 +104,7 @@ typedef struct {
 
 typedef struct {
     int32_t soil_model;   /* WSPM_SOIL_*                                      */
-    int32_t method;       /* WSPM_CG / WSPM_BICGSTAB / WSPM_GMRES             */
+    int32_t method;       /* WSPM_CG / WSPM_BICGSTAB / WSPM_GMRES / WSPM_DIRECT */
     int32_t precond;      /* WSPM_PRECOND_* (general ones: FP64, one mesh)    */
     int32_t precision;    /* WSPM_FP64 or WSPM_MIXED                          */
     double phi[6];        /* gardner: alpha0, theta_s, theta_r, Ks
@@ -316,6 +319,21 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm,
  * Picard iterations, summed Krylov iterations, all converged, max error. */
 int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylov_iters,
                             int32_t* converged, double* err);
+
+/* ---- Direct solve -------------------------------------------------------------
+ * x = A^{-1} b for CSR values A on the context's pattern (all dev): the
+ * `direct` solver of src/solvers/solver_types.py:60-63 (jnp.linalg.solve on
+ * the densified matrix, LU with partial pivoting).  Here: banded LU with
+ * partial pivoting in the band of the pattern after a reverse Cuthill-McKee
+ * permutation (kept when it narrows the band), factor and solve on the GPU.
+ * The band (n x (2 kl + ku + 1) doubles, kl/ku the permuted bandwidths) is
+ * allocated on first use; WSPM_ENOMEM beyond 32 GiB.  Returns WSPM_EINVAL
+ * with "singular matrix" when a pivot column is zero (info: the column + 1,
+ * may be NULL). */
+int32_t wspm_direct(wspm_ctx* ctx, const double* A_vals, const double* b, double* x, int32_t* info);
+/* Bandwidths of the direct solver's ordering (host out: kl, ku, 1 if the
+ * RCM permutation is used). */
+int32_t wspm_direct_info(wspm_ctx* ctx, int32_t* kl, int32_t* ku, int32_t* reordered);
 
 /* ---- Diagnostics (2D, single-mesh contexts) ---- */
 

@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwspm.so"
 WSPM_OK, WSPM_EINVAL, WSPM_EHIP, WSPM_ENOMEM, WSPM_ESTATE = 0, -1, -2, -3, -4
 SOIL = {"gardner": 0, "exponential": 0, "vg": 1, "van_genuchten": 1}
 METHOD = {"cg": 0, "bicgstab": 1, "gmres": 2}
+STEP_METHOD = dict(METHOD, direct=3)       # Picard steps also take the banded-LU direct solve
 PRECOND = {"none": 0, None: 0, "jacobi": 1, "block_jacobi": 2, "ssor": 3, "ilu": 4}
 PRECISION = {"fp64": 0, "mixed": 1}
 PICARD = {"rel": 0, "abs": 1}
@@ -102,6 +103,8 @@ PROTOTYPES = {
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),
     "wspm_domain_integral": (_I, [_P, _P, C.POINTER(_D)]),
+    "wspm_direct": (_I, [_P, _P, _P, _P, C.POINTER(_I)]),
+    "wspm_direct_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "wspm_element_size": (_I, [_P, _P]),
     "wspm_peclet_courant": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _P, _P, C.POINTER(_D)]),
 }

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_lib" / "libwspm.so"
-SOURCES = ["api.hip", "assemble.hip", "transport.hip", "adjoint.hip", "precond.hip", "diagnostics.hip", "pattern.cpp"]
+SOURCES = ["api.hip", "assemble.hip", "transport.hip", "adjoint.hip", "precond.hip", "diagnostics.hip", "direct.hip", "pattern.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function"]

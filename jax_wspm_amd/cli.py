@@ -98,8 +98,8 @@ def _history(v):
 
 def run(args):
     cfg = build_config(args)
-    if args.solver == "direct":
-        raise NotImplementedError("solver 'direct' (dense LU) is not on the MI355X path")
+    if args.precision == "mixed" and args.solver == "direct":
+        raise ValueError("the direct solver is FP64 (--precision fp64)")
     if args.precision == "mixed" and args.preconditioner not in ("none", "jacobi"):
         raise NotImplementedError("mixed precision takes --preconditioner none or jacobi")
     from .richards import RichardsSolver, RichardsSolver3D

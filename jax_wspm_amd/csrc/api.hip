@@ -19,6 +19,7 @@
 #include "assemble.h"
 #include "adjoint.h"
 #include "diagnostics.h"
+#include "direct.h"
 #include "krylov_kernels.h"
 #include "precond.h"
 #include "transport.h"
@@ -32,6 +33,11 @@ namespace {
 thread_local std::string g_create_error;
 
 struct Err {};
+// An error with its own code and message (thrown inside API_BEGIN / API_END).
+struct ApiFail {
+    int code;
+    std::string msg;
+};
 
 #define HIPCHK(expr)                                                                    \
     do {                                                                                \
@@ -471,6 +477,12 @@ struct wspm_ctx {
                *dinv = nullptr, *b = nullptr, *rhs = nullptr, *gf = nullptr, *w = nullptr, *gn = nullptr;
     } adjtr;
     double* diag_out = nullptr;     // device scalars of the diagnostics reductions
+    // direct solver (allocated on first use)
+    struct {
+        Band b{};
+        bool ready = false, reordered = false;
+        std::vector<int32_t> perm;
+    } band;
     // general preconditioners (allocated on first use)
     struct {
         Levels lo, up;
@@ -877,6 +889,10 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
     catch (const Err&) {                            \
         return ctx->err.find("hipMalloc") != std::string::npos ? WSPM_ENOMEM : WSPM_EHIP; \
     }                                               \
+    catch (const ApiFail& f_) {                     \
+        ctx->err = f_.msg;                          \
+        return f_.code;                             \
+    }                                               \
     catch (const std::exception& ex) {              \
         ctx->err = ex.what();                       \
         return WSPM_EHIP;                           \
@@ -1193,6 +1209,81 @@ static int check_krylov_args(wspm_ctx* ctx, int method, int maxiter, int restart
     return WSPM_OK;
 }
 
+// Band workspace of the direct solver: RCM when it narrows the band.
+static void ensure_band(wspm_ctx* ctx) {
+    auto& B = ctx->band;
+    if (B.ready) return;
+    const HostPattern& P = ctx->hp;
+    int kl0, ku0, kl1, ku1;
+    bandwidth(P, {}, kl0, ku0);
+    std::vector<int32_t> rc = rcm_order(P);
+    bandwidth(P, rc, kl1, ku1);
+    B.reordered = (2 * kl1 + ku1) < (2 * kl0 + ku0);
+    if (B.reordered) {
+        B.perm = rc;
+    } else {
+        B.perm.resize(ctx->n);
+        for (int i = 0; i < ctx->n; ++i) B.perm[i] = i;
+    }
+    const int kl = B.reordered ? kl1 : kl0, ku = B.reordered ? ku1 : ku0;
+    const size_t W = 2 * (size_t)kl + ku + 1;
+    const size_t bytes = (size_t)ctx->n * W * sizeof(double);
+    if (bytes > (size_t(32) << 30))
+        throw ApiFail{WSPM_ENOMEM, "direct solve: band of " + std::to_string(bytes >> 20) + " MiB exceeds 32 GiB"};
+    Band& b = B.b;
+    b.n = ctx->n;
+    b.kl = kl;
+    b.ku = ku;
+    b.W = (int)W;
+    b.ab = dalloc<double>(ctx, (size_t)ctx->n * W);
+    b.ipiv = dalloc<int32_t>(ctx, ctx->n);
+    b.perm = dupload(ctx, B.perm.data(), B.perm.size());
+    b.y = dalloc<double>(ctx, ctx->n);
+    b.info = dalloc<int32_t>(ctx, 1);
+    B.ready = true;
+}
+
+// Enqueue fill + factor + solve (no synchronisation).
+static void direct_solve(wspm_ctx* ctx, const double* A, const double* rhs, double* x) {
+    ensure_band(ctx);
+    const Band& b = ctx->band.b;
+    HIPCHK(launch_band_fill(b, ctx->row_ptr, ctx->cols, A, ctx->stream));
+    HIPCHK(launch_band_factor(b, ctx->stream));
+    HIPCHK(launch_band_solve(b, rhs, x, ctx->stream));
+}
+
+static int32_t direct_info(wspm_ctx* ctx) {
+    int32_t info = 0;
+    HIPCHK(hipMemcpyAsync(&info, ctx->band.b.info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return info;
+}
+
+int32_t wspm_direct(wspm_ctx* ctx, const double* A_vals, const double* b, double* x, int32_t* info) {
+    if (!ctx) return WSPM_EINVAL;
+    if (!A_vals || !b || !x) return fail(ctx, WSPM_EINVAL, "null array");
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    API_BEGIN
+    direct_solve(ctx, A_vals, b, x);
+    const int32_t inf = direct_info(ctx);
+    if (info) *info = inf;
+    if (inf) return fail(ctx, WSPM_EINVAL, "singular matrix (zero pivot column " + std::to_string(inf - 1) + ")");
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_direct_info(wspm_ctx* ctx, int32_t* kl, int32_t* ku, int32_t* reordered) {
+    if (!ctx) return WSPM_EINVAL;
+    if (ctx->tab) return fail(ctx, WSPM_EINVAL, "not available on an ensemble context");
+    API_BEGIN
+    ensure_band(ctx);
+    if (kl) *kl = ctx->band.b.kl;
+    if (ku) *ku = ctx->band.b.ku;
+    if (reordered) *reordered = ctx->band.reordered ? 1 : 0;
+    return WSPM_OK;
+    API_END
+}
+
 int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv, const double* b,
                     const double* x0, double* x, double tol, double atol, int32_t maxiter, int32_t restart,
                     wspm_krylov_stats* stats) {
@@ -1322,8 +1413,11 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
 static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm) {
     if (prm->soil_model != WSPM_SOIL_GARDNER && prm->soil_model != WSPM_SOIL_VAN_GENUCHTEN)
         return fail(ctx, WSPM_EINVAL, "unknown soil model");
-    int rc = check_krylov_args(ctx, prm->method, prm->maxiter, prm->restart);
+    const bool direct = prm->method == WSPM_DIRECT;
+    int rc = direct ? WSPM_OK : check_krylov_args(ctx, prm->method, prm->maxiter, prm->restart);
     if (rc) return rc;
+    if (direct && (ctx->tab || prm->precision != WSPM_FP64))
+        return fail(ctx, WSPM_EINVAL, "the direct solver is FP64 on a single-mesh context");
     if (prm->precond < WSPM_PRECOND_NONE || prm->precond > WSPM_PRECOND_ILU)
         return fail(ctx, WSPM_EINVAL, "unknown preconditioner");
     if (prm->precision != WSPM_FP64 && prm->precision != WSPM_MIXED)
@@ -1405,8 +1499,10 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     double err = INFINITY;
     // edge terms change b after the assembly, and a general preconditioner
     // needs M r0: in both cases the solver starts on its own
-    const bool gpc = P.precond > WSPM_PRECOND_JACOBI;
-    const bool own_start = ctx->rich_neumann || ctx->rich_freedrain || gpc;
+    // the direct solver ignores the preconditioner, as the reference's does
+    const bool direct = P.method == WSPM_DIRECT;
+    const bool gpc = !direct && P.precond > WSPM_PRECOND_JACOBI;
+    const bool own_start = ctx->rich_neumann || ctx->rich_freedrain || gpc || direct;
     while (err >= P.picard_tol && it < P.picard_maxit) {
         const int mode = own_start ? ASM_INIT_NONE
                          : mixed ? ASM_INIT_RESID
@@ -1417,7 +1513,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         }
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
-                    jac ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
+                    (jac && !direct) ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
                     mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
         if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
             launch_edge_list(ctx, 0, EDGE_NEUMANN, ctx->rich_flux, 0.0, nullptr, 0, nullptr, nullptr, ctx->bv,
@@ -1428,7 +1524,12 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         wspm_krylov_stats st{};
         const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                            P.picard_mode == WSPM_PICARD_REL};
-        if (mixed) {
+        if (direct) {
+            direct_solve(ctx, ctx->Av, ctx->bv, ctx->hB);
+            launch_vec(ctx, pop, 0);
+            poll(ctx, 0);
+            if (direct_info(ctx)) throw ApiFail{WSPM_EINVAL, "direct solve: singular matrix"};
+        } else if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
                         ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !own_start);
             launch_vec(ctx, pop, 0);

@@ -110,4 +110,94 @@ void block_maxima(const HostPattern& P, int R, int& max_nnz, int& max_inc) {
     }
 }
 
+namespace {
+
+// Breadth-first order from s over nodes not yet done (neighbours by
+// increasing degree); levels = number of levels, last = start of the last.
+std::vector<int32_t> bfs(const HostPattern& P, const std::vector<int32_t>& deg, int s,
+                         const std::vector<uint8_t>& done, int& levels, size_t& last) {
+    std::vector<int32_t> order{s};
+    std::vector<uint8_t> seen(done);
+    seen[s] = 1;
+    size_t lo = 0;
+    levels = 0;
+    last = 0;
+    std::vector<int32_t> nb;
+    while (lo < order.size()) {
+        const size_t hi = order.size();
+        last = lo;
+        ++levels;
+        for (size_t q = lo; q < hi; ++q) {
+            const int i = order[q];
+            nb.clear();
+            for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) {
+                const int j = P.cols[k];
+                if (!seen[j]) {
+                    seen[j] = 1;
+                    nb.push_back(j);
+                }
+            }
+            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+        lo = hi;
+    }
+    return order;
+}
+
+}  // namespace
+
+std::vector<int32_t> rcm_order(const HostPattern& P) {
+    const int n = (int)P.row_ptr.size() - 1;
+    std::vector<int32_t> deg(n);
+    for (int i = 0; i < n; ++i) deg[i] = P.row_ptr[i + 1] - P.row_ptr[i];
+    std::vector<uint8_t> done(n, 0);
+    std::vector<int32_t> cm;
+    cm.reserve(n);
+    for (int s0 = 0; s0 < n; ++s0) {
+        if (done[s0]) continue;
+        int levels = 0;
+        size_t last = 0;
+        std::vector<int32_t> order = bfs(P, deg, s0, done, levels, last);
+        int s = s0;                               // least degree in the component
+        for (int v : order)
+            if (deg[v] < deg[s]) s = v;
+        order = bfs(P, deg, s, done, levels, last);
+        // George-Liu: restart from a least-degree node of the last level while
+        // that deepens the level structure
+        for (int rep = 0; rep < 8; ++rep) {
+            int c = order[last];
+            for (size_t q = last; q < order.size(); ++q)
+                if (deg[order[q]] < deg[c]) c = order[q];
+            int lv2 = 0;
+            size_t last2 = 0;
+            std::vector<int32_t> o2 = bfs(P, deg, c, done, lv2, last2);
+            if (lv2 <= levels) break;
+            order.swap(o2);
+            levels = lv2;
+            last = last2;
+        }
+        for (int v : order) {
+            done[v] = 1;
+            cm.push_back(v);
+        }
+    }
+    std::vector<int32_t> perm(n);
+    for (int q = 0; q < n; ++q) perm[cm[n - 1 - q]] = q;   // reversed
+    return perm;
+}
+
+void bandwidth(const HostPattern& P, const std::vector<int32_t>& perm, int& kl, int& ku) {
+    kl = ku = 0;
+    const int n = (int)P.row_ptr.size() - 1;
+    for (int i = 0; i < n; ++i) {
+        const int r = perm.empty() ? i : perm[i];
+        for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) {
+            const int c = perm.empty() ? P.cols[k] : perm[P.cols[k]];
+            kl = std::max(kl, r - c);
+            ku = std::max(ku, c - r);
+        }
+    }
+}
+
 }  // namespace wspm

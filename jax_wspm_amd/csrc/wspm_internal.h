@@ -22,5 +22,11 @@ struct HostPattern {
 int build_pattern(int dim, int n, int E, const int32_t* conn, HostPattern& P, std::string& err);
 // Largest CSR-value and incidence counts spanned by R consecutive rows.
 void block_maxima(const HostPattern& P, int R, int& max_nnz, int& max_inc);
+// Reverse Cuthill-McKee ordering of the pattern's graph: perm[i] = new index
+// of node i.  Each connected component starts from a pseudo-peripheral node
+// of least degree; neighbours are visited by increasing degree.
+std::vector<int32_t> rcm_order(const HostPattern& P);
+// Lower and upper bandwidth of the pattern under perm (identity when empty).
+void bandwidth(const HostPattern& P, const std::vector<int32_t>& perm, int& kl, int& ku);
 
 }  // namespace wspm

@@ -119,7 +119,7 @@ class Context:
                   restart=30):
         """Krylov solve with any preconditioner of the reference built from A:
         none, jacobi, block_jacobi, ssor, ilu."""
-        if method not in _lib.METHOD:
+        if method not in _lib.STEP_METHOD:
             raise ValueError(f"Unsupported solver type: {method}")
         if precond not in _lib.PRECOND:
             raise ValueError(f"Unsupported preconditioner: {precond}")
@@ -130,6 +130,20 @@ class Context:
                                          ptr(x), float(tol), float(atol), int(maxiter), int(restart), C.byref(st)))
         return x, {"iters": st.iters, "arnoldi": st.arnoldi, "code": st.code, "refinements": st.refinements,
                    "res_norm": st.res_norm, "b_norm": st.b_norm}
+
+    def direct(self, A, b):
+        """x = A^{-1} b by the banded LU with partial pivoting (wspm_direct)."""
+        b = _f64(b)
+        x = torch.empty_like(b)
+        info = C.c_int32()
+        self._ck(self.lib.wspm_direct(self.h, ptr(A), ptr(b), ptr(x), C.byref(info)))
+        return x
+
+    def direct_info(self):
+        """(kl, ku, reordered) of the direct solver's band."""
+        kl, ku, r = C.c_int32(), C.c_int32(), C.c_int32()
+        self._ck(self.lib.wspm_direct_info(self.h, C.byref(kl), C.byref(ku), C.byref(r)))
+        return kl.value, ku.value, bool(r.value)
 
     def krylov(self, method: str, A, b, x0, dinv=None, tol=1e-10, atol=0.0, maxiter=100, restart=30,
                precision="fp64", inner_tol=1e-6, max_refine=50):
@@ -153,14 +167,14 @@ class Context:
     def richards_init(self, soil_model: str, phi, method: str, precond: Optional[str], h0, *,
                       Ks_elem=None, tol=1e-10, atol=0.0, maxiter=100, restart=30, picard_tol=1e-6,
                       picard_maxit=100, picard_mode="rel", precision="fp64", inner_tol=1e-6):
-        if method not in _lib.METHOD:
+        if method not in _lib.STEP_METHOD:
             raise ValueError(f"Unsupported solver type: {method}")
         if precond not in _lib.PRECOND:
             raise NotImplementedError(f"preconditioner {precond!r} is not available on the MI355X path "
                                       "(supported: none, jacobi, block_jacobi, ssor, ilu)")
         prm = _lib.RichardsParams()
         prm.soil_model = _lib.SOIL[soil_model]
-        prm.method = _lib.METHOD[method]
+        prm.method = _lib.STEP_METHOD[method]
         prm.precond = _lib.PRECOND[precond]
         prm.precision = _lib.PRECISION[precision]
         pa = np.asarray(phi, dtype=np.float64)

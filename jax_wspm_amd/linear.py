@@ -47,8 +47,8 @@ def solve_system(matrix: CSRMatrix, rhs, x0, solver_config):
     pt = solver_config.precond_type
     if st not in ("gmres", "bicgstab", "cg", "direct"):
         raise ValueError(f"Unsupported solver type: {st}")
-    if st == "direct":
-        raise NotImplementedError("solver 'direct' (dense LU) is not on the MI355X path")
+    if st == "direct":   # solver_types.py:60-63 (dense LU there, banded LU here); preconditioner ignored
+        return matrix.ctx.direct(matrix.values, rhs), None
     if pt in ("block_jacobi", "ssor", "ilu"):
         x, info = matrix.ctx.krylov_pc(st, pt, matrix.values, rhs, x0, tol=solver_config.tol,
                                        maxiter=solver_config.maxiter, restart=solver_config.restart)

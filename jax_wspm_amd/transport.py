@@ -90,7 +90,15 @@ class _TransportBase:
                                picard_maxit=100, picard_mode="abs")
 
     def _krylov(self, A, b, x0, dinv):
+        """One species / solute system with the configured solver
+        (solver_types.solve_system's dispatch)."""
         s = self.config.solver
+        if s.solver_type == "direct":
+            return self.ctx.direct(A, b)
+        if s.precond_type in ("block_jacobi", "ssor", "ilu"):
+            x, _ = self.ctx.krylov_pc(s.solver_type, s.precond_type, A, b, x0, tol=s.tol, maxiter=s.maxiter,
+                                      restart=s.restart)
+            return x
         x, _ = self.ctx.krylov(s.solver_type, A, b, x0, dinv=dinv if s.precond_type == "jacobi" else None,
                                tol=s.tol, maxiter=s.maxiter, restart=s.restart)
         return x

@@ -14,6 +14,7 @@ import time as _time
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.sparse.linalg as spla
 
 from . import assembly, precond, krylov, soil
 
@@ -76,6 +77,11 @@ class Stepper:
         vals, b = assembly.assemble(self.pattern, pb.points, th, theta_n, h_m, K, C, dt,
                                     pb.bc_lists, pb.Ks_elem)
         A = self.pattern.to_scipy(vals)
+        if pb.solver == "direct":
+            # solver_types.py:60-63: jnp.linalg.solve (LU with partial pivoting) on
+            # the dense matrix; SciPy's sparse LU solves the same system
+            self.krylov_iters.append(0)
+            return spla.spsolve(A.tocsc(), b)
         if pb.precond == "jacobi":
             dinv = assembly.jacobi_inverse(self.pattern, vals)
         elif pb.precond in ("block_jacobi", "ssor", "ilu"):

@@ -59,7 +59,7 @@ def main(ref: Path):
         with open(ref / "resultsRE2D" / f"benchmark_results_cpu_{tc}_WA.csv") as fh:
             for r in csv.DictReader(fh):
                 if r["mesh_size"] in ("25", "50") and r["preconditioner"] in ("none", "jacobi") \
-                        and r["solver"] in ("cg", "bicgstab", "gmres"):
+                        and r["solver"] in ("cg", "bicgstab", "gmres", "direct"):
                     rows.append({k: r[k] for k in ("test_case", "mesh_size", "solver", "preconditioner",
                                                    "iterations", "l2_pressure", "linf_pressure",
                                                    "l2_saturation", "linf_saturation",

@@ -56,8 +56,14 @@ def test_solve_system_mirror_matches_oracle():
         assert (conv == 0) if solver == "gmres" else conv is None
     with pytest.raises(ValueError):
         solve_system(A, b, b, SolverParameters(solver_type="lu"))
-    with pytest.raises(NotImplementedError):
-        solve_system(A, b, b, SolverParameters(solver_type="direct"))
+    # direct: banded LU with partial pivoting against SciPy's sparse LU
+    import scipy.sparse.linalg as spla
+    x = solve_system(A, b, b, SolverParameters(solver_type="direct"))[0].cpu().numpy()
+    pat = oasm.Pattern(el, n)
+    Co, Ko, tho = osoil.van_genuchten(h, vg)
+    vals, bo = oasm.assemble(pat, pts, tho, tho, h, Ko, Co, 1e-3, [(top, np.full(top.size, -75.0))])
+    xo = spla.spsolve(pat.to_scipy(vals).tocsc(), bo)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-12
     # the general preconditioners go through wspm_krylov_pc
     from oracle import precond as opc
     x, _ = solve_system(A, b, torch.as_tensor(h, device="cuda"),

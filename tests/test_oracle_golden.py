@@ -53,7 +53,8 @@ KEYS = ("l2_pressure", "linf_pressure", "l2_saturation", "linf_saturation", "l2_
 
 
 @pytest.mark.parametrize("tc", ["Test1", "Test2", "Test3"])
-@pytest.mark.parametrize("solver,precond", [("cg", "none"), ("bicgstab", "jacobi"), ("gmres", "jacobi")])
+@pytest.mark.parametrize("solver,precond", [("cg", "none"), ("bicgstab", "jacobi"), ("gmres", "jacobi"),
+                                           ("direct", "none")])
 def test_tracy_mesh25_all_metrics(golden, tc, solver, precond):
     rows = [r for r in _rows(golden, {"25"}) if r["test_case"] == tc and r["solver"] == solver
             and r["preconditioner"] == precond]
