@@ -54,13 +54,17 @@ WORKLOADS = {
     "c3": ("batched ensemble: 256 independent 2D van Genuchten instances, side N in {32,48,...,256}, "
            "per-instance alpha, n, Ks, theta_r, theta_s, fixed dt 1e-4, Jacobi-BiCGSTAB, FP64",
            None, 2, "bicgstab", "fp64", None, 1e-4),
+    "c3-mixed": ("batched ensemble: 256 independent 2D van Genuchten instances, side N in {32,48,...,256}, "
+                 "per-instance alpha, n, Ks, theta_r, theta_s, fixed dt 1e-4, Jacobi-BiCGSTAB, FP32 inner / "
+                 "FP64 outer (per-instance iterative refinement to the FP64 tolerance)",
+                 None, 2, "bicgstab", "mixed", None, 1e-4),
     "c4": ("discrete adjoint of 2D Richards flow + NH4 -> NO2 -> NO3 transport: 256x256 triangles "
            "(131,072 elements), log-normal Ks, rates 0.2/0.5/0.05, rho Kd 1.5 x 3.5 (NH4), dispersivities 0.5/0.1, "
            "200 steps dt 1e-3, squared misfit of theta and NO3 at 64 seeded nodes, gradient w.r.t. per-element "
            "log Ks, alpha, n, checkpoint every 20 steps, FP64", 256, 2, "bicgstab", "fp64",
            {"sigma": 0.5, "length": 0.05, "seed": 0}, 1e-3),
 }
-DEFAULT_STEPS = {"c3": 100, "c4": 200}
+DEFAULT_STEPS = {"c3": 100, "c3-mixed": 100, "c4": 200}
 
 # HBM-side bytes per working launch of the dominant SpMV (k_spmv<8,double,BicgQ>),
 # from two rocprofv3 --pmc runs of this script with --calibrate-copy: FETCH_SIZE
@@ -121,19 +125,20 @@ def cpu_baseline(pb, steps):
     return steps / el, el, h
 
 
-def roofline(n, nnz, launches, spmv_ms, rows, traffic=None):
+def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8):
     """Mean HBM rate of the timed SpMV launches: algorithmic bytes of the rows
     each launch worked on (the full operator's bytes per row) over their
-    summed duration."""
+    summed duration.  value_bytes 4: the FP32 SpMVs of a mixed-precision run."""
     if not launches:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None}
-    full = spmv_bytes(n, nnz, 8)
+    full = spmv_bytes(n, nnz, value_bytes)
     moved = full * (rows / n)
     achieved = moved / (spmv_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_spmv (CSR SpMV, fused Krylov epilogues)", "bytes_per_launch": round(moved / launches),
+            "kernel": f"k_spmv (CSR SpMV, {'FP32' if value_bytes == 4 else 'FP64'}, fused Krylov epilogues)",
+            "bytes_per_launch": round(moved / launches),
             "full_operator_bytes": full, "launches": launches,
             "avg_launch_us": round(spmv_ms / launches * 1e3, 3)}
 
@@ -149,14 +154,15 @@ def run_ensemble(args):
 
     desc, _, _, solver, precision, _, dt = WORKLOADS[args.workload]
     insts = vg_ensemble(256, seed=0)
+    mixed = precision == "mixed"
     ens = Ensemble(insts)
-    ens.richards_init("vg", solver, "jacobi")
-    print(f"[bench] c3: instances={ens.n_inst} n={ens.n} E={ens.n_elems} nnz={ens.nnz} lanes={ens.lanes}",
+    ens.richards_init("vg", solver, "jacobi", precision=precision)
+    print(f"[bench] {args.workload}: instances={ens.n_inst} n={ens.n} E={ens.n_elems} nnz={ens.nnz} lanes={ens.lanes}",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ens.step(dt)
     torch.cuda.synchronize()
-    ens.timing(not args.no_timing)
+    ens.timing(not args.no_timing, fp32=mixed)
     picard = kry = 0
     conv = True
     t0 = time.perf_counter()
@@ -177,7 +183,7 @@ def run_ensemble(args):
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "time steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (structured meshes, seeded per-instance soil parameters generated in-process)",
         "config": {"workload": desc, "instances": ens.n_inst, "nodes": ens.n, "elements": ens.n_elems,
                    "nnz": ens.nnz, "dt": dt, "solver": f"{solver}+jacobi", "precision": precision,
@@ -186,9 +192,45 @@ def run_ensemble(args):
                    "max_picard_iters_per_step": round(picard / args.steps, 3),
                    "krylov_iters_per_step_summed": round(kry / args.steps, 1),
                    "all_converged": bool(conv), "spmv_lanes": ens.lanes},
-        "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows),
+        "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8),
     }
-    ens.close()
+    if mixed:
+        # error against the FP64 ensemble: both forms rerun untimed for the
+        # same number of steps with per-instance Picard counts recorded, plus an
+        # FP64 run with the Krylov tolerance at 1e-13 ("tight").  Both forms
+        # stop their solves at a 1e-10 relative residual along different
+        # iterates, so on an ill-conditioned instance (dry soil: C ~ 0) their
+        # solutions differ by up to cond x 1e-10; the tight run shows how far
+        # each form is from the converged solution.
+        ens.close()
+        res = {}
+        for prec, ktol in (("mixed", 1e-10), ("fp64", 1e-10), ("fp64", 1e-13)):
+            e2 = Ensemble(insts)
+            e2.richards_init("vg", solver, "jacobi", precision=prec, tol=ktol)
+            its = [e2.step(dt, per_instance=True)["picard_iters_i"].copy() for _ in range(args.warmup + args.steps)]
+            h2, th2, _ = e2.state()
+            res[(prec, ktol)] = (np.array(its), e2.split(h2), e2.split(th2))
+            e2.close()
+            del e2
+        (it32, h32, th32), (it64, h64, th64) = res[("mixed", 1e-10)], res[("fp64", 1e-10)]
+        _, hT, _ = res[("fp64", 1e-13)]
+
+        def rel(a, b):
+            return np.array([np.linalg.norm(x - y) / np.linalg.norm(y) for x, y in zip(a, b)])
+        same = np.all(it32 == it64, axis=0)
+        eh, et = rel(h32, h64), rel(th32, th64)
+        e32T, e64T = rel(h32, hT), rel(h64, hT)
+        cfg = out["config"]
+        cfg["instances_with_identical_picard_histories"] = int(same.sum())
+        cfg["max_rel_l2_vs_fp64_h"] = float(eh.max())
+        cfg["max_rel_l2_vs_fp64_theta"] = float(et.max())
+        cfg["median_rel_l2_vs_fp64_h"] = float(np.median(eh))
+        cfg["instances_within_1e-8_of_fp64_h"] = int((eh <= 1e-8).sum())
+        cfg["max_rel_l2_h_vs_tight_fp64"] = {"mixed": float(e32T.max()), "fp64": float(e64T.max())}
+        cfg["median_rel_l2_h_vs_tight_fp64"] = {"mixed": float(np.median(e32T)), "fp64": float(np.median(e64T))}
+        cfg["error_sample"] = f"every instance after {args.warmup + args.steps} steps; tight = FP64, Krylov tol 1e-13"
+    else:
+        ens.close()
     del ens
     if args.plain_steps > 0:
         # the plain FP64 single-instance path on every instance (one context each)
@@ -353,7 +395,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.workload == "c3":
+    if args.workload in ("c3", "c3-mixed"):
         run_ensemble(args)
         return
     if args.workload == "c4":
@@ -377,7 +419,7 @@ def main():
     for _ in range(args.warmup):
         ctx.richards_step(dt)
     torch.cuda.synchronize()
-    ctx.timing(not args.no_timing)
+    ctx.timing(not args.no_timing, fp32=precision == "mixed")
     picard = kry = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -409,9 +451,9 @@ def main():
                    "solver": f"{solver}+jacobi", "precision": precision,
                    "picard_iters_per_step": round(picard / args.steps, 3),
                    "krylov_iters_per_step": round(kry / args.steps, 3)},
-        # in mixed mode most SpMVs are FP32 (inner): the FP64 byte count used
-        # here overstates their traffic, so the line is an upper bound there
-        "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows, MEASURED_TRAFFIC.get(args.workload)),
+        # mixed precision: the FP32 SpMVs of the inner solves, at FP32 bytes
+        "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows, MEASURED_TRAFFIC.get(args.workload),
+                             value_bytes=4 if precision == "mixed" else 8),
     }
     if args.cpu_steps > 0:
         pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,

@@ -189,7 +189,8 @@ int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* the
 /* Timing hooks for the benchmark (no counterpart in the reference): when
  * enabled, the SpMV launches of the Krylov solvers are bracketed by HIP events
  * on the context's stream, and each launch records on the device whether it
- * did its work or returned early (solve already converged). */
+ * did its work or returned early (solve already converged).  on: 0 off,
+ * 1 the FP64 SpMV launches, 2 the FP32 ones (mixed precision inner solves). */
 int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
 /* Number of SpMV launches that did their work since enable/last read, their
  * summed event-to-event duration (ms) and the rows they worked on (rows may

@@ -164,7 +164,11 @@ __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__
 // Workgroup i places instance i: its offset is the entry count of the active
 // instances before it (one pass over their flags), its entries are written
 // in parallel.
-__device__ inline bool ens_on(const KState& s, int mode) { return !s.pdone && (mode == 1 || !s.done); }
+// mode 0: Krylov solve running; 1: Picard iterating; 2: mixed-precision
+// refinement still running (rdone clear).
+__device__ inline bool ens_on(const KState& s, int mode) {
+    return !s.pdone && (mode == 1 || (mode == 2 ? !s.rdone : !s.done));
+}
 
 __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict__ st, int n_inst,
                                                        const int* __restrict__ iblk, int* act, int mode) {
@@ -197,6 +201,73 @@ __global__ void k_ens_collect(const KState* __restrict__ st, int n_inst, int* ip
         er[i] = st[i].picard_err;
     }
 }
+
+// Iterative-refinement sweeps per solve (single system and ensembles).
+constexpr int MAX_REFINE = 50;
+
+// Ensemble mixed precision, per instance: FP64 residual r = b - A x of the
+// instances still refining; the finish records |r| and |b| and marks the
+// refinement converged (rdone; done too, which ends the instance's solve for
+// the loops that test done) once |r| <= max(tol |b|, atol) or after
+// MAX_REFINE sweeps.  first: the residual of the Picard iteration's start.
+struct RefResidE {
+    static constexpr int ND = 2;
+    const double* b;
+    const double* x;
+    double* r;
+    double tol, atol;
+    int first;
+    __device__ bool skip(const KState* s) const { return !first && s->rdone; }
+    __device__ void load(const KState*) {}
+    __device__ double xv(int j) const { return x[j]; }
+    __device__ void row(int i, double ax, double* acc) const {
+        const double bi = b[i];
+        const double ri = bi - ax;
+        r[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+    __device__ void fin(const double* t, KState* s) const {
+        s->rnorm = sqrt(t[0]);
+        s->bnorm = sqrt(t[1]);
+        s->refine = first ? 0 : s->refine + 1;
+        s->rdone = !(s->rnorm > fmax(tol * s->bnorm, atol)) || s->refine >= MAX_REFINE;
+        s->done = s->rdone;
+    }
+};
+
+struct RefScaleE {  // r32 = r / |r|, d32 = 0 on the instances whose inner solve starts
+    static constexpr int ND = 0;
+    const double* r;
+    float* r32;
+    float* d32;
+    double inv;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { inv = s->rnorm > 0.0 ? 1.0 / s->rnorm : 0.0; }
+    __device__ void row(int i, double*) const {
+        r32[i] = (float)(r[i] * inv);
+        d32[i] = 0.0f;
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct RefCorrectE {  // x += |r| d32 on the instances refined in this sweep
+    static constexpr int ND = 0;
+    double* x;
+    const float* d32;
+    double scale;
+    __device__ bool skip(const KState* s) const { return s->rdone; }
+    __device__ void load(const KState* s) { scale = s->rnorm; }
+    __device__ void row(int i, double*) const { x[i] += scale * (double)d32[i]; }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// A solver-start functor restricted to the instances whose solve starts
+// (done clear): the others keep their state.
+template <class Op>
+struct ActiveOnly : Op {
+    __device__ bool skip(const KState* s) const { return s->done; }
+};
 
 // Mixed precision: FP64 residual r = b - A x, |r|, |b|
 struct RefResid {
@@ -446,6 +517,7 @@ struct wspm_ctx {
     int last_iters[3] = {1, 1, 1};
     // timing
     bool timing = false;
+    bool timing_fp32 = false;       // time the FP32 SpMV launches instead of the FP64 ones
     int* tflags = nullptr;          // [TIMING_SLOTS] rows each timed launch worked on (0: skipped)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -541,11 +613,12 @@ EnsWalk walk_of(wspm_ctx* ctx, int list) {
     return EnsWalk{ctx->tab, list == LIST_K ? ctx->kact : list == LIST_P ? ctx->pact : nullptr, ctx->n_tab};
 }
 
-// Rebuild an active-entry list (LIST_K or LIST_P) from the instance states.
-void ens_compact(wspm_ctx* ctx, int list) {
+// Rebuild an active-entry list (LIST_K or LIST_P) from the instance states;
+// mode as ens_on (default: 0 for LIST_K, 1 for LIST_P).
+void ens_compact(wspm_ctx* ctx, int list, int mode = -1) {
+    if (mode < 0) mode = list == LIST_K ? 0 : 1;
     hipLaunchKernelGGL(k_ens_compact, dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, (const KState*)ctx->states,
-                       ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact,
-                       list == LIST_K ? 0 : 1);
+                       ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact, mode);
     HIPCHK(hipGetLastError());
 }
 
@@ -586,7 +659,8 @@ constexpr size_t TIMING_SLOTS = (size_t)1 << 18;
 
 template <class T, class Epi>
 void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false, int list = LIST_ALL) {
-    const bool t = timed && ctx->timing && ctx->ev_used / 2 < TIMING_SLOTS;
+    const bool t = timed && ctx->timing && ctx->timing_fp32 == std::is_same<T, float>::value &&
+                   ctx->ev_used / 2 < TIMING_SLOTS;
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
     const Red red = red_of(ctx, which);
@@ -768,6 +842,12 @@ struct KrylovRun {
             KWork<T>& w = ensure_work<T>(ctx, 0);
             v = make_kvec<T>(ctx, w, dinv, b, x0, x, gpc);
             if (started) {
+            } else if (ctx->tab) {
+                // ensemble: start the instances of the active list only
+                if (method == WSPM_BICGSTAB)
+                    launch_spmv<T>(ctx, A, ActiveOnly<BicgInit<T>>{{v, tol, atol, maxiter}}, which, true, LIST_K);
+                else
+                    launch_spmv<T>(ctx, A, ActiveOnly<CgInit<T>>{{v, tol, atol, maxiter}}, which, true, LIST_K);
             } else if (method == WSPM_BICGSTAB) {
                 launch_spmv<T>(ctx, A, BicgInit<T>{v, tol, atol, maxiter}, which, true);
             } else {
@@ -1169,7 +1249,10 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
             break;
         }
         default:
-            launch_finish<RefResid, true>(ctx, RefResid{b, x, ctx->ref_r}, grid, 2);
+            if (ctx->tab)
+                launch_finish<RefResidE, true>(ctx, RefResidE{b, x, ctx->ref_r, tol, atol, 1}, grid, 0);
+            else
+                launch_finish<RefResid, true>(ctx, RefResid{b, x, ctx->ref_r}, grid, 2);
             break;
     }
     if (ctx->tab) ens_compact(ctx, LIST_K);
@@ -1590,6 +1673,7 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on) {
     if (on && !ctx->tflags) ctx->tflags = dalloc<int>(ctx, TIMING_SLOTS);
     if (on) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
     ctx->timing = on != 0;
+    ctx->timing_fp32 = on == 2;
     ctx->ev_used = 0;
     return WSPM_OK;
     API_END
@@ -2125,7 +2209,8 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
     if (rc) return rc;
     if (prm->method == WSPM_GMRES) return fail(ctx, WSPM_EINVAL, "ensembles solve with CG or BiCGSTAB");
     if (prm->precond > WSPM_PRECOND_JACOBI) return fail(ctx, WSPM_EINVAL, "ensembles take no preconditioner or Jacobi");
-    if (prm->precision != WSPM_FP64) return fail(ctx, WSPM_EINVAL, "ensembles are FP64");
+    if (prm->precision == WSPM_MIXED && !(prm->inner_tol > 0.0 && prm->inner_tol < 1.0))
+        return fail(ctx, WSPM_EINVAL, "inner_tol must be in (0, 1)");
     API_BEGIN
     richards_alloc(ctx, prm, Ks_elem, h0);
     const int ni = ctx->n_inst;
@@ -2160,6 +2245,34 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
 // iterating, their Krylov solves (instances drop out as they converge), and
 // the Picard error per instance; the host reads back three counters per
 // iteration chunk.
+// Mixed-precision linear solves of one ensemble Picard iteration: the
+// assembly left r = b - A x (x = h_m) and |r|, |b| per instance.  Sweeps of
+// FP32 Krylov corrections on the instances still refining (their own
+// tolerance test, mode 2 lists), each followed by the FP64 update and
+// residual, until every instance's FP64 residual meets the Krylov tolerance.
+static void ens_mixed_solve(wspm_ctx* ctx, bool jac) {
+    const wspm_richards_params& P = ctx->prm;
+    while (true) {
+        ens_compact(ctx, LIST_K);                 // instances whose refinement goes on (done clear)
+        poll_ens(ctx, 0);
+        if (ctx->hstatus[0] == 0) break;
+        launch_vec(ctx, RefScaleE{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
+        KrylovRun<float> run(ctx, P.method, ctx->A32, jac ? ctx->dinv32 : nullptr, ctx->ref_r32, ctx->ref_d32,
+                             ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, false);
+        ens_compact(ctx, LIST_K);
+        run.enqueue(2);
+        poll_ens(ctx, 0);
+        while (ctx->hstatus[0] > 0 && run.enq < P.maxiter) {
+            run.enqueue(2);
+            poll_ens(ctx, 0);
+        }
+        ens_compact(ctx, LIST_K, 2);              // the instances refined in this sweep
+        launch_vec(ctx, RefCorrectE{ctx->hB, ctx->ref_d32, 0.0}, 0, LIST_K);
+        launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 0}, 0, true,
+                            LIST_K);
+    }
+}
+
 static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const wspm_richards_params& P = ctx->prm;
     const bool jac = P.precond == WSPM_PRECOND_JACOBI;
@@ -2170,9 +2283,26 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     HIPCHK(hipGetLastError());
     ens_compact(ctx, LIST_P);
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    const int mode = P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
+    const bool mixed = P.precision == WSPM_MIXED;
+    const int mode = mixed ? ASM_INIT_RESID : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
     int it = 0;
     while (true) {
+        if (mixed) {
+            ensure_mixed(ctx);
+            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
+                        jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr, mode, ctx->hB, P.tol,
+                        P.atol, P.maxiter, P.restart);
+            ens_mixed_solve(ctx, jac);
+            const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
+                               P.picard_mode == WSPM_PICARD_REL, 1};
+            launch_vec(ctx, pop, 0, LIST_P);
+            poll_ens(ctx, 0);
+            ens_compact(ctx, LIST_P);
+            ++it;
+            if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
+            continue;
+        }
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     jac ? ctx->dinv : nullptr, nullptr, nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter,
                     P.restart);

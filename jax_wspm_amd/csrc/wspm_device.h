@@ -34,7 +34,9 @@ struct KState {
     int32_t maxiter, k, steps, done;
     int32_t exit_early, pass2, active, code;
     int32_t cycles, arnoldi, restart, nan_flag;
-    int32_t jdone, refine, pad0, pad1;
+    int32_t jdone, refine;
+    int32_t rdone;                           // ensemble mixed precision: refinement converged
+    int32_t pad1;
     // Ensemble mode (one KState per instance): Picard control of the instance
     // within the current time step, and its soil parameters.
     int32_t pdone;                           // Picard converged or capped this step

@@ -278,8 +278,10 @@ class Context:
         return A, b, dinv
 
     # -- timing hooks ---------------------------------------------------------------
-    def timing(self, on: bool):
-        self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
+    def timing(self, on: bool, fp32: bool = False):
+        """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
+        mixed-precision inner solves)."""
+        self._ck(self.lib.wspm_timing_enable(self.h, (2 if fp32 else 1) if on else 0))
 
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""

@@ -112,7 +112,10 @@ class Ensemble:
         check(rc, self.h)
 
     def richards_init(self, soil_model: str = "vg", method: str = "bicgstab", precond: Optional[str] = "jacobi", *,
-                      tol=1e-10, atol=0.0, maxiter=100, picard_tol=1e-6, picard_maxit=100, picard_mode="rel"):
+                      tol=1e-10, atol=0.0, maxiter=100, picard_tol=1e-6, picard_maxit=100, picard_mode="rel",
+                      precision: str = "fp64", inner_tol: float = 1e-6):
+        """precision "mixed": FP32 Krylov corrections of the FP64 residual,
+        refined per instance until that residual meets ``tol``."""
         if method not in ("cg", "bicgstab"):
             raise ValueError(f"ensembles solve with cg or bicgstab, not {method!r}")
         if precond not in _lib.PRECOND:
@@ -121,7 +124,8 @@ class Ensemble:
         prm.soil_model = _lib.SOIL[soil_model]
         prm.method = _lib.METHOD[method]
         prm.precond = _lib.PRECOND[precond]
-        prm.precision = _lib.PRECISION["fp64"]
+        prm.precision = _lib.PRECISION[precision]
+        prm.inner_tol = float(inner_tol)
         prm.tol, prm.atol, prm.maxiter, prm.restart = float(tol), float(atol), int(maxiter), 30
         prm.picard_tol, prm.picard_maxit = float(picard_tol), int(picard_maxit)
         prm.picard_mode = _lib.PICARD[picard_mode]
@@ -167,8 +171,10 @@ class Ensemble:
         x = x.cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
         return [x[self.node_off[i]:self.node_off[i + 1]] for i in range(self.n_inst)]
 
-    def timing(self, on: bool):
-        self._ck(self.lib.wspm_timing_enable(self.h, 1 if on else 0))
+    def timing(self, on: bool, fp32: bool = False):
+        """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
+        mixed-precision inner solves)."""
+        self._ck(self.lib.wspm_timing_enable(self.h, (2 if fp32 else 1) if on else 0))
 
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""

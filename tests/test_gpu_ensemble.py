@@ -112,3 +112,30 @@ def test_ensemble_rejects_single_instance_calls():
     with pytest.raises(ValueError):
         ens.richards_init("vg", "gmres", "jacobi")
     ens.close()
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cg"])
+def test_mixed_ensemble_within_1e8_of_fp64(solver):
+    """FP32 inner / FP64 outer ensemble: FP32 Krylov corrections of the FP64
+    residual, refined per instance until that residual meets the FP64
+    tolerance.  Every instance's h and theta stay within 1e-8 relative L2 of
+    the FP64 ensemble (and of the oracle) over the steps, with the same
+    Picard iteration counts."""
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    insts = vg_ensemble(n_inst=6, sides=(8, 12, 16, 20, 24, 32), seed=11)
+    print(f"[case] mixed ensemble 6 {solver}+jacobi", flush=True)
+    steps, dt = 4, 1e-4
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        ens = Ensemble(insts)
+        ens.richards_init("vg", solver, "jacobi", precision=prec)
+        runs[prec] = _run(ens, steps, dt)
+        ens.close()
+    (h64, th64, it64), (h32, th32, it32) = runs["fp64"], runs["mixed"]
+    for i, inst in enumerate(insts):
+        o = _oracle(inst, steps, dt, solver, "jacobi")
+        assert [int(it32[k][i]) for k in range(steps)] == [int(it64[k][i]) for k in range(steps)], i
+        for k in range(steps):
+            assert rel_l2(h32[k][i], h64[k][i]) <= 1e-8, (i, k, rel_l2(h32[k][i], h64[k][i]))
+            assert rel_l2(th32[k][i], th64[k][i]) <= 1e-8, (i, k)
+            assert rel_l2(h32[k][i], o["pressure_head"][k]) <= 1e-8, (i, k)
