@@ -481,6 +481,13 @@ struct wspm_ctx {
     int asm_win = 0, asm_inc = 0, asm_grid = 0;   // assembly LDS window / incidence capacity
     size_t asm_lds = 0;
     bool asm_reg = false;                         // rows fit the register assembly kernel
+    // ELL copy of the operator for the time-step solves (rows of 7 or 8
+    // entries): column table, width, whether this run's steps use it, and
+    // whether the SpMV launches read ELL right now
+    int32_t* ell_col = nullptr;
+    int ell_w = 0;
+    bool ell = false;
+    bool spmv_ell = false;
     int asm_kernel = ASM_K_LDS;                   // ASM_K_*
     int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
@@ -664,9 +671,22 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
     if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
     const Red red = red_of(ctx, which);
+    const EnsWalk tab = walk_of(ctx, list);
+    if (ctx->spmv_ell) {
+        const int eg = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
+        if (ctx->ell_w == 7)
+            hipLaunchKernelGGL((k_spmv_ell<7, T, Epi>), dim3(eg), dim3(BLOCK), 0, ctx->stream, ctx->n, ctx->ell_col,
+                               av, epi, red, tflag, tab);
+        else
+            hipLaunchKernelGGL((k_spmv_ell<8, T, Epi>), dim3(eg), dim3(BLOCK), 0, ctx->stream, ctx->n, ctx->ell_col,
+                               av, epi, red, tflag, tab);
+        HIPCHK(hipGetLastError());
+        if (t) HIPCHK(hipEventRecord(next_event(ctx), ctx->stream));
+        launch_finish(ctx, epi, eg, which);
+        return;
+    }
     const int grid = ctx->tab ? ctx->ens_grid : ctx->spmv_grid;
     const dim3 g(grid), b(BLOCK);
-    const EnsWalk tab = walk_of(ctx, list);
     switch (ctx->lanes) {
         case 2: hipLaunchKernelGGL((k_spmv<2, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
         case 4: hipLaunchKernelGGL((k_spmv<4, T, Epi>), g, b, 0, ctx->stream, ctx->n, ctx->row_ptr, ctx->cols, av, epi, red, tflag, tab); break;
@@ -1200,7 +1220,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
                         const double* K, const double* theta_m, const double* theta_n, double dt,
                         double* A, double* b, double* dinv, float* A32, float* dinv32,
                         int mode = ASM_INIT_NONE, double* x = nullptr, double tol = 0.0, double atol = 0.0,
-                        int maxiter = 0, int restart = 0) {
+                        int maxiter = 0, int restart = 0, bool ell = false) {
     AsmArgs a{};
     a.n = ctx->n;
     a.win_cap = ctx->asm_win;
@@ -1213,6 +1233,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
+    a.ell_n = ell ? ctx->n : 0;    // callers pass ell only with the register kernels
     a.qc = ctx->qc;
     a.tab = ctx->tab;
     a.act = ctx->pact;
@@ -1418,6 +1439,8 @@ int32_t wspm_krylov_pc(wspm_ctx* ctx, int32_t method, int32_t precond, const dou
     API_END
 }
 
+static size_t op_cap(const wspm_ctx* ctx);
+
 static void ensure_mixed(wspm_ctx* ctx) {
     if (!ctx->ref_r) {
         ctx->ref_r = dalloc<double>(ctx, ctx->n);
@@ -1471,7 +1494,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
     if (max_refine < 0) return fail(ctx, WSPM_EINVAL, "max_refine must be >= 0");
     API_BEGIN
     if (!ctx->A32) {
-        ctx->A32 = dalloc<float>(ctx, ctx->nnz);
+        ctx->A32 = dalloc<float>(ctx, op_cap(ctx));
         ctx->dinv32 = dalloc<float>(ctx, ctx->n);
     }
     // FP32 copies of the operator (the Richards driver gets them from assembly)
@@ -1511,6 +1534,47 @@ static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm)
     return WSPM_OK;
 }
 
+// Values held for the time-step operator: CSR, or ELL of width max_row.
+static size_t op_cap(const wspm_ctx* ctx) {
+    const size_t ell = (ctx->dim == 2 && ctx->max_row <= 8) ? (size_t)ctx->max_row * ctx->n : 0;
+    return std::max((size_t)ctx->nnz, ell);
+}
+
+// ELL for the time steps: 2D register assembly, rows of 7 or 8 entries, at
+// most 25 % padding, and a Krylov solve with no preconditioner or Jacobi
+// (the general preconditioners and the direct solver read CSR).
+// WSPM_SPMV=csr keeps CSR.
+static bool ell_ok(const wspm_ctx* ctx, const wspm_richards_params* prm) {
+    const char* env = std::getenv("WSPM_SPMV");
+    if (env && std::string(env) == "csr") return false;
+    if (ctx->dim != 2 || !ctx->asm_reg || ctx->asm_kernel == ASM_K_LDS) return false;
+    if (ctx->max_row != 7 && ctx->max_row != 8) return false;
+    if ((double)ctx->max_row * ctx->n > 1.25 * (double)ctx->nnz) return false;
+    return prm->method != WSPM_DIRECT && prm->precond <= WSPM_PRECOND_JACOBI;
+}
+
+// Column table of the ELL layout: entry k of row i at k * n + i; padding
+// points at the row itself (its value stays 0).
+static void ensure_ell(wspm_ctx* ctx) {
+    if (ctx->ell_col) return;
+    const HostPattern& P = ctx->hp;
+    const int n = ctx->n, W = ctx->max_row;
+    std::vector<int32_t> c((size_t)W * n);
+    for (int i = 0; i < n; ++i) {
+        const int rs = P.row_ptr[i], rl = P.row_ptr[i + 1] - rs;
+        for (int k = 0; k < W; ++k) c[(size_t)k * n + i] = k < rl ? P.cols[rs + k] : i;
+    }
+    ctx->ell_col = dupload(ctx, c.data(), c.size());
+    ctx->ell_w = W;
+}
+
+// The SpMV launches of a time step read the ELL operator while this lives.
+struct EllScope {
+    wspm_ctx* ctx;
+    EllScope(wspm_ctx* c, bool on) : ctx(c) { ctx->spmv_ell = on; }
+    ~EllScope() { ctx->spmv_ell = false; }
+};
+
 // Time-stepping arrays, Ks_e and h0 (both device, copied).
 static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const double* Ks_elem,
                            const double* h0) {
@@ -1519,7 +1583,7 @@ static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const
         ctx->hA = dalloc<double>(ctx, n); ctx->hB = dalloc<double>(ctx, n);
         ctx->Cn = dalloc<double>(ctx, n);
         ctx->thm = dalloc<double>(ctx, n); ctx->thn = dalloc<double>(ctx, n);
-        ctx->Av = dalloc<double>(ctx, ctx->nnz); ctx->bv = dalloc<double>(ctx, n);
+        ctx->Av = dalloc<double>(ctx, op_cap(ctx)); ctx->bv = dalloc<double>(ctx, n);
         ctx->dinv = dalloc<double>(ctx, n);
     }
     if (Ks_elem) {
@@ -1527,10 +1591,18 @@ static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const
         HIPCHK(hipMemcpyAsync(ctx->Ks, Ks_elem, ctx->E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
     if (prm->precision == WSPM_MIXED && !ctx->A32) {
-        ctx->A32 = dalloc<float>(ctx, ctx->nnz);
+        ctx->A32 = dalloc<float>(ctx, op_cap(ctx));
         ctx->dinv32 = dalloc<float>(ctx, n);
     }
     ctx->prm = *prm;
+    // the time-step solves read the operator in ELL order when the rows allow
+    ctx->ell = ell_ok(ctx, prm);
+    if (ctx->ell) {
+        ensure_ell(ctx);
+        // padding slots stay zero: the assembly writes only the real entries
+        HIPCHK(hipMemsetAsync(ctx->Av, 0, op_cap(ctx) * sizeof(double), ctx->stream));
+        if (ctx->A32) HIPCHK(hipMemsetAsync(ctx->A32, 0, op_cap(ctx) * sizeof(float), ctx->stream));
+    }
     HIPCHK(hipMemcpyAsync(ctx->hA, h0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
 }
 
@@ -1586,6 +1658,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const bool direct = P.method == WSPM_DIRECT;
     const bool gpc = !direct && P.precond > WSPM_PRECOND_JACOBI;
     const bool own_start = ctx->rich_neumann || ctx->rich_freedrain || gpc || direct;
+    const EllScope ell_scope(ctx, ctx->ell);
     while (err >= P.picard_tol && it < P.picard_maxit) {
         const int mode = own_start ? ASM_INIT_NONE
                          : mixed ? ASM_INIT_RESID
@@ -1597,7 +1670,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         }
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     (jac && !direct) ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
-                    mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart);
+                    mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, ctx->ell);
         if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
             launch_edge_list(ctx, 0, EDGE_NEUMANN, ctx->rich_flux, 0.0, nullptr, 0, nullptr, nullptr, ctx->bv,
                              nullptr);
@@ -2285,6 +2358,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     const bool mixed = P.precision == WSPM_MIXED;
     const int mode = mixed ? ASM_INIT_RESID : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
+    const EllScope ell_scope(ctx, ctx->ell);
     int it = 0;
     while (true) {
         if (mixed) {
@@ -2292,7 +2366,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
             do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                         jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr, mode, ctx->hB, P.tol,
-                        P.atol, P.maxiter, P.restart);
+                        P.atol, P.maxiter, P.restart, ctx->ell);
             ens_mixed_solve(ctx, jac);
             const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                                P.picard_mode == WSPM_PICARD_REL, 1};
@@ -2305,7 +2379,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         }
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     jac ? ctx->dinv : nullptr, nullptr, nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter,
-                    P.restart);
+                    P.restart, ctx->ell);
         KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
                               P.tol, P.atol, P.maxiter, P.restart, 0, true);
         const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),

@@ -72,6 +72,8 @@ struct AsmArgs {
     const double* bc_val;
     double* A;
     float* A32;              // nullable: also write an FP32 copy of A
+    int ell_n;               // 0: A (and A32) in CSR order; else ELL, entry s of row i at s * ell_n + i
+                             // (register kernels only)
     double* b;
     double* dinv;            // nullable
     float* dinv32;           // nullable

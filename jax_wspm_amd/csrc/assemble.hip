@@ -362,8 +362,9 @@ __device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int 
 #pragma unroll
     for (int s = 0; s < MAXR; ++s) {
         if (s < rl) {
-            a.A[rs + s] = v[s];
-            if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+            const size_t o = a.ell_n ? (size_t)s * a.ell_n + i : (size_t)rs + s;
+            a.A[o] = v[s];
+            if (a.A32 != nullptr) a.A32[o] = (float)v[s];
         }
         if (s == dg) dgv = v[s];
     }
@@ -474,8 +475,9 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
 #pragma unroll
     for (int s = 0; s < MAXR; ++s) {
         if (s < rl) {
-            a.A[rs + s] = v[s];
-            if (a.A32 != nullptr) a.A32[rs + s] = (float)v[s];
+            const size_t o = a.ell_n ? (size_t)s * a.ell_n + i : (size_t)rs + s;
+            a.A[o] = v[s];
+            if (a.A32 != nullptr) a.A32[o] = (float)v[s];
         }
         if (s == dg) dgv = v[s];
     }

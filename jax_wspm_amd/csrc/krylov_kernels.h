@@ -124,6 +124,69 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// ELL SpMV: one lane per row.  Entry k of row i sits at k * n + i
+// (column-major over the rows, width W = the longest row; shorter rows are
+// padded with value 0 and their own column).  A wavefront's loads of entry k
+// cover 64 consecutive rows, so every value and column load is coalesced,
+// and all W of them are issued together before the W gathers of x: two
+// dependent memory round trips per row, no row pointers and no cross-lane
+// reduction.  The row sum runs over k in order.
+// ---------------------------------------------------------------------------
+template <int W, class T, class Epi>
+__device__ inline void ell_rows(int rb, int re, int n, const int32_t* __restrict__ ci, const T* __restrict__ av,
+                                const Epi& epi, double* acc) {
+    for (int i = rb + (int)threadIdx.x; i < re; i += BLOCK) {
+        int c[W];
+        double v[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            c[k] = ci[(size_t)k * n + i];
+            v[k] = (double)av[(size_t)k * n + i];
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) s += v[k] * epi.xv(c[k]);
+        epi.row(i, s, acc);
+    }
+}
+
+template <int W, class T, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __restrict__ ci, const T* __restrict__ av,
+                                                     Epi epi, Red red, int* tflag, EnsWalk ew) {
+    constexpr int ND = Epi::ND;
+    constexpr int NA = ND > 0 ? ND : 1;
+    if (ew.tab == nullptr) {
+        if (epi.skip(red.st)) return;
+        if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
+        double acc[NA];
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        epi.load(red.st);
+        // contiguous chunk per workgroup, a multiple of BLOCK rows
+        const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
+        ell_rows<W, T>(rb, min(n, rb + chunk), n, ci, av, epi, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
+        return;
+    }
+    const int cnt = ew.count();
+    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+        const int t = ew.entry(v);
+        const RowTab tb = ew.tab[t];
+        KState* st = red.st + tb.z;
+        if (epi.skip(st)) continue;
+        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
+        Epi e = epi;
+        e.load(st);
+        double acc[NA];
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+        ell_rows<W, T>(tb.x, tb.y, n, ci, av, e, acc);
+        if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow, t);
+    }
+}
+
 // Elementwise kernel with fused dot products.
 template <class Op>
 __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew) {
