@@ -71,7 +71,7 @@ DEFAULT_STEPS = {"c3": 100, "c3-mixed": 100, "c4": 200}
 # (KiB, x2.000 by the 256 MiB copy calibration) + WRITE_SIZE (KiB, x1.000).
 # Counter values are not readable from inside the process, so the figure of
 # the profiled run is reported here.
-MEASURED_TRAFFIC = {"c1": 29.64e6}
+MEASURED_TRAFFIC = {"c1": 28.59e6}
 
 
 def parse():
@@ -125,7 +125,7 @@ def cpu_baseline(pb, steps):
     return steps / el, el, h
 
 
-def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8):
+def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8, fmt="csr"):
     """Mean HBM rate of the timed SpMV launches: algorithmic bytes of the rows
     each launch worked on (the full operator's bytes per row) over their
     summed duration.  value_bytes 4: the FP32 SpMVs of a mixed-precision run."""
@@ -137,7 +137,8 @@ def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8):
     achieved = moved / (spmv_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"k_spmv (CSR SpMV, {'FP32' if value_bytes == 4 else 'FP64'}, fused Krylov epilogues)",
+            "kernel": (f"{'k_spmv_ell (ELL' if fmt == 'ell' else 'k_spmv (CSR'} SpMV, "
+                       f"{'FP32' if value_bytes == 4 else 'FP64'}, fused Krylov epilogues)"),
             "bytes_per_launch": round(moved / launches),
             "full_operator_bytes": full, "launches": launches,
             "avg_launch_us": round(spmv_ms / launches * 1e3, 3)}
@@ -191,8 +192,9 @@ def run_ensemble(args):
                    "ensemble_steps_per_s": round(args.steps / el, 3),
                    "max_picard_iters_per_step": round(picard / args.steps, 3),
                    "krylov_iters_per_step_summed": round(kry / args.steps, 1),
-                   "all_converged": bool(conv), "spmv_lanes": ens.lanes},
-        "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8),
+                   "all_converged": bool(conv), "spmv_format": ens.spmv_format},
+        "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8,
+                             fmt=ens.spmv_format),
     }
     if mixed:
         # error against the FP64 ensemble: both forms rerun untimed for the
@@ -453,7 +455,7 @@ def main():
                    "krylov_iters_per_step": round(kry / args.steps, 3)},
         # mixed precision: the FP32 SpMVs of the inner solves, at FP32 bytes
         "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows, MEASURED_TRAFFIC.get(args.workload),
-                             value_bytes=4 if precision == "mixed" else 8),
+                             value_bytes=4 if precision == "mixed" else 8, fmt=ctx.spmv_format),
     }
     if args.cpu_steps > 0:
         pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,

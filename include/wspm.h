@@ -133,7 +133,10 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems,
                         wspm_ctx** out);
 void wspm_ctx_destroy(wspm_ctx* ctx);
 int32_t wspm_set_stream(wspm_ctx* ctx, void* stream);
-/* info (host int64[8]): n_nodes, n_elems, nnz, dim, max_row_len, n_bc, 0, 0 */
+/* info (host int64[8]): n_nodes, n_elems, nnz, dim, max_row_len, n_bc, the
+ * CSR SpMV's lanes per row, and 1 if the time-step solves of the last
+ * wspm_richards_init / wspm_ens_richards_init read the operator in ELL order
+ * (0: CSR). */
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info);
 /* Copy the CSR pattern to host: row_ptr int32[n+1], cols int32[nnz]. */
 int32_t wspm_get_pattern(const wspm_ctx* ctx, int32_t* row_ptr, int32_t* cols);

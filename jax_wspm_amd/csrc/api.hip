@@ -488,6 +488,7 @@ struct wspm_ctx {
     int ell_w = 0;
     bool ell = false;
     bool spmv_ell = false;
+    bool ell_faster = true;         // ensembles: the autotuner timed ELL against the best CSR width
     int asm_kernel = ASM_K_LDS;                   // ASM_K_*
     int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
@@ -1131,7 +1132,7 @@ int32_t wspm_set_stream(wspm_ctx* ctx, void* stream) {
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info) {
     if (!ctx || !info) return WSPM_EINVAL;
     info[0] = ctx->n; info[1] = ctx->E; info[2] = ctx->nnz; info[3] = ctx->dim;
-    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = (int64_t)ctx->asm_lds;
+    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->ell ? 1 : 0;
     return WSPM_OK;
 }
 
@@ -1542,14 +1543,18 @@ static size_t op_cap(const wspm_ctx* ctx) {
 
 // ELL for the time steps: 2D register assembly, rows of 7 or 8 entries, at
 // most 25 % padding, and a Krylov solve with no preconditioner or Jacobi
-// (the general preconditioners and the direct solver read CSR).
-// WSPM_SPMV=csr keeps CSR.
-static bool ell_ok(const wspm_ctx* ctx, const wspm_richards_params* prm) {
+// (the general preconditioners and the direct solver read CSR); on an
+// ensemble, only if the autotuner measured it faster.  WSPM_SPMV=csr keeps CSR.
+static bool ell_shape_ok(const wspm_ctx* ctx) {
     const char* env = std::getenv("WSPM_SPMV");
     if (env && std::string(env) == "csr") return false;
     if (ctx->dim != 2 || !ctx->asm_reg || ctx->asm_kernel == ASM_K_LDS) return false;
     if (ctx->max_row != 7 && ctx->max_row != 8) return false;
-    if ((double)ctx->max_row * ctx->n > 1.25 * (double)ctx->nnz) return false;
+    return (double)ctx->max_row * ctx->n <= 1.25 * (double)ctx->nnz;
+}
+
+static bool ell_ok(const wspm_ctx* ctx, const wspm_richards_params* prm) {
+    if (!ell_shape_ok(ctx) || (ctx->tab && !ctx->ell_faster)) return false;
     return prm->method != WSPM_DIRECT && prm->precond <= WSPM_PRECOND_JACOBI;
 }
 
@@ -2132,7 +2137,8 @@ static void ens_tune_lanes(wspm_ctx* ctx) {
     }
     double* av = nullptr;
     double* xy = nullptr;
-    HIPCHK(hipMalloc(&av, ctx->nnz * sizeof(double)));
+    const size_t cap = op_cap(ctx);
+    HIPCHK(hipMalloc(&av, cap * sizeof(double)));
     hipError_t e = hipMalloc(&xy, 2 * (size_t)ctx->n * sizeof(double));
     if (e != hipSuccess) {
         (void)hipFree(av);
@@ -2140,7 +2146,7 @@ static void ens_tune_lanes(wspm_ctx* ctx) {
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     try {
-        HIPCHK(hipMemsetAsync(av, 0, ctx->nnz * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(av, 0, cap * sizeof(double), ctx->stream));
         HIPCHK(hipMemsetAsync(xy, 0, 2 * (size_t)ctx->n * sizeof(double), ctx->stream));
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -2163,6 +2169,19 @@ static void ens_tune_lanes(wspm_ctx* ctx) {
             }
         }
         ctx->lanes = best;
+        // the ELL layout against the best CSR width (same zero operator)
+        if (ell_shape_ok(ctx)) {
+            ensure_ell(ctx);
+            const EllScope scope(ctx, true);
+            for (int r = 0; r < 3; ++r) launch_spmv<double>(ctx, av, epi, 0);
+            HIPCHK(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 20; ++r) launch_spmv<double>(ctx, av, epi, 0);
+            HIPCHK(hipEventRecord(e1, ctx->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ctx->ell_faster = ms < best_ms;
+        }
     } catch (const Err&) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);

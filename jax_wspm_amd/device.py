@@ -278,6 +278,14 @@ class Context:
         return A, b, dinv
 
     # -- timing hooks ---------------------------------------------------------------
+    @property
+    def spmv_format(self) -> str:
+        """Operator layout of the time-step solves since the last Richards
+        init: "ell" or "csr"."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        return "ell" if info[7] else "csr"
+
     def timing(self, on: bool, fp32: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
         mixed-precision inner solves)."""

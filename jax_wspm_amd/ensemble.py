@@ -171,6 +171,14 @@ class Ensemble:
         x = x.cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
         return [x[self.node_off[i]:self.node_off[i + 1]] for i in range(self.n_inst)]
 
+    @property
+    def spmv_format(self) -> str:
+        """Operator layout of the time-step solves since the last Richards
+        init: "ell" or "csr"."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        return "ell" if info[7] else "csr"
+
     def timing(self, on: bool, fp32: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
         mixed-precision inner solves)."""
