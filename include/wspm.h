@@ -312,7 +312,10 @@ int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, co
 int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
                         const int32_t* elem_counts, const int32_t* conn,
                         const double* coords, void* stream, wspm_ctx** out);
-/* prm: shared settings (CG or BiCGSTAB, FP64); phi: host double [n_inst * 6]
+/* prm: shared settings (CG or BiCGSTAB; no preconditioner or Jacobi; FP64,
+ * or WSPM_MIXED: per-instance iterative refinement, FP32 corrections of the
+ * FP64 residual until it meets prm->tol, with prm->inner_tol for the FP32
+ * solves); phi: host double [n_inst * 6]
  * per-instance soil parameters (layout of wspm_richards_params.phi), or NULL
  * for prm->phi everywhere.  Ks_elem dev [total elements] or NULL; h0 dev
  * [total nodes]. */

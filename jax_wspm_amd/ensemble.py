@@ -12,6 +12,10 @@ inputs (its dot products cover its own rows only).
 each unit-square instance drawn from {32, 48, ..., 256}, alpha ~ U(0.01, 0.2),
 n ~ U(1.3, 3.0) (m = 1 - 1/n), Ks ~ logU(1e-4, 1e-2), theta_r ~ U(0.02, 0.1),
 theta_s ~ U(0.35, 0.5); h0 = -1000, h = -75 on the top edge.
+
+``richards_init(precision="mixed")`` solves each instance's systems with FP32
+Krylov corrections of the FP64 residual (refined per instance until that
+residual meets the FP64 tolerance).
 """
 from __future__ import annotations
 

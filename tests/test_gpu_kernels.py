@@ -257,3 +257,32 @@ def test_krylov_general_preconditioners_match_oracle(method, precond):
     if method == "gmres":
         assert st["arnoldi"] == info["arnoldi"]
     assert rel_l2(x_g.cpu().numpy(), x_o) <= 1e-10
+
+
+def test_ell_and_csr_time_steps_agree(monkeypatch):
+    """The time-step solves read the operator in ELL order where the rows
+    allow it; WSPM_SPMV=csr keeps CSR.  Both formats give the same
+    trajectory (different summation order only: <= 1e-13) with the same
+    Picard and Krylov iteration counts."""
+    import numpy as np
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    from jax_wspm_amd.richards import RichardsSolver
+    print("[case] ELL vs CSR time steps 48x48", flush=True)
+    out = {}
+    for fmt in ("ell", "csr"):
+        if fmt == "csr":
+            monkeypatch.setenv("WSPM_SPMV", "csr")
+        else:
+            monkeypatch.delenv("WSPM_SPMV", raising=False)
+        pb = from_config(baseline_vg_config(48, 2, solver="bicgstab", precond="jacobi",
+                                            ks_field={"sigma": 0.5, "length": 0.1, "seed": 1}, nsteps=6, dt=1e-4))
+        s = RichardsSolver(pb)
+        res = s.solve()
+        assert s.ctx.spmv_format == fmt
+        s.ctx.close()
+        out[fmt] = res
+    a, b = out["ell"], out["csr"]
+    assert list(a["iterations"]) == list(b["iterations"])
+    assert list(a["krylov_iterations"]) == list(b["krylov_iterations"])
+    for k in range(len(a["times"])):
+        assert np.linalg.norm(a["pressure_head"][k] - b["pressure_head"][k]) <= 1e-13 * np.linalg.norm(b["pressure_head"][k])
