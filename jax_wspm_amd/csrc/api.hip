@@ -2339,29 +2339,33 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
 // iteration chunk.
 // Mixed-precision linear solves of one ensemble Picard iteration: the
 // assembly left r = b - A x (x = h_m) and |r|, |b| per instance.  Sweeps of
-// FP32 Krylov corrections on the instances still refining (their own
-// tolerance test, mode 2 lists), each followed by the FP64 update and
-// residual, until every instance's FP64 residual meets the Krylov tolerance.
+// FP32 Krylov corrections on the instances still refining (device-compacted
+// lists; each instance's own FP64 residual test sets rdone), each followed by
+// the FP64 update and residual, until every instance's FP64 residual meets
+// the Krylov tolerance.  The host reads the count of instances still refining
+// once per ENS_SWEEPS sweeps: the corrections use the true FP64 residual, so
+// an inner solve cut at ENS_INNER iterations only costs another sweep, and
+// instances that are done drop out of the lists on the device.
+static constexpr int ENS_SWEEPS = 2;
+static constexpr int ENS_INNER = 2;
+
 static void ens_mixed_solve(wspm_ctx* ctx, bool jac) {
     const wspm_richards_params& P = ctx->prm;
     while (true) {
-        ens_compact(ctx, LIST_K);                 // instances whose refinement goes on (done clear)
-        poll_ens(ctx, 0);
-        if (ctx->hstatus[0] == 0) break;
-        launch_vec(ctx, RefScaleE{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
-        KrylovRun<float> run(ctx, P.method, ctx->A32, jac ? ctx->dinv32 : nullptr, ctx->ref_r32, ctx->ref_d32,
-                             ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, false);
-        ens_compact(ctx, LIST_K);
-        run.enqueue(2);
-        poll_ens(ctx, 0);
-        while (ctx->hstatus[0] > 0 && run.enq < P.maxiter) {
-            run.enqueue(2);
-            poll_ens(ctx, 0);
+        for (int sw = 0; sw < ENS_SWEEPS; ++sw) {
+            ens_compact(ctx, LIST_K);                 // instances still refining (done clear)
+            launch_vec(ctx, RefScaleE{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
+            KrylovRun<float> run(ctx, P.method, ctx->A32, jac ? ctx->dinv32 : nullptr, ctx->ref_r32, ctx->ref_d32,
+                                 ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, false);
+            ens_compact(ctx, LIST_K);
+            run.enqueue(std::min(ENS_INNER, P.maxiter));
+            ens_compact(ctx, LIST_K, 2);              // the instances refined in this sweep
+            launch_vec(ctx, RefCorrectE{ctx->hB, ctx->ref_d32, 0.0}, 0, LIST_K);
+            launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 0}, 0, true,
+                                LIST_K);
         }
-        ens_compact(ctx, LIST_K, 2);              // the instances refined in this sweep
-        launch_vec(ctx, RefCorrectE{ctx->hB, ctx->ref_d32, 0.0}, 0, LIST_K);
-        launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 0}, 0, true,
-                            LIST_K);
+        poll_ens(ctx, 0);                             // hstatus[0]: instances whose refinement goes on
+        if (ctx->hstatus[0] == 0) break;
     }
 }
 

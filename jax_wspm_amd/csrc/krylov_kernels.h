@@ -301,9 +301,12 @@ struct BicgQ {  // q = A p_hat; alpha = rho_ / (r_hat . q)
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return (double)v.ph[j]; }
     __device__ void row(int i, double ax, double* acc) const {
+        // the row-local operand is read before the store (q may alias it for
+        // all the compiler knows), so its load issues with the SpMV's loads
+        const double rh = (double)v.rh[i];
         const T qT = (T)ax;
         v.q[i] = qT;
-        acc[0] += (double)v.rh[i] * (double)qT;
+        acc[0] += rh * (double)qT;
     }
     __device__ void fin(const double* t, KState* s) const { s->alpha = s->rho_next / t[0]; }
 };
@@ -336,10 +339,11 @@ struct BicgT {  // t = A s_hat; omega = (t.s)/(t.t)
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return (double)v.sh[j]; }
     __device__ void row(int i, double ax, double* acc) const {
+        const double ri = (double)v.r[i];   // before the store, as in BicgQ
         const T tT = (T)ax;
         v.t[i] = tT;
         const double td = (double)tT;
-        acc[0] += td * (double)v.r[i];
+        acc[0] += td * ri;
         acc[1] += td * td;
     }
     __device__ void fin(const double* t, KState* s) const { s->omega = t[0] / t[1]; }
@@ -427,9 +431,10 @@ struct CgAp {  // q = A p; alpha = gamma / (p . q)
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return (double)v.p[j]; }
     __device__ void row(int i, double ax, double* acc) const {
+        const double pi = (double)v.p[i];   // before the store, as in BicgQ
         const T qT = (T)ax;
         v.q[i] = qT;
-        acc[0] += (double)v.p[i] * (double)qT;
+        acc[0] += pi * (double)qT;
     }
     __device__ void fin(const double* t, KState* s) const { s->alpha = s->gamma / t[0]; }
 };
