@@ -144,6 +144,38 @@ def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8, fmt="
             "avg_launch_us": round(spmv_ms / launches * 1e3, 3)}
 
 
+def spmv_timing_region(timing, timing_read, step, dt, steps, fp32):
+    """Run `steps` more steps with every SpMV launch bracketed by HIP events on
+    the solver's stream; returns (working launches, their summed ms, rows
+    worked on, wall seconds of the region)."""
+    import torch
+    if steps <= 0:
+        return 0, 0.0, 0, 0.0
+    torch.cuda.synchronize()
+    timing(True, fp32=fp32)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    launches, spmv_ms, rows = timing_read()
+    timing(False)
+    return launches, spmv_ms, rows, el
+
+
+def launch_note():
+    graphs = os.environ.get("WSPM_GRAPHS") == "1"
+    poll = "copy + stream synchronisation" if os.environ.get("WSPM_POLL") == "sync" else \
+        "a device-written sequence word in pinned memory"
+    return (("Picard iterations replayed from HIP graphs" if graphs else "stream launches") +
+            f"; the host reads the solver state back through {poll}")
+
+
+def timing_note(steps, el):
+    return (f"roofline from {steps} further steps of the same run with HIP events around every SpMV launch "
+            f"(stream launches, {el / max(steps, 1) * 1e3:.4f} ms/step there)")
+
+
 def run_ensemble(args):
     """c3: the whole ensemble in one context; the plain single-instance path and
     the oracle are timed on the same instances for comparison."""
@@ -163,7 +195,6 @@ def run_ensemble(args):
     for _ in range(args.warmup):
         ens.step(dt)
     torch.cuda.synchronize()
-    ens.timing(not args.no_timing, fp32=mixed)
     picard = kry = 0
     conv = True
     t0 = time.perf_counter()
@@ -176,8 +207,8 @@ def run_ensemble(args):
             print(f"[bench] c3 step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    launches, spmv_ms, rows = ens.timing_read()
-    ens.timing(False)
+    launches, spmv_ms, rows, el_ev = spmv_timing_region(ens.timing, ens.timing_read, ens.step, dt,
+                                                         0 if args.no_timing else args.steps, mixed)
     h = ens.state()[0]
     assert bool(torch.isfinite(h).all()), "non-finite pressure head"
     value = ens.n_inst * args.steps / el
@@ -192,7 +223,9 @@ def run_ensemble(args):
                    "ensemble_steps_per_s": round(args.steps / el, 3),
                    "max_picard_iters_per_step": round(picard / args.steps, 3),
                    "krylov_iters_per_step_summed": round(kry / args.steps, 1),
-                   "all_converged": bool(conv), "spmv_format": ens.spmv_format},
+                   "all_converged": bool(conv), "spmv_format": ens.spmv_format,
+                   "launch": launch_note(),
+                   "timing_region": timing_note(args.steps, el_ev)},
         "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8,
                              fmt=ens.spmv_format),
     }
@@ -421,9 +454,7 @@ def main():
     for _ in range(args.warmup):
         ctx.richards_step(dt)
     torch.cuda.synchronize()
-    ctx.timing(not args.no_timing, fp32=precision == "mixed")
     picard = kry = 0
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = ctx.richards_step(dt)
@@ -431,8 +462,10 @@ def main():
         kry += st["krylov_iters"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    launches, spmv_ms, rows = ctx.timing_read()
-    ctx.timing(False)
+    # SpMV launch durations: the next steps of the same run with every SpMV
+    # launch bracketed by HIP events (graph replay is off while events are on)
+    launches, spmv_ms, rows, el_ev = spmv_timing_region(ctx.timing, ctx.timing_read, ctx.richards_step, dt,
+                                                         0 if args.no_timing else args.steps, precision == "mixed")
     h, th, _ = ctx.richards_state()
     assert bool(torch.isfinite(h).all()), "non-finite pressure head"
     value = args.steps / el
@@ -452,7 +485,9 @@ def main():
         "config": {"workload": desc, "nodes": ctx.n, "elements": ctx.n_elems, "nnz": ctx.nnz, "dt": dt,
                    "solver": f"{solver}+jacobi", "precision": precision,
                    "picard_iters_per_step": round(picard / args.steps, 3),
-                   "krylov_iters_per_step": round(kry / args.steps, 3)},
+                   "krylov_iters_per_step": round(kry / args.steps, 3),
+                   "launch": launch_note(),
+                   "timing_region": timing_note(args.steps, el_ev)},
         # mixed precision: the FP32 SpMVs of the inner solves, at FP32 bytes
         "roofline": roofline(ctx.n, ctx.nnz, launches, spmv_ms, rows, MEASURED_TRAFFIC.get(args.workload),
                              value_bytes=4 if precision == "mixed" else 8, fmt=ctx.spmv_format),

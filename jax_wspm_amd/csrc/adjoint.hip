@@ -40,6 +40,18 @@ __device__ inline D3 dpow(const D3& a, const D3& b) {
     for (int k = 0; k < 3; ++k) r.d[k] = p * (b.d[k] * la + b.v * a.d[k] / a.v);
     return r;
 }
+// log1p(a), d = a' / (1 + a); expm1(a), d = e^a a'
+__device__ inline D3 dlog1p(const D3& a) {
+    D3 r{log1p(a.v), {}};
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] / (1.0 + a.v);
+    return r;
+}
+__device__ inline D3 dexpm1(const D3& a) {
+    const double e = expm1(a.v);
+    D3 r{e, {}};
+    for (int k = 0; k < 3; ++k) r.d[k] = (e + 1.0) * a.d[k];
+    return r;
+}
 __device__ inline D3 dsqrt(const D3& a) {
     const double s = sqrt(a.v);
     D3 r{s, {}};
@@ -66,7 +78,7 @@ __device__ inline void soil_dual(const SoilParams& sp, double hv, D3& th, D3& kr
     const D3 bm = dpow(base, -m);
     th = dconst(ts - tr) * bm + dconst(tr);
     const D3 se = (th - dconst(tr)) / dconst(ts - tr);
-    const D3 inner = one - dpow(one - dpow(se, one / m), m);
+    const D3 inner = -dexpm1(m * dlog1p(-(one / base)));
     kr = dsqrt(se) * (inner * inner);
 }
 

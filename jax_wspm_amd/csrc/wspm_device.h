@@ -190,18 +190,22 @@ __device__ inline void soil_eval(const SoilParams& sp, double h, double& C, doub
     if (sp.model == 1) {
         const double alpha = sp.p[0], ts = sp.p[1], tr = sp.p[2], n = sp.p[3], m = sp.p[4], ks = sp.p[5];
         if (h < 0.0) {
-            // four pow calls instead of six: (a|h|)^(n-1) = (a|h|)^n / (a|h|),
+            // two pow calls instead of six: (a|h|)^(n-1) = (a|h|)^n / (a|h|),
             // (1+(a|h|)^n)^(-m-1) = (1+(a|h|)^n)^(-m) / (1+(a|h|)^n), Se^0.5 = sqrt(Se)
-            // (each within an ulp of the reference's pow); the Mualem term keeps
-            // the reference's form 1 - (1 - Se^(1/m))^m, whose cancellation near
-            // saturation a rewrite would change.
+            // (each within an ulp of the reference's pow).  The Mualem term
+            // 1 - (1 - Se^(1/m))^m, with Se^(1/m) = 1 / (1+(a|h|)^n), is taken as
+            // -expm1(m log1p(-1/(1+(a|h|)^n))): the reference's form cancels
+            // when Se is small (dry soil) and carries the rounding of
+            // theta - theta_r; this one does neither.  K agrees with the
+            // reference's expression to its own rounding error (about 1e-12
+            // relative where K is above 1e-16 Ks, measured).
             const double ah = alpha * fabs(h);
             const double ahn = pow(ah, n);
             const double base = 1.0 + ahn;
             const double bm = pow(base, -m);
             th = (ts - tr) * bm + tr;
             const double se = (th - tr) / (ts - tr);
-            const double inner = 1.0 - pow(1.0 - pow(se, 1.0 / m), m);
+            const double inner = -expm1(m * log1p(-1.0 / base));
             K = ks * (sqrt(se) * (inner * inner));
             const double ahn1 = ah > 0.0 ? ahn / ah : pow(ah, n - 1.0);
             C = (ts - tr) * alpha * m * n * ahn1 * (bm / base);

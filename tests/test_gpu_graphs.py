@@ -1,0 +1,78 @@
+"""HIP-graph replay of whole Picard iterations and the spin-wait read-back.
+
+The time step replays each Picard iteration (assembly + solver start, the
+expected Krylov iterations, the Picard error, the state publication) from a
+captured HIP graph and reads the device state back through a sequence word in
+pinned host memory.  WSPM_GRAPHS=0 launches every kernel on the stream and
+WSPM_POLL=sync reads back by copy + stream synchronisation.  Every kernel is
+deterministic, so all modes must give bitwise identical trajectories and
+iteration counts; each graph is replayed many times, with the host's own
+launches (Krylov follow-ups, the theta_n copy) and synchronisations between
+the replays.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = [("1", "spin"), ("0", "spin"), ("0", "sync")]  # (WSPM_GRAPHS, WSPM_POLL)
+
+
+def _single(monkeypatch, graphs, poll, solver, precond, dts):
+    from jax_wspm_amd.device import Context
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    monkeypatch.setenv("WSPM_GRAPHS", graphs)
+    monkeypatch.setenv("WSPM_POLL", poll)
+    pb = from_config(baseline_vg_config(40, 2, solver=solver, precond=precond, nsteps=1, dt=1e-4,
+                                        ks_field={"sigma": 0.5, "length": 0.05, "seed": 3}))
+    ctx = Context(pb.points, pb.elements)
+    ctx.set_dirichlet(pb.bc_lists)
+    ctx.richards_init("vg", pb.phi, solver, precond, pb.h0, Ks_elem=pb.Ks_elem)
+    hs, its = [], []
+    for dt in dts:
+        st = ctx.richards_step(dt)
+        its.append((st["picard_iters"], st["krylov_iters"]))
+        hs.append(ctx.richards_state()[0].cpu().numpy().copy())
+    ctx.close()
+    return hs, its
+
+
+@pytest.mark.parametrize("solver,precond", [("bicgstab", "jacobi"), ("cg", "jacobi"), ("bicgstab", "none")])
+def test_single_context_modes_bitwise(monkeypatch, solver, precond):
+    # repeated and alternating dt values: graphs are looked up by dt, by the
+    # buffer holding the Picard iterate and by the Krylov chunk
+    dts = [1e-4] * 5 + [2e-4, 1e-4, 2e-4, 2e-4, 1e-4]
+    print(f"[case] single context {solver}+{precond}", flush=True)
+    ref_h, ref_it = _single(monkeypatch, "0", "sync", solver, precond, dts)
+    for graphs, poll in MODES[:2]:
+        h, it = _single(monkeypatch, graphs, poll, solver, precond, dts)
+        assert it == ref_it, (graphs, poll)
+        for k in range(len(dts)):
+            assert np.array_equal(h[k], ref_h[k]), (graphs, poll, k)
+
+
+def _ensemble(monkeypatch, graphs, poll, solver, steps):
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    monkeypatch.setenv("WSPM_GRAPHS", graphs)
+    monkeypatch.setenv("WSPM_POLL", poll)
+    insts = vg_ensemble(n_inst=6, sides=(16, 32, 24, 40, 20, 36), seed=11)
+    ens = Ensemble(insts)
+    ens.richards_init("vg", solver, "jacobi")
+    hs, its = [], []
+    for _ in range(steps):
+        st = ens.step(1e-4, per_instance=True)
+        its.append((st["picard_iters_i"].copy(), st["krylov_iters_i"].copy()))
+        hs.append(ens.state()[0].cpu().numpy().copy())
+    ens.close()
+    return hs, its
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cg"])
+def test_ensemble_modes_bitwise(monkeypatch, solver):
+    print(f"[case] ensemble {solver}", flush=True)
+    ref_h, ref_it = _ensemble(monkeypatch, "0", "sync", solver, 6)
+    for graphs, poll in MODES[:2]:
+        h, it = _ensemble(monkeypatch, graphs, poll, solver, 6)
+        for k in range(len(h)):
+            assert np.array_equal(it[k][0], ref_it[k][0]) and np.array_equal(it[k][1], ref_it[k][1]), (graphs, k)
+            assert np.array_equal(h[k], ref_h[k]), (graphs, poll, k)
