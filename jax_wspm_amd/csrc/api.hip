@@ -519,6 +519,9 @@ struct wspm_ctx {
     // whether the SpMV launches read ELL right now
     int32_t* ell_col = nullptr;
     int ell_w = 0;
+    int2* ell_inc = nullptr;        // neighbourhood assembly: incidences in ELL layout
+    uint32_t* ell_info = nullptr;   // and rl | dg << 4 | nq << 8 per row
+    int ell_iw = 0;
     bool ell = false;
     bool spmv_ell = false;
     bool ell_faster = true;         // ensembles: the autotuner timed ELL against the best CSR width
@@ -1396,6 +1399,13 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.ell_n = ell ? ctx->n : 0;    // callers pass ell only with the register kernels
+    if (ell && ctx->ell_info && ctx->asm_kernel == ASM_K_NBR) {
+        a.ell_col = ctx->ell_col;
+        a.ell_inc = ctx->ell_inc;
+        a.ell_info = ctx->ell_info;
+        a.ell_w = ctx->ell_w;
+        a.ell_iw = ctx->ell_iw;
+    }
     a.qc = ctx->qc;
     a.tab = ctx->tab;
     a.act = ctx->pact;
@@ -1732,6 +1742,24 @@ static void ensure_ell(wspm_ctx* ctx) {
     }
     ctx->ell_col = dupload(ctx, c.data(), c.size());
     ctx->ell_w = W;
+    // the neighbourhood assembly's incidences and row shape in the same layout
+    // (WSPM_ASM_ROWS=csr: it reads them through the CSR row pointers instead)
+    const char* ar = std::getenv("WSPM_ASM_ROWS");
+    if (ctx->asm_kernel == ASM_K_NBR && !(ar && std::string(ar) == "csr")) {
+        int iw = 0;
+        for (int i = 0; i < n; ++i) iw = std::max(iw, P.inc_ptr[i + 1] - P.inc_ptr[i]);
+        std::vector<int2> ic((size_t)iw * n, make_int2(0, 0));
+        std::vector<uint32_t> info(n);
+        for (int i = 0; i < n; ++i) {
+            const int q0 = P.inc_ptr[i], nq = P.inc_ptr[i + 1] - q0;
+            for (int u = 0; u < nq; ++u) ic[(size_t)u * n + i] = make_int2(P.inc_code[q0 + u], (int)P.inc_slot[q0 + u]);
+            const int rl = P.row_ptr[i + 1] - P.row_ptr[i], dg = P.diag_pos[i] - P.row_ptr[i];
+            info[i] = (uint32_t)rl | (uint32_t)dg << 4 | (uint32_t)nq << 8;
+        }
+        ctx->ell_inc = dupload(ctx, ic.data(), ic.size());
+        ctx->ell_info = dupload(ctx, info.data(), info.size());
+        ctx->ell_iw = iw;
+    }
 }
 
 // The SpMV launches of a time step read the ELL operator while this lives.

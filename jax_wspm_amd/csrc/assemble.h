@@ -77,6 +77,16 @@ struct AsmArgs {
     double* b;
     double* dinv;            // nullable
     float* dinv32;           // nullable
+    // Row-major-free row access for the neighbourhood kernel (nullable): the
+    // ELL column table (entry s of row i at s * n + i, width ell_w), the
+    // incidences in the same layout (width ell_iw, padding (0, 0)) and per
+    // row rl | dg << 4 | nq << 8 (entries, diagonal slot, incidences).  Every
+    // load that depends only on the row index is then issued at once: two
+    // dependent memory round trips per row instead of three.
+    const int32_t* ell_col;
+    const int2* ell_inc;
+    const uint32_t* ell_info;
+    int ell_w, ell_iw;
     QuadConst qc;
     // Ensemble mode (register kernel only): a grid of ens_grid workgroups
     // walks the row-table entries listed in act (act[0] = count; null: all

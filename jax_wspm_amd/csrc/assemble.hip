@@ -393,17 +393,32 @@ __device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int 
 template <int MAXR, int MODE>
 __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
     constexpr int MAXI = MAXR - 1;
-    const int rs = a.row_ptr[i];
-    const int rl = a.row_ptr[i + 1] - rs;
-    const int dg = a.diag_pos[i] - rs;
-    const int q0 = a.inc_ptr[i], nq = a.inc_ptr[i + 1] - q0;
-    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
+    int rs, rl, dg, nq;
     int col[MAXR];
-#pragma unroll
-    for (int s = 0; s < MAXR; ++s) col[s] = s < rl ? a.cols[rs + s] : i;
     int2 ic[MAXI];
+    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
+    if (a.ell_info != nullptr) {
+        // all of the row's index data in one round trip (ELL layouts)
+        const uint32_t inf = a.ell_info[i];
+        rs = 0;
+        rl = (int)(inf & 15u);
+        dg = (int)((inf >> 4) & 15u);
+        nq = (int)((inf >> 8) & 15u);
 #pragma unroll
-    for (int u = 0; u < MAXI; ++u) ic[u] = (!dir && u < nq) ? a.inc[q0 + u] : make_int2(0, 0);
+        for (int s = 0; s < MAXR; ++s) col[s] = s < a.ell_w ? a.ell_col[(size_t)s * a.n + i] : i;
+#pragma unroll
+        for (int u = 0; u < MAXI; ++u) ic[u] = u < a.ell_iw ? a.ell_inc[(size_t)u * a.n + i] : make_int2(0, 0);
+    } else {
+        rs = a.row_ptr[i];
+        rl = a.row_ptr[i + 1] - rs;
+        dg = a.diag_pos[i] - rs;
+        const int q0 = a.inc_ptr[i];
+        nq = a.inc_ptr[i + 1] - q0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) col[s] = s < rl ? a.cols[rs + s] : i;
+#pragma unroll
+        for (int u = 0; u < MAXI; ++u) ic[u] = (!dir && u < nq) ? a.inc[q0 + u] : make_int2(0, 0);
+    }
     double4 nr[MAXR];
     double xv[MAXR];
 #pragma unroll
