@@ -239,7 +239,7 @@ def run_ensemble(args):
         # each form is from the converged solution.
         ens.close()
         res = {}
-        for prec, ktol in (("mixed", 1e-10), ("fp64", 1e-10), ("fp64", 1e-13)):
+        for prec, ktol in (("mixed", 1e-10), ("fp64", 1e-10), ("fp64", 1e-13), ("mixed", 1e-12), ("fp64", 1e-12)):
             e2 = Ensemble(insts)
             e2.richards_init("vg", solver, "jacobi", precision=prec, tol=ktol)
             its = [e2.step(dt, per_instance=True)["picard_iters_i"].copy() for _ in range(args.warmup + args.steps)]
@@ -264,6 +264,17 @@ def run_ensemble(args):
         cfg["max_rel_l2_h_vs_tight_fp64"] = {"mixed": float(e32T.max()), "fp64": float(e64T.max())}
         cfg["median_rel_l2_h_vs_tight_fp64"] = {"mixed": float(np.median(e32T)), "fp64": float(np.median(e64T))}
         cfg["error_sample"] = f"every instance after {args.warmup + args.steps} steps; tight = FP64, Krylov tol 1e-13"
+        # both forms with the Krylov tolerance at 1e-12: the dry instances'
+        # conditioning no longer hides the FP32 inner solves' effect
+        (i32b, h32b, _), (i64b, h64b, _) = res[("mixed", 1e-12)], res[("fp64", 1e-12)]
+        sameb = np.all(i32b == i64b, axis=0)
+        ehb = rel(h32b, h64b)
+        cfg["krylov_tol_1e-12"] = {
+            "max_rel_l2_mixed_vs_fp64_h": float(ehb.max()),
+            "median_rel_l2_mixed_vs_fp64_h": float(np.median(ehb)),
+            "instances_within_1e-8": int((ehb <= 1e-8).sum()),
+            "instances_with_identical_picard_histories": int(sameb.sum()),
+            "max_rel_l2_h_identical_histories": float(ehb[sameb].max()) if sameb.any() else None}
     else:
         ens.close()
     del ens

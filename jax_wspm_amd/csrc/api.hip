@@ -134,8 +134,6 @@ __global__ void k_ens_step_begin(KState* st, int n_inst) {
     }
 }
 
-// out[0] = instances whose Krylov solve is still running, out[1] = instances
-// still iterating Picard, out[2] = most Krylov steps of a solve.  One workgroup.
 // Host read-back without a stream synchronisation: copy nwords 32-bit words
 // of device state into pinned host memory, then raise the sequence word
 // there.  The sequence value comes from a device counter (dseq, incremented
@@ -162,9 +160,35 @@ __global__ __launch_bounds__(BLOCK) void k_publish(const uint32_t* __restrict__ 
     publish_seq(dseq, hseq);
 }
 
-// Counts of running instances: out[0] Krylov-active, out[1] Picard-active,
-// out[2] the largest Krylov step count.  hseq non-null: out is pinned host
-// memory and the counts are published as by k_publish.
+// k_finish (one workgroup) that then publishes the whole state as k_publish
+// does: the Picard error's reduction and the host read-back in one kernel.
+template <int ND, class Op>
+__global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __restrict__ partials, int prow,
+                                                      int nblk, KState* st, uint32_t* hdst, int* dseq, int* hseq) {
+    if (!op.skip(st)) {   // uniform over the workgroup
+        double acc[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += BLOCK) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
+        }
+        __shared__ double sh[ND * NWAVE];
+        block_sum<ND>(acc, sh);
+        if (threadIdx.x == 0) op.fin(acc, st);
+    }
+    // thread 0's state stores, seen by the whole workgroup (one CU)
+    __threadfence_block();
+    __syncthreads();
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
+    for (int k = threadIdx.x; k < (int)(sizeof(KState) / 4); k += BLOCK) hdst[k] = src[k];
+    publish_seq(dseq, hseq);
+}
+
+// Counts of running instances (one workgroup): out[0] instances whose Krylov
+// solve is still running, out[1] instances still iterating Picard, out[2] the
+// most Krylov steps of a solve.  hseq non-null: out is pinned host memory and
+// the counts are published as by k_publish.
 __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__ st, int n_inst, int* out,
                                                       int* dseq, int* hseq) {
     __shared__ int cnt[3];
@@ -820,6 +844,31 @@ void poll_ens(wspm_ctx* ctx, int which) {
     }
     HIPCHK(hipMemcpyAsync(ctx->hstatus, ctx->ens_status, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+// The Picard error and the soil model at the new iterate (one system), with
+// the state published for the host when fast_poll: the error's finish kernel
+// does the publication (one kernel less per Picard iteration).  picard_poll
+// also waits for it.
+void picard_enqueue(wspm_ctx* ctx, const PicardOp& pop) {
+    if (!ctx->fast_poll) {
+        launch_vec(ctx, pop, 0);
+        return;
+    }
+    const int grid = ctx->vec_grid;
+    hipLaunchKernelGGL(k_vec<PicardOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, pop, red_of(ctx, 0),
+                       walk_of(ctx, LIST_ALL));
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_finish_pub<PicardOp::ND, PicardOp>), dim3(1), dim3(BLOCK), 0, ctx->stream, pop,
+                       ctx->partials, ctx->prow, grid, states_of(ctx, 0),
+                       reinterpret_cast<uint32_t*>(ctx->hstate), ctx->dseq, ctx->hseq);
+    HIPCHK(hipGetLastError());
+}
+
+void picard_poll(wspm_ctx* ctx, const PicardOp& pop) {
+    picard_enqueue(ctx, pop);
+    if (ctx->fast_poll) wait_published(ctx);
+    else poll(ctx, 0);
 }
 
 enum { GRAPH_PICARD = 0, GRAPH_ENS_PICARD = 1, GRAPH_ENS_CHUNK = 2 };
@@ -1884,8 +1933,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                             jac ? ctx->dinv : nullptr, nullptr, nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter,
                             P.restart, ctx->ell);
                 run.enqueue(K);
-                launch_vec(ctx, pop, 0);
-                publish_state(ctx, 0);
+                picard_enqueue(ctx, pop);
             });
             run.enq = K;
             HIPCHK(hipGraphLaunch(ex, ctx->stream));
@@ -1895,8 +1943,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                     run.enqueue(1);
                     poll(ctx, 0);
                 }
-                launch_vec(ctx, pop, 0);
-                poll(ctx, 0);
+                picard_poll(ctx, pop);
             }
             run.stats(&st);
             kry += st.iters;
@@ -1924,14 +1971,12 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                            P.picard_mode == WSPM_PICARD_REL};
         if (direct) {
             direct_solve(ctx, ctx->Av, ctx->bv, ctx->hB);
-            launch_vec(ctx, pop, 0);
-            poll(ctx, 0);
+            picard_poll(ctx, pop);
             if (direct_info(ctx)) throw ApiFail{WSPM_EINVAL, "direct solve: singular matrix"};
         } else if (mixed) {
             mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
                         ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !own_start);
-            launch_vec(ctx, pop, 0);
-            poll(ctx, 0);
+            picard_poll(ctx, pop);
         } else {
             // Enqueue the expected number of Krylov iterations and the Picard
             // error behind them, then read both back with one synchronisation;
@@ -1939,15 +1984,13 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
                                   P.tol, P.atol, P.maxiter, P.restart, 0, !own_start, P.precond);
             run.enqueue(run.first_chunk());
-            launch_vec(ctx, pop, 0);
-            poll(ctx, 0);
+            picard_poll(ctx, pop);
             if (run.more()) {
                 while (run.more()) {
                     run.enqueue(1);
                     poll(ctx, 0);
                 }
-                launch_vec(ctx, pop, 0);
-                poll(ctx, 0);
+                picard_poll(ctx, pop);
             }
             run.stats(&st);
         }
