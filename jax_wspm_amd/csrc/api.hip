@@ -1258,7 +1258,10 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
             for (int i = 0; i < n_nodes; ++i) max_row_inc = std::max(max_row_inc, P.inc_ptr[i + 1] - P.inc_ptr[i]);
             // WSPM_ASM_KERNEL=inc keeps the incidence-gather register kernel
             const bool nbr = dim == 2 && max_row_inc <= asm_reg_max_row(dim) - 1 && !(ak && std::string(ak) == "inc");
-            ctx->asm_kernel = !ctx->asm_reg ? ASM_K_LDS : nbr ? ASM_K_NBR : ASM_K_REG;
+            // WSPM_ASM_KERNEL=blend: the neighbourhood kernel selecting its
+            // records from registers instead of LDS
+            const int nbk = (ak && std::string(ak) == "blend") ? ASM_K_NBR_BLEND : ASM_K_NBR;
+            ctx->asm_kernel = !ctx->asm_reg ? ASM_K_LDS : nbr ? nbk : ASM_K_REG;
             ctx->prow = std::max(MAX_BLOCKS, std::max(ctx->asm_grid, (n_nodes + R - 1) / R));
         }
         if (!ctx->asm_reg && ctx->asm_lds > 64 * 1024) {
@@ -1448,7 +1451,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.ell_n = ell ? ctx->n : 0;    // callers pass ell only with the register kernels
-    if (ell && ctx->ell_info && ctx->asm_kernel == ASM_K_NBR) {
+    if (ell && ctx->ell_info && asm_is_nbr(ctx->asm_kernel)) {
         a.ell_col = ctx->ell_col;
         a.ell_inc = ctx->ell_inc;
         a.ell_info = ctx->ell_info;
@@ -1794,7 +1797,7 @@ static void ensure_ell(wspm_ctx* ctx) {
     // the neighbourhood assembly's incidences and row shape in the same layout
     // (WSPM_ASM_ROWS=csr: it reads them through the CSR row pointers instead)
     const char* ar = std::getenv("WSPM_ASM_ROWS");
-    if (ctx->asm_kernel == ASM_K_NBR && !(ar && std::string(ar) == "csr")) {
+    if (asm_is_nbr(ctx->asm_kernel) && !(ar && std::string(ar) == "csr")) {
         int iw = 0;
         for (int i = 0; i < n; ++i) iw = std::max(iw, P.inc_ptr[i + 1] - P.inc_ptr[i]);
         std::vector<int2> ic((size_t)iw * n, make_int2(0, 0));

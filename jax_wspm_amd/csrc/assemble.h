@@ -117,7 +117,10 @@ int asm_reg_max_row(int dim);
 // Assembly kernels: LDS-staged (any rows), register with incidence gathers
 // (2D rows <= 8 entries), register with the row's neighbourhood gathered once
 // (2D rows <= 8 entries and <= 7 incidences).
-enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2 };
+// ASM_K_NBR stages the neighbourhood in LDS; ASM_K_NBR_BLEND selects from
+// registers (WSPM_ASM_KERNEL=blend).
+enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3 };
+inline bool asm_is_nbr(int k) { return k == ASM_K_NBR || k == ASM_K_NBR_BLEND; }
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, int kernel);
 

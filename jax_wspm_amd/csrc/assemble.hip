@@ -390,7 +390,17 @@ __device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int 
 // incidences, and rl node-record loads instead of 3 per incidence.  The
 // element arithmetic and the order of every sum are those of assemble_row,
 // so the results are identical.  Needs at most MAXR - 1 incidences per row.
-template <int MAXR, int MODE>
+// LDS = 1 (default): the row's neighbour records are staged in a per-thread
+// slice of LDS and each incidence reads its three vertices back by slot index
+// (3 LDS reads per vertex).  LDS = 0: they are taken from registers by a
+// one-hot blend over the slots (about 30 instructions per vertex).  Each
+// thread touches only its own slice ([component][slot][thread], so the lanes
+// of a wavefront hit distinct banks whatever their slots): no barrier.  Both
+// give bitwise the same results; LDS = 1 measured 2 % (512^2) to 4 %
+// (256-instance ensemble) faster per time step.  (Accumulating the row's
+// entries in LDS as well was no faster and changed the FMA contraction of
+// the sums.)
+template <int MAXR, int MODE, int LDS = 1>
 __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
     constexpr int MAXI = MAXR - 1;
     int rs, rl, dg, nq;
@@ -440,6 +450,17 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     } else {
         bi = 0.0;
         double msum = 0.0;
+        // per-thread LDS slice of the records (LDS == 1): [3][MAXR][BLOCK]
+        __shared__ double lds[LDS == 1 ? 3 * MAXR * BLOCK : 1];
+        double* const rec = lds + threadIdx.x;
+        if constexpr (LDS == 1) {
+#pragma unroll
+            for (int s = 0; s < MAXR; ++s) {
+                rec[(0 * MAXR + s) * BLOCK] = nr[s].x;
+                rec[(1 * MAXR + s) * BLOCK] = nr[s].y;
+                rec[(2 * MAXR + s) * BLOCK] = nr[s].z;
+            }
+        }
         // every iteration predicated (no early exit), so the loop unrolls fully
         // and ic / ks stay in registers
 #pragma unroll
@@ -449,23 +470,29 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
             double px[3], py[3], pz[3], kn[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                // one-hot blend of the row's records (exact: one weight is 1,
-                // the others 0 against finite values).  A select chain here is
-                // turned by the compiler into an indexed load from a private
-                // array, i.e. scratch memory.
                 const uint32_t sl = (w >> (8 * c)) & 0xffu;
-                double rx = 0.0, ry = 0.0, rk = 0.0;
+                if constexpr (LDS == 1) {
+                    px[c] = rec[(0 * MAXR + sl) * BLOCK];
+                    py[c] = rec[(1 * MAXR + sl) * BLOCK];
+                    kn[c] = rec[(2 * MAXR + sl) * BLOCK];
+                } else {
+                    // one-hot blend of the row's records (exact: one weight is 1,
+                    // the others 0 against finite values).  A select chain here is
+                    // turned by the compiler into an indexed load from a private
+                    // array, i.e. scratch memory.
+                    double rx = 0.0, ry = 0.0, rk = 0.0;
 #pragma unroll
-                for (int s = 0; s < MAXR; ++s) {
-                    const double m = (sl == (uint32_t)s) ? 1.0 : 0.0;
-                    rx += m * nr[s].x;
-                    ry += m * nr[s].y;
-                    rk += m * nr[s].z;
+                    for (int s = 0; s < MAXR; ++s) {
+                        const double m = (sl == (uint32_t)s) ? 1.0 : 0.0;
+                        rx += m * nr[s].x;
+                        ry += m * nr[s].y;
+                        rk += m * nr[s].z;
+                    }
+                    px[c] = rx;
+                    py[c] = ry;
+                    kn[c] = rk;
                 }
-                px[c] = rx;
-                py[c] = ry;
                 pz[c] = 0.0;
-                kn[c] = rk;
             }
             double out[5];
             element_compute<2>(a, px, py, pz, kn, ks[u], ic[u].x & 3, out);
@@ -509,13 +536,14 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
-template <int D, int MAXR, int MODE, bool NBR>
+// NB: 0 incidence gathers; 1, 2 neighbourhood gathers with LDS = NB - 1
+template <int D, int MAXR, int MODE, int NB>
 __device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
-    if constexpr (NBR && D == 2) assemble_row_nbr<MAXR, MODE>(a, ki, i, acc);
+    if constexpr (NB > 0 && D == 2) assemble_row_nbr<MAXR, MODE, NB - 1>(a, ki, i, acc);
     else assemble_row<D, MAXR, MODE>(a, ki, i, acc);
 }
 
-template <int D, int MAXR, int MODE, bool NBR>
+template <int D, int MAXR, int MODE, int NBR>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     if (a.tab == nullptr) {
         const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -593,7 +621,7 @@ static hipError_t launch_mode(int mode, dim3 g, size_t lds, hipStream_t s, const
     return hipGetLastError();
 }
 
-template <int D, int MAXR, bool NBR>
+template <int D, int MAXR, int NBR>
 static hipError_t launch_reg(int mode, dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki,
                              double* partials, int prow) {
     switch (mode) {
@@ -618,8 +646,9 @@ hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit
     if (kernel != ASM_K_LDS) {
         const int grid = a.tab ? a.ens_grid : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
-        if (kernel == ASM_K_NBR) return launch_reg<2, ASM_REG_ROW_2D, true>(mode, dim3(grid), s, a, ki, partials, prow);
-        return launch_reg<2, ASM_REG_ROW_2D, false>(mode, dim3(grid), s, a, ki, partials, prow);
+        if (kernel == ASM_K_NBR) return launch_reg<2, ASM_REG_ROW_2D, 2>(mode, dim3(grid), s, a, ki, partials, prow);
+        if (kernel == ASM_K_NBR_BLEND) return launch_reg<2, ASM_REG_ROW_2D, 1>(mode, dim3(grid), s, a, ki, partials, prow);
+        return launch_reg<2, ASM_REG_ROW_2D, 0>(mode, dim3(grid), s, a, ki, partials, prow);
     }
     const int R = asm_rows_per_block(dim);
     const int grid = (a.n + R - 1) / R;

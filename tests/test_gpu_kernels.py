@@ -130,9 +130,10 @@ def test_pattern_soil_assembly(name, mk, model, phi, use_ks, kernel, monkeypatch
 
 
 def test_register_kernels_agree_bitwise(monkeypatch):
-    """The neighbourhood kernel does the incidence kernel's arithmetic in the
-    same order: A, b and the Jacobi diagonal are bitwise equal (jittered mesh,
-    per-element Ks, Dirichlet rows)."""
+    """The neighbourhood kernels (records staged in LDS, the default; records
+    selected from registers, WSPM_ASM_KERNEL=blend) do the incidence kernel's
+    arithmetic in the same order: A, b and the Jacobi diagonal are bitwise
+    equal (jittered mesh, per-element Ks, Dirichlet rows)."""
     import torch
     pts, el = omesh.structured_tri(17, 1.0)
     rng = np.random.default_rng(8)
@@ -146,7 +147,7 @@ def test_register_kernels_agree_bitwise(monkeypatch):
     ks = np.exp(np.log(1e-3) + 0.5 * rng.standard_normal(len(el)))
     top = np.nonzero(pts[:, 1] == 1.0)[0]
     out = []
-    for kernel in ("auto", "inc"):
+    for kernel in ("auto", "inc", "blend"):
         print(f"[case] bitwise register kernels {kernel}", flush=True)
         monkeypatch.setenv("WSPM_ASM_KERNEL", kernel)
         ctx = _ctx(pts, el)
@@ -154,8 +155,10 @@ def test_register_kernels_agree_bitwise(monkeypatch):
         A, b, d = ctx.assemble_richards(h_m, Cm, Km, thm, thn, 1e-4, Ks_elem=ks)
         out.append([t.cpu().numpy() for t in (A, b, d)])
         ctx.close()
-    for x, y in zip(*out):
-        assert np.array_equal(x, y)
+    for kernel, other in zip(("inc", "blend"), out[1:]):
+        for name, x, y in zip("Abd", out[0], other):
+            assert np.array_equal(x, y), (kernel, name, int(np.sum(x != y)),
+                                          float(np.max(np.abs(x - y) / np.maximum(np.abs(x), 1e-300))))
 
 
 def _system(cells=12, dim=2, seed=0, dt=1e-4):
