@@ -145,9 +145,10 @@ def roofline(n, nnz, launches, spmv_ms, rows, traffic=None, value_bytes=8, fmt="
 
 
 def spmv_timing_region(timing, timing_read, step, dt, steps, fp32):
-    """Run `steps` more steps with every SpMV launch bracketed by HIP events on
-    the solver's stream; returns (working launches, their summed ms, rows
-    worked on, wall seconds of the region)."""
+    """Run `steps` more steps with start/stop HIP events attached to every
+    SpMV dispatch on the solver's stream (the kernel's own execution time);
+    returns (working launches, their summed ms, rows worked on, wall seconds
+    of the region)."""
     import torch
     if steps <= 0:
         return 0, 0.0, 0, 0.0
@@ -172,8 +173,8 @@ def launch_note():
 
 
 def timing_note(steps, el):
-    return (f"roofline from {steps} further steps of the same run with HIP events around every SpMV launch "
-            f"(stream launches, {el / max(steps, 1) * 1e3:.4f} ms/step there)")
+    return (f"roofline from {steps} further steps of the same run, every SpMV launched with start/stop HIP "
+            f"events attached to its dispatch (hipExtLaunchKernelGGL; {el / max(steps, 1) * 1e3:.4f} ms/step there)")
 
 
 def run_ensemble(args):
@@ -473,8 +474,8 @@ def main():
         kry += st["krylov_iters"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    # SpMV launch durations: the next steps of the same run with every SpMV
-    # launch bracketed by HIP events (graph replay is off while events are on)
+    # SpMV launch durations: the next steps of the same run with HIP events
+    # attached to every SpMV dispatch (graph replay is off while they are on)
     launches, spmv_ms, rows, el_ev = spmv_timing_region(ctx.timing, ctx.timing_read, ctx.richards_step, dt,
                                                          0 if args.no_timing else args.steps, precision == "mixed")
     h, th, _ = ctx.richards_state()
