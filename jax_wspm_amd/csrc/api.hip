@@ -309,6 +309,28 @@ struct RefScaleE {  // r32 = r / |r|, d32 = 0 on the instances whose inner solve
     __device__ void fin(const double*, KState*) const {}
 };
 
+// Start of an inner FP32 solve from a zero guess, fused with the scaling of
+// its right-hand side: r32 = r / |r|, d32 = 0, then the method's start row
+// (BicgInit / CgInit) with A d32 = 0, so no SpMV.  Instances whose solve does
+// not start (done set) keep their state.
+template <class Init>
+struct RefStartE {
+    static constexpr int ND = Init::ND;
+    Init init;            // init.v: b = r32, x0 = x = d32
+    const double* r;
+    float* r32;
+    float* d32;
+    double inv;
+    __device__ bool skip(const KState* s) const { return s->done; }
+    __device__ void load(const KState* s) { inv = s->rnorm > 0.0 ? 1.0 / s->rnorm : 0.0; }
+    __device__ void row(int i, double* acc) const {
+        r32[i] = (float)(r[i] * inv);
+        d32[i] = 0.0f;
+        init.row(i, 0.0, acc);
+    }
+    __device__ void fin(const double* t, KState* s) const { init.fin(t, s); }
+};
+
 struct RefCorrectE {  // x += |r| d32 on the instances refined in this sweep
     static constexpr int ND = 0;
     double* x;
@@ -362,6 +384,27 @@ struct RefScale {  // r32 = r / |r|; d32 = 0
         d32[i] = 0.0f;
     }
     __device__ void fin(const double*, KState*) const {}
+};
+
+// RefStartE for one system: the scale |r| comes from the refinement state,
+// the solver state written by the start is the inner one.
+template <class Init>
+struct RefStart {
+    static constexpr int ND = Init::ND;
+    Init init;
+    const double* r;
+    float* r32;
+    float* d32;
+    const double* rnorm;  // device address of the refinement state's |r|
+    double inv;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState*) { inv = *rnorm > 0.0 ? 1.0 / *rnorm : 0.0; }
+    __device__ void row(int i, double* acc) const {
+        r32[i] = (float)(r[i] * inv);
+        d32[i] = 0.0f;
+        init.row(i, 0.0, acc);
+    }
+    __device__ void fin(const double* t, KState* s) const { init.fin(t, s); }
 };
 
 struct RefCorrect {  // x += |r| d32
@@ -1696,10 +1739,30 @@ static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double
         poll(ctx, 2);
         const double thresh = std::max(tol * hs.bnorm, atol);
         if (!(hs.rnorm > thresh) || sweeps >= max_refine) break;
-        launch_vec(ctx, RefScale{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 2);
         wspm_krylov_stats ist{};
-        krylov_solve<float>(ctx, method, A32, dinv32, ctx->ref_r32, ctx->ref_d32, ctx->ref_d32, inner_tol, 0.0,
-                            maxiter, restart, 1, &ist);
+        if (method == WSPM_GMRES) {
+            launch_vec(ctx, RefScale{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 2);
+            krylov_solve<float>(ctx, method, A32, dinv32, ctx->ref_r32, ctx->ref_d32, ctx->ref_d32, inner_tol, 0.0,
+                                maxiter, restart, 1, &ist);
+        } else {
+            // zero initial guess: scaling and solver start in one pass, no SpMV
+            KrylovRun<float> run(ctx, method, A32, dinv32, ctx->ref_r32, ctx->ref_d32, ctx->ref_d32, inner_tol, 0.0,
+                                 maxiter, restart, 1, true);
+            const double* rn = state_field<double>(states_of(ctx, 2), offsetof(KState, rnorm));
+            if (method == WSPM_BICGSTAB)
+                launch_vec(ctx, RefStart<BicgInit<float>>{{run.v, inner_tol, 0.0, maxiter}, ctx->ref_r, ctx->ref_r32,
+                                                          ctx->ref_d32, rn, 0.0}, 1);
+            else
+                launch_vec(ctx, RefStart<CgInit<float>>{{run.v, inner_tol, 0.0, maxiter}, ctx->ref_r, ctx->ref_r32,
+                                                        ctx->ref_d32, rn, 0.0}, 1);
+            run.enqueue(run.first_chunk());
+            poll(ctx, 1);
+            while (run.more()) {
+                run.enqueue(1);
+                poll(ctx, 1);
+            }
+            run.stats(&ist);
+        }
         inner_total += ist.iters;
         arn += ist.arnoldi;
         launch_vec(ctx, RefCorrect{x, ctx->ref_d32, 0.0}, 2);
@@ -2646,9 +2709,16 @@ static void ens_mixed_solve(wspm_ctx* ctx, bool jac) {
     while (true) {
         for (int sw = 0; sw < ENS_SWEEPS; ++sw) {
             ens_compact(ctx, LIST_K);                 // instances still refining (done clear)
-            launch_vec(ctx, RefScaleE{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
+            // the inner solve starts from zero: scaling and solver start in one
+            // pass, without the start's SpMV
             KrylovRun<float> run(ctx, P.method, ctx->A32, jac ? ctx->dinv32 : nullptr, ctx->ref_r32, ctx->ref_d32,
-                                 ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, false);
+                                 ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, true);
+            if (P.method == WSPM_BICGSTAB)
+                launch_vec(ctx, RefStartE<BicgInit<float>>{{run.v, P.inner_tol, 0.0, P.maxiter}, ctx->ref_r,
+                                                           ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
+            else
+                launch_vec(ctx, RefStartE<CgInit<float>>{{run.v, P.inner_tol, 0.0, P.maxiter}, ctx->ref_r,
+                                                         ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
             ens_compact(ctx, LIST_K);
             run.enqueue(std::min(ENS_INNER, P.maxiter));
             ens_compact(ctx, LIST_K, 2);              // the instances refined in this sweep
