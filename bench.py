@@ -86,6 +86,8 @@ def parse():
                     help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
     ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
+    ap.add_argument("--fd-order", type=int, default=4, choices=(2, 4),
+                    help="c4: order of the central-difference stencil (4: four forward runs per direction)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: steps per instance of the plain single-instance path (0 = skip)")
     args = ap.parse_args()
@@ -396,10 +398,15 @@ def run_adjoint(args):
     drng = np.random.default_rng(1)
     for k in range(args.fd_dirs):
         d = drng.normal(size=p0.size) * scale
-        # central differences, eps 1e-4: truncation ~eps^2, and the forward's
-        # solver noise is divided by eps
+        # fourth-order central differences, eps 1e-4: truncation ~eps^4 (the
+        # second-order stencil's eps^2 term was up to 1.6e-6 of the derivative
+        # on these directions); the forward's solver noise is divided by eps
         eps = 1e-4
-        fd = (loss(p0 + eps * d) - loss(p0 - eps * d)) / (2 * eps)
+        if args.fd_order == 2:
+            fd = (loss(p0 + eps * d) - loss(p0 - eps * d)) / (2 * eps)
+        else:
+            fd = (8.0 * (loss(p0 + eps * d) - loss(p0 - eps * d))
+                  - (loss(p0 + 2 * eps * d) - loss(p0 - 2 * eps * d))) / (12 * eps)
         errs.append(abs(g @ d - fd) / max(abs(fd), 1e-300))
         nerrs.append(abs(g @ d - fd) / (np.linalg.norm(g) * np.linalg.norm(d)))
         print(f"[bench] c4 direction {k + 1}/{args.fd_dirs}: g.d {g @ d:.6e} fd {fd:.6e} rel err {errs[-1]:.2e}",
@@ -420,6 +427,7 @@ def run_adjoint(args):
                    "flow_adjoint_bicgstab_iters_per_step": round(float(np.mean(model.adj_iters)), 2),
                    "species_adjoint_bicgstab_iters_per_step": round(float(np.mean(model.adj_tr_iters)), 2),
                    "fd_directions": args.fd_dirs,
+                   "fd_stencil": f"central, order {args.fd_order}, eps 1e-4",
                    "fd_max_rel_err": float(max(errs)) if errs else None,
                    "fd_median_rel_err": float(np.median(errs)) if errs else None,
                    "fd_max_err_over_norms": float(max(nerrs)) if nerrs else None,
