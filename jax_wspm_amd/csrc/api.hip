@@ -644,6 +644,7 @@ struct wspm_ctx {
     int* kact = nullptr;            // [1 + n_tab] entries of instances whose Krylov solve runs
     int* pact = nullptr;            // [1 + n_tab] entries of instances still iterating Picard
     int ens_grid = 0;               // workgroups of an ensemble launch
+    int ens_compact_every = 1;      // Krylov iterations between rebuilds of the active list
     // boundary node lists for edge terms: [0] inlet (top), [1] outlet (bottom)
     struct EdgeList {
         int cnt = 0;
@@ -1151,7 +1152,9 @@ struct KrylovRun {
             } else {
                 gmres_cycle();
             }
-            if (ctx->tab) ens_compact(ctx, LIST_K);   // drop the instances that converged
+            // drop the instances that converged (every ens_compact_every-th
+            // iteration; in between, their entries are walked and skipped)
+            if (ctx->tab && (enq + 1) % ctx->ens_compact_every == 0) ens_compact(ctx, LIST_K);
         }
     }
 
@@ -2627,6 +2630,8 @@ int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
         HIPCHK(hipMemset(ctx->kact, 0, sizeof(int)));
         HIPCHK(hipMemset(ctx->pact, 0, sizeof(int)));
         {
+            const char* ce = std::getenv("WSPM_ENS_COMPACT");
+            ctx->ens_compact_every = std::max(1, ce ? std::atoi(ce) : 1);
             const char* g = std::getenv("WSPM_ENS_GRID");
             const int want = g ? std::atoi(g) : ENS_GRID;
             ctx->ens_grid = std::max(1, std::min(want > 0 ? want : ENS_GRID, ctx->n_tab));
@@ -2795,7 +2800,9 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             if (graphs && run.enq + 2 <= P.maxiter) {
                 // two more Krylov iterations, the Picard error, the counts
                 const int enq0 = run.enq;
-                const hipGraphExec_t ex = step_graph(ctx, GRAPH_ENS_CHUNK, dt, 2, [&] {
+                // the compactions inside the chunk depend on the iteration count
+                const hipGraphExec_t ex = step_graph(ctx, GRAPH_ENS_CHUNK, dt,
+                                                     2 | (enq0 % ctx->ens_compact_every) << 8, [&] {
                     run.enqueue(2);
                     launch_vec(ctx, pop, 0, LIST_P);
                     ens_status_publish(ctx, 0);
