@@ -51,6 +51,11 @@ struct ApiFail {
         }                                                                               \
     } while (0)
 
+bool env_is(const char* name, const char* value) {
+    const char* e = std::getenv(name);
+    return e && std::string(e) == value;
+}
+
 // ---------------------------------------------------------------------------
 // Soil / Picard kernels
 // ---------------------------------------------------------------------------
@@ -590,6 +595,11 @@ struct wspm_ctx {
     int2* ell_inc = nullptr;        // neighbourhood assembly: incidences in ELL layout
     uint32_t* ell_info = nullptr;   // and rl | dg << 4 | nq << 8 per row
     int ell_iw = 0;
+    // finish-free CG / BiCGSTAB: second partial-row region and the iteration
+    // histories of the main (0) and FP32 inner (1) solves
+    double* partials2 = nullptr;
+    KHist* khist[2] = {nullptr, nullptr};
+    int khist_cap[2] = {0, 0};
     bool ell = false;
     bool spmv_ell = false;
     bool ell_faster = true;         // ensembles: the autotuner timed ELL against the best CSR width
@@ -796,8 +806,11 @@ constexpr size_t TIMING_SLOTS = (size_t)1 << 18;
 // stop events as part of the kernel's own dispatch (hipExtLaunchKernelGGL),
 // so the elapsed time is the kernel's execution, as a kernel trace reports it,
 // without the dispatch gaps of separately recorded events.
+// part: partial rows to write (default ctx->partials); finish false: no
+// finish kernel (a folded Krylov kernel's consumer reduces the rows itself).
 template <class T, class Epi>
-void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false, int list = LIST_ALL) {
+void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false, int list = LIST_ALL,
+                 double* part = nullptr, bool finish = true) {
     const bool t = timed && ctx->timing && ctx->timing_fp32 == std::is_same<T, float>::value &&
                    ctx->ev_used / 2 < TIMING_SLOTS;
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
@@ -806,7 +819,8 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         e0 = next_event(ctx);
         e1 = next_event(ctx);
     }
-    const Red red = red_of(ctx, which);
+    Red red = red_of(ctx, which);
+    if (part) red.partials = part;
     const EnsWalk tab = walk_of(ctx, list);
     auto go = [&](auto kernel, dim3 g, auto... args) {
         if (t) hipExtLaunchKernelGGL(kernel, g, dim3(BLOCK), 0, ctx->stream, e0, e1, 0, args...);
@@ -817,7 +831,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         const int eg = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
         if (ctx->ell_w == 7) go(k_spmv_ell<7, T, Epi>, dim3(eg), ctx->n, (const int32_t*)ctx->ell_col, av, epi, red, tflag, tab);
         else go(k_spmv_ell<8, T, Epi>, dim3(eg), ctx->n, (const int32_t*)ctx->ell_col, av, epi, red, tflag, tab);
-        launch_finish(ctx, epi, eg, which);
+        if (finish) launch_finish(ctx, epi, eg, which);
         return;
     }
     const int grid = ctx->tab ? ctx->ens_grid : ctx->spmv_grid;
@@ -832,7 +846,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         case 32: go(k_spmv<32, T, Epi>, g, ctx->n, rp, ci, av, epi, red, tflag, tab); break;
         default: go(k_spmv<64, T, Epi>, g, ctx->n, rp, ci, av, epi, red, tflag, tab); break;
     }
-    launch_finish(ctx, epi, grid, which);
+    if (finish) launch_finish(ctx, epi, grid, which);
 }
 
 // Wait for the next publication (k_publish / k_ens_status with a sequence
@@ -1072,6 +1086,20 @@ struct KrylovRun {
     int nbx = 0;
     int enq = 0;        // iterations (CG/BiCGSTAB) or cycles (GMRES) enqueued
     int pc = 0;         // general preconditioner kind (WSPM_PRECOND_BLOCK_JACOBI..ILU) or 0
+    // CG / BiCGSTAB of one system with Jacobi or none: no finish kernels
+    // (FBicg* / FCg*); WSPM_KRYLOV=unfolded keeps a finish after each reduction
+    bool fold = false;
+    KHist* hist = nullptr;
+
+    // a folded kernel: no finish, partial rows into part
+    template <class Op>
+    void vec_nf(const Op& op, double* part, int rows, int grid) {
+        Red red = red_of(ctx, which);
+        red.partials = part ? part : ctx->partials;
+        hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, rows, op, red,
+                           EnsWalk{nullptr, nullptr, 0});
+        HIPCHK(hipGetLastError());
+    }
 
     // dst = M src for a general preconditioner (FP64 only)
     void apply(const T* src, T* dst, int skipmode) {
@@ -1088,6 +1116,16 @@ struct KrylovRun {
         const int n = ctx->n;
         ds = states_of(ctx, which);
         const bool gpc = pc != 0;
+        fold = !gpc && !ctx->tab && method != WSPM_GMRES && which < 2 && !env_is("WSPM_KRYLOV", "unfolded");
+        if (fold) {
+            if (!ctx->partials2) ctx->partials2 = dalloc<double>(ctx, (size_t)ctx->prow * MAX_DOTS);
+            const int need = std::max(maxiter, 0) + 2;
+            if (ctx->khist_cap[which] < need) {
+                ctx->khist[which] = dalloc<KHist>(ctx, need);
+                ctx->khist_cap[which] = need;
+            }
+            hist = ctx->khist[which];
+        }
         if (gpc) {
             if constexpr (std::is_same<T, double>::value) {
                 pc_ensure(ctx, pc == WSPM_PRECOND_SSOR || pc == WSPM_PRECOND_ILU);
@@ -1131,7 +1169,43 @@ struct KrylovRun {
         }
     }
 
+    // Folded iterations: partial rows alternate between two regions so that
+    // a kernel never overwrites the rows its consumer has not read yet
+    // (q, t -> A; s, x/r -> B; p reads B).
+    void enqueue_folded(int count) {
+        double* PA = ctx->partials;
+        double* PB = ctx->partials2;
+        const int sb = ctx->spmv_ell ? ctx->vec_grid : ctx->spmv_grid;   // SpMV workgroups
+        const int vb = ctx->vec_grid;
+        const int pr = ctx->prow;
+        KState* stw = states_of(ctx, which);
+        int c = 0;
+        for (; c < count && enq < maxiter; ++c, ++enq) {
+            const int k = enq;
+            if (method == WSPM_BICGSTAB) {
+                vec_nf(FBicgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, k, 0.0, 0.0, 0, 0}, nullptr, ctx->n, vb);
+                launch_spmv<T>(ctx, A, FBicgQ<T>{{v}, hist, k}, which, true, LIST_ALL, PA, false);
+                vec_nf(FBicgS<T>{{v, 0.0}, hist, FoldIn{PA, pr, sb}, k}, PB, ctx->n, vb);
+                launch_spmv<T>(ctx, A, FBicgT<T>{{v}, hist, stw, FoldIn{PB, pr, vb}, k, 0}, which, true, LIST_ALL,
+                               PA, false);
+                vec_nf(FBicgXR<T>{{v, 0.0, 0.0, 0}, hist, FoldIn{PA, pr, sb}, k}, PB, ctx->n, vb);
+            } else {
+                launch_spmv<T>(ctx, A, FCgAp<T>{{v}, hist, k}, which, true, LIST_ALL, PA, false);
+                vec_nf(FCgXR<T>{{v, 0.0}, hist, FoldIn{PA, pr, sb}, k}, PB, ctx->n, vb);
+                vec_nf(FCgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, k, 0.0, 0}, nullptr, ctx->n, vb);
+            }
+        }
+        // BiCGSTAB: the loop test of the next iteration, run alone, leaves the
+        // state for the host (CG's last kernel already did)
+        if (method == WSPM_BICGSTAB && c > 0)
+            vec_nf(FBicgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, enq, 0.0, 0.0, 0, 0}, nullptr, 0, 1);
+    }
+
     void enqueue(int count) {
+        if (fold) {
+            enqueue_folded(count);
+            return;
+        }
         for (int c = 0; c < count && enq < maxiter; ++c, ++enq) {
             if (method == WSPM_BICGSTAB) {
                 launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which, LIST_K);

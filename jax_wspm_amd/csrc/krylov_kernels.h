@@ -14,6 +14,16 @@
 
 namespace wspm {
 
+// A functor may decide in load() that its launch does no row work (a folded
+// Krylov kernel whose solve has ended, or whose step is cut short): active()
+// false.  Functors without active() always work.
+template <class Op>
+__device__ inline auto active_impl(const Op& op, int) -> decltype(op.active()) { return op.active(); }
+template <class Op>
+__device__ inline bool active_impl(const Op&, long) { return true; }
+template <class Op>
+__device__ inline bool active_of(const Op& op) { return active_impl(op, 0); }
+
 template <class T>
 struct KVec {
     int n;
@@ -96,14 +106,16 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
     const int g = threadIdx.x / LANES;
     if (ew.tab == nullptr) {
         if (epi.skip(red.st)) return;
-        if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
         epi.load(red.st);
-        const int chunk = (n + gridDim.x - 1) / gridDim.x;
-        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
-        spmv_rows<LANES, T>(rb + g, min(n, rb + chunk), GPB, rp, ci, av, epi, acc);
+        if (active_of(epi)) {
+            if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
+            const int chunk = (n + gridDim.x - 1) / gridDim.x;
+            const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
+            spmv_rows<LANES, T>(rb + g, min(n, rb + chunk), GPB, rp, ci, av, epi, acc);
+        }
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
@@ -158,15 +170,17 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __rest
     constexpr int NA = ND > 0 ? ND : 1;
     if (ew.tab == nullptr) {
         if (epi.skip(red.st)) return;
-        if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
         epi.load(red.st);
-        // contiguous chunk per workgroup, a multiple of BLOCK rows
-        const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
-        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
-        ell_rows<W, T>(rb, min(n, rb + chunk), n, ci, av, epi, acc);
+        if (active_of(epi)) {
+            if (tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tflag = n;
+            // contiguous chunk per workgroup, a multiple of BLOCK rows
+            const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+            const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
+            ell_rows<W, T>(rb, min(n, rb + chunk), n, ci, av, epi, acc);
+        }
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
@@ -198,10 +212,12 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
         op.load(red.st);
-        // contiguous chunk per workgroup, a multiple of BLOCK rows
-        const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
-        const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk, re = min(n, rb + chunk);
-        for (int i = rb + threadIdx.x; i < re; i += BLOCK) op.row(i, acc);
+        if (active_of(op)) {
+            // contiguous chunk per workgroup, a multiple of BLOCK rows
+            const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+            const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk, re = min(n, rb + chunk);
+            for (int i = rb + threadIdx.x; i < re; i += BLOCK) op.row(i, acc);
+        }
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
@@ -389,6 +405,190 @@ struct BicgXR {  // x, r update; |r|^2, r_hat.r; loop control
     }
 };
 
+// ------------------- CG / BiCGSTAB without finish kernels -------------------
+// One system, Jacobi or no preconditioner.  Instead of a one-workgroup
+// finish kernel after each reduction, every workgroup of the next kernel sums
+// the producer's partial rows itself (reduce_all: the same rows in the same
+// order as k_finish, so the scalars are bitwise those of the finish), and the
+// scalars of iteration k live in an iteration history H[k] (written by
+// workgroup 0 of the kernel that derives them, read by later kernels only).
+// A BiCGSTAB iteration is then five kernels instead of nine, CG three instead
+// of five.  The solver state (KState) the host reads is written by workgroup
+// 0 of the kernel that decides whether the solve goes on.
+struct KHist {
+    double rho, alpha, omega, rs, gamma;
+    int32_t done, early, code, pad;
+};
+
+// The producing kernel's partial rows: dot j of workgroup b at p[j * prow + b].
+struct FoldIn {
+    const double* p;
+    int prow, nblk;
+};
+
+template <int ND>
+__device__ inline void reduce_all(const FoldIn& in, double (&tot)[ND]) {
+    double acc[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+    for (int b = threadIdx.x; b < in.nblk; b += BLOCK) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] += in.p[(size_t)j * in.prow + b];
+    }
+    __shared__ double sh[ND * NWAVE];
+    __shared__ double bc[ND];
+    block_sum<ND>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) bc[j] = acc[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ND; ++j) tot[j] = bc[j];
+}
+
+__device__ inline bool lead() { return blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0; }
+
+// Iteration k's first kernel: decide whether the solve goes on (k = 0: from
+// the start's state; else from the previous iteration's r-update sums), then
+// p = r + beta (p - omega q), p_hat = M p.  With n = 0 (one workgroup) it is
+// the end-of-chunk finish that leaves the state for the host.
+template <class T>
+struct FBicgP {
+    static constexpr int ND = 0;
+    KVec<T> v;
+    KHist* H;
+    KState* stw;
+    FoldIn in;          // BicgXR of iteration k - 1
+    int k;
+    double beta, omega;
+    int first, go;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState* s) {
+        first = k == 0;
+        if (k == 0) {
+            go = !s->done;
+            if (lead()) {
+                H[0].rho = s->rho_next;
+                H[0].rs = s->rs;
+                H[0].alpha = s->alpha;
+                H[0].omega = s->omega;
+                H[0].done = s->done;
+            }
+            beta = 0.0;
+            omega = s->omega;
+            return;
+        }
+        if (H[k - 1].done) {
+            go = 0;
+            if (lead()) H[k].done = 1;
+            return;
+        }
+        double t[2];
+        reduce_all<2>(in, t);
+        const KHist& hp = H[k - 1];
+        const int kk = hp.code;
+        const double rs = t[0], rho = t[1];
+        const int done = !(rs > s->atol2 && kk < s->maxiter && kk >= 0);
+        go = !done;
+        beta = rho / hp.rho * hp.alpha / hp.omega;
+        omega = hp.omega;
+        if (lead()) {
+            H[k].rho = rho;
+            H[k].rs = rs;
+            H[k].done = done;
+            stw->k = kk;
+            stw->code = kk;
+            stw->steps = k;
+            stw->rho = hp.rho;
+            stw->rho_next = rho;
+            stw->rs = rs;
+            stw->alpha = hp.alpha;
+            stw->omega = hp.omega;
+            stw->done = done;
+        }
+    }
+    __device__ bool active() const { return go; }
+    __device__ void row(int i, double*) const {
+        const double pi = first ? (double)v.r[i]
+                                : (double)v.r[i] + beta * ((double)v.p[i] - omega * (double)v.q[i]);
+        v.p[i] = (T)pi;
+        if (v.dinv) v.ph[i] = (T)((double)v.dinv[i] * (double)(T)pi);
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+template <class T>
+struct FBicgQ : BicgQ<T> {   // q = A p_hat; r_hat.q
+    const KHist* H;
+    int k;
+    __device__ bool skip(const KState*) const { return H[k].done; }
+};
+
+template <class T>
+struct FBicgS : BicgS<T> {   // alpha from q's sums; s = r - alpha q; s_hat; |s|^2
+    KHist* H;
+    FoldIn in;
+    int k;
+    __device__ bool skip(const KState*) const { return H[k].done; }
+    __device__ void load(const KState*) {
+        double t[1];
+        reduce_all<1>(in, t);
+        this->alpha = H[k].rho / t[0];
+        if (lead()) H[k].alpha = this->alpha;
+    }
+};
+
+template <class T>
+struct FBicgT : BicgT<T> {   // exit test from |s|^2; t = A s_hat; t.s, t.t
+    KHist* H;
+    KState* stw;
+    FoldIn in;
+    int k;
+    int go;
+    __device__ bool skip(const KState*) const { return H[k].done; }
+    __device__ void load(const KState* s) {
+        double t[1];
+        reduce_all<1>(in, t);
+        const int early = t[0] < s->atol2;
+        go = !early;
+        if (lead()) {
+            H[k].early = early;
+            stw->ss = t[0];
+            stw->exit_early = early;
+        }
+    }
+    __device__ bool active() const { return go; }
+};
+
+template <class T>
+struct FBicgXR : BicgXR<T> {  // omega from t's sums; x, r update; |r|^2, r_hat.r
+    KHist* H;
+    FoldIn in;
+    int k;
+    __device__ bool skip(const KState*) const { return H[k].done; }
+    __device__ void load(const KState*) {
+        const KHist& h = H[k];
+        this->alpha = h.alpha;
+        this->early = h.early;
+        // early exit: omega keeps its value (t was not formed)
+        double om = k > 0 ? H[k - 1].omega : 1.0;
+        if (!h.early) {
+            double t[2];
+            reduce_all<2>(in, t);
+            om = t[0] / t[1];
+        }
+        this->omega = om;
+        if (lead()) {
+            int kk = k + 1;
+            if ((!h.early && om == 0.0) || h.alpha == 0.0) kk = -11;
+            if (h.rho == 0.0) kk = -10;
+            H[k].omega = om;
+            H[k].code = kk;
+        }
+    }
+};
+
 // ----------------------------------- CG ------------------------------------
 template <class T>
 struct CgInit {  // r = b - A x0; z = M r; p = z
@@ -477,6 +677,82 @@ struct CgP {  // p = z + beta p (the new p of JAX's body, computed after the tes
     __device__ void load(const KState* s) { beta = s->beta; }
     __device__ void row(int i, double*) const {
         const double zi = (v.dinv || v.gpc) ? (double)v.z[i] : (double)v.r[i];
+        v.p[i] = (T)(zi + beta * (double)v.p[i]);
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+// CG: the first two kernels of iteration k go on while H[k] (k = 0: the
+// start's state) says so; CgP's load decides for iteration k + 1.
+__device__ inline bool cg_done(const KHist* H, int k, const KState* s) { return k == 0 ? s->done : H[k].done; }
+
+template <class T>
+struct FCgAp : CgAp<T> {      // q = A p; p.q
+    const KHist* H;
+    int k;
+    __device__ bool skip(const KState* s) const { return cg_done(H, k, s); }
+};
+
+template <class T>
+struct FCgXR : CgXR<T> {      // alpha from p.q; x += alpha p; r -= alpha q; z = M r; r.z, |r|^2
+    KHist* H;
+    FoldIn in;
+    int k;
+    __device__ bool skip(const KState* s) const { return cg_done(H, k, s); }
+    __device__ void load(const KState* s) {
+        double t[1];
+        reduce_all<1>(in, t);
+        const double gamma = k == 0 ? s->gamma : H[k].gamma;
+        this->alpha = gamma / t[0];
+        if (lead()) {
+            H[k].alpha = this->alpha;
+            if (k == 0) H[0].gamma = gamma;
+        }
+    }
+};
+
+template <class T>
+struct FCgP {                 // beta, |r|^2 and the loop test from CgXR's sums; p = z + beta p
+    static constexpr int ND = 0;
+    KVec<T> v;
+    KHist* H;
+    KState* stw;
+    FoldIn in;
+    int k;
+    double beta;
+    int go;
+    __device__ bool skip(const KState*) const { return false; }
+    __device__ void load(const KState* s) {
+        if (cg_done(H, k, s)) {   // ended earlier: carry the flag to iteration k + 1
+            go = 0;
+            if (lead()) H[k + 1].done = 1;
+            return;
+        }
+        double t[2];
+        reduce_all<2>(in, t);
+        const double gamma = H[k].gamma;
+        const double g1 = t[0];
+        const double rs = v.dinv ? t[1] : t[0];
+        beta = g1 / gamma;
+        const int kk = k + 1;
+        const int done = !(rs > s->atol2 && kk < s->maxiter);
+        go = !done;
+        if (lead()) {
+            H[k + 1].gamma = g1;
+            H[k + 1].rs = rs;
+            H[k + 1].done = done;
+            stw->beta = beta;
+            stw->gamma = g1;
+            stw->alpha = H[k].alpha;
+            stw->rs = rs;
+            stw->k = kk;
+            stw->steps = kk;
+            stw->done = done;
+        }
+    }
+    __device__ bool active() const { return go; }
+    __device__ void row(int i, double*) const {
+        const double zi = v.dinv ? (double)v.z[i] : (double)v.r[i];
         v.p[i] = (T)(zi + beta * (double)v.p[i]);
     }
     __device__ void fin(const double*, KState*) const {}

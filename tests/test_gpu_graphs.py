@@ -1,4 +1,5 @@
-"""HIP-graph replay of whole Picard iterations and the spin-wait read-back.
+"""Launch modes of the time step: HIP-graph replay of whole Picard
+iterations, the spin-wait read-back and the finish-free Krylov iterations.
 
 The time step replays each Picard iteration (assembly + solver start, the
 expected Krylov iterations, the Picard error, the state publication) from a
@@ -18,16 +19,17 @@ pytestmark = pytest.mark.gpu
 MODES = [("1", "spin"), ("0", "spin"), ("0", "sync")]  # (WSPM_GRAPHS, WSPM_POLL)
 
 
-def _single(monkeypatch, graphs, poll, solver, precond, dts):
+def _single(monkeypatch, graphs, poll, solver, precond, dts, krylov="folded", precision="fp64"):
     from jax_wspm_amd.device import Context
     from jax_wspm_amd.problems import baseline_vg_config, from_config
     monkeypatch.setenv("WSPM_GRAPHS", graphs)
     monkeypatch.setenv("WSPM_POLL", poll)
+    monkeypatch.setenv("WSPM_KRYLOV", krylov)
     pb = from_config(baseline_vg_config(40, 2, solver=solver, precond=precond, nsteps=1, dt=1e-4,
                                         ks_field={"sigma": 0.5, "length": 0.05, "seed": 3}))
     ctx = Context(pb.points, pb.elements)
     ctx.set_dirichlet(pb.bc_lists)
-    ctx.richards_init("vg", pb.phi, solver, precond, pb.h0, Ks_elem=pb.Ks_elem)
+    ctx.richards_init("vg", pb.phi, solver, precond, pb.h0, Ks_elem=pb.Ks_elem, precision=precision)
     hs, its = [], []
     for dt in dts:
         st = ctx.richards_step(dt)
@@ -49,6 +51,23 @@ def test_single_context_modes_bitwise(monkeypatch, solver, precond):
         assert it == ref_it, (graphs, poll)
         for k in range(len(dts)):
             assert np.array_equal(h[k], ref_h[k]), (graphs, poll, k)
+
+
+@pytest.mark.parametrize("solver,precond,precision", [("bicgstab", "jacobi", "fp64"), ("cg", "jacobi", "fp64"),
+                                                     ("bicgstab", "none", "fp64"), ("cg", "none", "fp64"),
+                                                     ("bicgstab", "jacobi", "mixed"), ("cg", "jacobi", "mixed")])
+def test_finish_free_krylov_bitwise(monkeypatch, solver, precond, precision):
+    """CG / BiCGSTAB iterations without finish kernels (each kernel re-reduces
+    its producer's partial rows; scalars in an iteration history) against one
+    finish kernel per reduction (WSPM_KRYLOV=unfolded): the same sums in the
+    same order, so bitwise the same trajectories and iteration counts."""
+    dts = [1e-4] * 4 + [2e-4, 2e-4]
+    print(f"[case] finish-free {solver}+{precond} {precision}", flush=True)
+    ref_h, ref_it = _single(monkeypatch, "0", "spin", solver, precond, dts, "unfolded", precision)
+    h, it = _single(monkeypatch, "0", "spin", solver, precond, dts, "folded", precision)
+    assert it == ref_it
+    for k in range(len(dts)):
+        assert np.array_equal(h[k], ref_h[k]), k
 
 
 def _ensemble(monkeypatch, graphs, poll, solver, steps):
