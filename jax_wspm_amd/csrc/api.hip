@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1090,6 +1091,10 @@ struct KrylovRun {
     // (FBicg* / FCg*); WSPM_KRYLOV=unfolded keeps a finish after each reduction
     bool fold = false;
     KHist* hist = nullptr;
+    // folded BiCGSTAB started by the assembly with its finish deferred: the
+    // start's partial rows, reduced by the first iteration's p kernel
+    FoldIn start_in{nullptr, 0, 0};
+    double tol0 = 0.0, atol0 = 0.0;
 
     // a folded kernel: no finish, partial rows into part
     template <class Op>
@@ -1112,7 +1117,7 @@ struct KrylovRun {
     KrylovRun(wspm_ctx* c, int m, const T* A_, const T* dinv_, const T* b, const T* x0, T* x, double tol,
               double atol, int maxit, int restart, int wh, bool started, int pc_kind = 0)
         : ctx(c), method(m), A(A_), dinv(dinv_), which(wh), maxiter(maxit),
-          pc(pc_kind >= WSPM_PRECOND_BLOCK_JACOBI ? pc_kind : 0) {
+          pc(pc_kind >= WSPM_PRECOND_BLOCK_JACOBI ? pc_kind : 0), tol0(tol), atol0(atol) {
         const int n = ctx->n;
         ds = states_of(ctx, which);
         const bool gpc = pc != 0;
@@ -1183,7 +1188,10 @@ struct KrylovRun {
         for (; c < count && enq < maxiter; ++c, ++enq) {
             const int k = enq;
             if (method == WSPM_BICGSTAB) {
-                vec_nf(FBicgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, k, 0.0, 0.0, 0, 0}, nullptr, ctx->n, vb);
+                FBicgP<T> P{v, hist, stw, FoldIn{PB, pr, vb}, k, 0.0, 0.0, 0, 0, FoldIn{nullptr, 0, 0}, tol0, atol0,
+                            maxiter};
+                if (k == 0) P.start = start_in;
+                vec_nf(P, nullptr, ctx->n, vb);
                 launch_spmv<T>(ctx, A, FBicgQ<T>{{v}, hist, k}, which, true, LIST_ALL, PA, false);
                 vec_nf(FBicgS<T>{{v, 0.0}, hist, FoldIn{PA, pr, sb}, k}, PB, ctx->n, vb);
                 launch_spmv<T>(ctx, A, FBicgT<T>{{v}, hist, stw, FoldIn{PB, pr, vb}, k, 0}, which, true, LIST_ALL,
@@ -1198,7 +1206,9 @@ struct KrylovRun {
         // BiCGSTAB: the loop test of the next iteration, run alone, leaves the
         // state for the host (CG's last kernel already did)
         if (method == WSPM_BICGSTAB && c > 0)
-            vec_nf(FBicgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, enq, 0.0, 0.0, 0, 0}, nullptr, 0, 1);
+            vec_nf(FBicgP<T>{v, hist, stw, FoldIn{PB, pr, vb}, enq, 0.0, 0.0, 0, 0, FoldIn{nullptr, 0, 0}, tol0, atol0,
+                             maxiter},
+                   nullptr, 0, 1);
     }
 
     void enqueue(int count) {
@@ -1565,7 +1575,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
                         const double* K, const double* theta_m, const double* theta_n, double dt,
                         double* A, double* b, double* dinv, float* A32, float* dinv32,
                         int mode = ASM_INIT_NONE, double* x = nullptr, double tol = 0.0, double atol = 0.0,
-                        int maxiter = 0, int restart = 0, bool ell = false) {
+                        int maxiter = 0, int restart = 0, bool ell = false, int* start_grid = nullptr) {
     AsmArgs a{};
     a.n = ctx->n;
     a.win_cap = ctx->asm_win;
@@ -1611,7 +1621,10 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     // the ones its functor uses (column sums are independent)
     switch (mode) {
         case ASM_INIT_BICG:
-            launch_finish<BicgInit<double>, true>(ctx, BicgInit<double>{v, tol, atol, maxiter}, grid, 0);
+            // start_grid: the finish-free BiCGSTAB reduces the start's rows in
+            // its first kernel
+            if (start_grid && !ctx->tab) *start_grid = grid;
+            else launch_finish<BicgInit<double>, true>(ctx, BicgInit<double>{v, tol, atol, maxiter}, grid, 0);
             break;
         case ASM_INIT_CG:
             launch_finish<CgInit<double>, true>(ctx, CgInit<double>{v, tol, atol, maxiter}, grid, 0);
@@ -2079,10 +2092,13 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                                P.picard_mode == WSPM_PICARD_REL};
             const int K = std::min(run.first_chunk(), P.maxiter);
+            const bool defer = run.fold && P.method == WSPM_BICGSTAB;
             const hipGraphExec_t ex = step_graph(ctx, GRAPH_PICARD, dt, K, [&] {
+                int sg = 0;
                 do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                             jac ? ctx->dinv : nullptr, nullptr, nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter,
-                            P.restart, ctx->ell);
+                            P.restart, ctx->ell, defer ? &sg : nullptr);
+                if (defer) run.start_in = FoldIn{ctx->partials, ctx->prow, sg};
                 run.enqueue(K);
                 picard_enqueue(ctx, pop);
             });
@@ -2108,9 +2124,22 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             ensure_mixed(ctx);
             HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         }
+        // the FP64 Krylov solve (constructed here so that a finish-free
+        // BiCGSTAB can take over the assembly start's finish)
+        std::unique_ptr<KrylovRun<double>> krun;
+        int start_grid = 0;
+        const bool plain = !direct && !mixed;
+        // CG / BiCGSTAB started by the assembly: the constructor enqueues nothing
+        // (GMRES's scales its first basis vector, so it comes after the assembly)
+        if (plain && !own_start && P.method != WSPM_GMRES) {
+            krun.reset(new KrylovRun<double>(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA,
+                                             ctx->hB, P.tol, P.atol, P.maxiter, P.restart, 0, true, P.precond));
+        }
+        const bool defer = krun && krun->fold && P.method == WSPM_BICGSTAB;
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                     (jac && !direct) ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
-                    mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, ctx->ell);
+                    mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, ctx->ell, defer ? &start_grid : nullptr);
+        if (defer) krun->start_in = FoldIn{ctx->partials, ctx->prow, start_grid};
         if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
             launch_edge_list(ctx, 0, EDGE_NEUMANN, ctx->rich_flux, 0.0, nullptr, 0, nullptr, nullptr, ctx->bv,
                              nullptr);
@@ -2132,8 +2161,11 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             // Enqueue the expected number of Krylov iterations and the Picard
             // error behind them, then read both back with one synchronisation;
             // if the solve needed more iterations, finish it and redo the error.
-            KrylovRun<double> run(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA, ctx->hB,
-                                  P.tol, P.atol, P.maxiter, P.restart, 0, !own_start, P.precond);
+            if (!krun)   // after the assembly and the edge terms
+                krun.reset(new KrylovRun<double>(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->bv, ctx->hA,
+                                                 ctx->hB, P.tol, P.atol, P.maxiter, P.restart, 0, !own_start,
+                                                 P.precond));
+            KrylovRun<double>& run = *krun;
             run.enqueue(run.first_chunk());
             picard_poll(ctx, pop);
             if (run.more()) {

@@ -453,19 +453,50 @@ __device__ inline bool lead() { return blockIdx.x == 0 && blockIdx.y == 0 && thr
 // the start's state; else from the previous iteration's r-update sums), then
 // p = r + beta (p - omega q), p_hat = M p.  With n = 0 (one workgroup) it is
 // the end-of-chunk finish that leaves the state for the host.
+// start.p != nullptr (k = 0): the solve was started by the assembly epilogue,
+// whose partial rows (r0.r0, b.b) this kernel reduces in place of that start's
+// finish kernel, setting the state as BicgInit::fin does.
 template <class T>
 struct FBicgP {
     static constexpr int ND = 0;
     KVec<T> v;
     KHist* H;
     KState* stw;
-    FoldIn in;          // BicgXR of iteration k - 1
+    FoldIn in;          // BicgXR of iteration k - 1 (or the assembly's start)
     int k;
     double beta, omega;
     int first, go;
+    FoldIn start;
+    double tol, atol;
+    int maxiter;
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState* s) {
         first = k == 0;
+        if (k == 0 && start.p != nullptr) {
+            double t[2];
+            reduce_all<2>(start, t);
+            const double atol2 = fmax(tol * tol * t[1], atol * atol);
+            const int done = !(t[0] > atol2 && 0 < maxiter);
+            go = !done;
+            beta = 0.0;
+            omega = 1.0;
+            if (lead()) {
+                stw->tol = tol; stw->atol = atol; stw->maxiter = maxiter;
+                stw->bs = t[1];
+                stw->atol2 = atol2;
+                stw->rs = t[0];
+                stw->rho_next = t[0];
+                stw->rho = stw->alpha = stw->omega = 1.0;
+                stw->k = 0; stw->steps = 0; stw->code = 0; stw->exit_early = 0; stw->nan_flag = 0;
+                stw->done = done;
+                H[0].rho = t[0];
+                H[0].rs = t[0];
+                H[0].alpha = 1.0;
+                H[0].omega = 1.0;
+                H[0].done = done;
+            }
+            return;
+        }
         if (k == 0) {
             go = !s->done;
             if (lead()) {
