@@ -85,6 +85,10 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--skip-errors", action="store_true",
+                    help="c3-mixed: skip the untimed error runs against the FP64 ensemble")
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="c3-mixed: relative tolerance of the FP32 inner solves (each refinement sweep)")
     ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
     ap.add_argument("--fd-order", type=int, default=4, choices=(2, 4),
                     help="c4: order of the central-difference stencil (4: four forward runs per direction)")
@@ -192,7 +196,7 @@ def run_ensemble(args):
     insts = vg_ensemble(256, seed=0)
     mixed = precision == "mixed"
     ens = Ensemble(insts)
-    ens.richards_init("vg", solver, "jacobi", precision=precision)
+    ens.richards_init("vg", solver, "jacobi", precision=precision, inner_tol=args.inner_tol)
     print(f"[bench] {args.workload}: instances={ens.n_inst} n={ens.n} E={ens.n_elems} nnz={ens.nnz} lanes={ens.lanes}",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -222,6 +226,7 @@ def run_ensemble(args):
         "data": "synthetic (structured meshes, seeded per-instance soil parameters generated in-process)",
         "config": {"workload": desc, "instances": ens.n_inst, "nodes": ens.n, "elements": ens.n_elems,
                    "nnz": ens.nnz, "dt": dt, "solver": f"{solver}+jacobi", "precision": precision,
+                   **({"inner_tol": args.inner_tol} if mixed else {}),
                    "value_counts": "instance time steps (each timed step advances every instance once)",
                    "ensemble_steps_per_s": round(args.steps / el, 3),
                    "max_picard_iters_per_step": round(picard / args.steps, 3),
@@ -232,7 +237,7 @@ def run_ensemble(args):
         "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8,
                              fmt=ens.spmv_format),
     }
-    if mixed:
+    if mixed and not args.skip_errors:
         # error against the FP64 ensemble: both forms rerun untimed for the
         # same number of steps with per-instance Picard counts recorded, plus an
         # FP64 run with the Krylov tolerance at 1e-13 ("tight").  Both forms
@@ -244,7 +249,7 @@ def run_ensemble(args):
         res = {}
         for prec, ktol in (("mixed", 1e-10), ("fp64", 1e-10), ("fp64", 1e-13), ("mixed", 1e-12), ("fp64", 1e-12)):
             e2 = Ensemble(insts)
-            e2.richards_init("vg", solver, "jacobi", precision=prec, tol=ktol)
+            e2.richards_init("vg", solver, "jacobi", precision=prec, tol=ktol, inner_tol=args.inner_tol)
             its = [e2.step(dt, per_instance=True)["picard_iters_i"].copy() for _ in range(args.warmup + args.steps)]
             h2, th2, _ = e2.state()
             res[(prec, ktol)] = (np.array(its), e2.split(h2), e2.split(th2))
