@@ -85,6 +85,9 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--slow-group", type=lambda v: [int(k) for k in v.split(",")], default=[8],
+                    help="c3: instances (the slowest Picard loops of a probe) stepped in a second context and "
+                         "stream; a,b: the a slowest and the next b in two more; 0 = one context")
     ap.add_argument("--skip-errors", action="store_true",
                     help="c3-mixed: skip the untimed error runs against the FP64 ensemble")
     ap.add_argument("--inner-tol", type=float, default=1e-4,
@@ -184,18 +187,33 @@ def timing_note(steps, el):
 
 
 def run_ensemble(args):
-    """c3: the whole ensemble in one context; the plain single-instance path and
-    the oracle are timed on the same instances for comparison."""
+    """c3: the whole ensemble (one context, or two on separate streams with the
+    slowest instances in the second, --slow-group); the plain single-instance
+    path and the oracle are timed on the same instances for comparison."""
     import numpy as np
     import torch
 
     from jax_wspm_amd.device import Context
-    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    from jax_wspm_amd.ensemble import Ensemble, EnsembleGroups, vg_ensemble
 
     desc, _, _, solver, precision, _, dt = WORKLOADS[args.workload]
     insts = vg_ensemble(256, seed=0)
     mixed = precision == "mixed"
-    ens = Ensemble(insts)
+    groups = None
+    if sum(args.slow_group) > 0:
+        # the instances with the most Picard iterations over a few probe steps
+        # (untimed, from the initial state) get a context and stream of their own
+        probe = Ensemble(insts)
+        probe.richards_init("vg", solver, "jacobi", precision=precision, inner_tol=args.inner_tol)
+        counts = np.zeros(len(insts))
+        for _ in range(max(args.warmup, 1)):
+            counts += probe.step(dt, per_instance=True)["picard_iters_i"]
+        probe.close()
+        del probe
+        groups = EnsembleGroups.split_slowest(counts, args.slow_group)
+        ens = EnsembleGroups(insts, groups)
+    else:
+        ens = Ensemble(insts)
     ens.richards_init("vg", solver, "jacobi", precision=precision, inner_tol=args.inner_tol)
     print(f"[bench] {args.workload}: instances={ens.n_inst} n={ens.n} E={ens.n_elems} nnz={ens.nnz} lanes={ens.lanes}",
           file=sys.stderr, flush=True)
@@ -233,7 +251,12 @@ def run_ensemble(args):
                    "krylov_iters_per_step_summed": round(kry / args.steps, 1),
                    "all_converged": bool(conv), "spmv_format": ens.spmv_format,
                    "launch": launch_note(),
-                   "timing_region": timing_note(args.steps, el_ev)},
+                   "timing_region": timing_note(args.steps, el_ev),
+                   "contexts": ([len(g) for g in groups] if groups else [len(insts)]),
+                   "grouping": (f"instances ranked by Picard iterations over {max(args.warmup, 1)} probe "
+                                f"steps; the slowest {'/'.join(map(str, args.slow_group))} in ensemble contexts of "
+                                "their own, each on its own stream stepped by its own host thread"
+                                if groups else "one context")},
         "roofline": roofline(ens.n, ens.nnz, launches, spmv_ms, rows, value_bytes=4 if mixed else 8,
                              fmt=ens.spmv_format),
     }

@@ -195,3 +195,117 @@ class Ensemble:
         rows = C.c_int64()
         self._ck(self.lib.wspm_timing_read(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
         return int(n.value), float(ms.value), int(rows.value)
+
+
+class EnsembleGroups:
+    """Instances split over several ensemble contexts, each on its own HIP
+    stream and stepped by its own host thread (the library's calls release the
+    GIL), so that the kernels of different groups run concurrently.
+
+    Instances are independent, so every instance's results are bitwise those
+    of any other grouping.  The point of a split is the shape of a time step:
+    the few instances whose Picard loops take the most iterations (stiff, dry
+    soils) keep a step going for many short, latency-bound rounds after the
+    others have converged.  In a group of their own (``split_slowest``) those
+    rounds overlap the other instances' work instead of following it.
+    """
+
+    def __init__(self, instances: Sequence[Instance], groups: Sequence[Sequence[int]]):
+        from concurrent.futures import ThreadPoolExecutor
+        self.instances = list(instances)
+        self.groups = [[int(i) for i in g] for g in groups if len(g)]
+        seen = sorted(i for g in self.groups for i in g)
+        if seen != list(range(len(self.instances))):
+            raise ValueError("groups must hold every instance exactly once")
+        self.n_inst = len(self.instances)
+        self.streams = [torch.cuda.Stream() for _ in self.groups]
+        self.members: List[Ensemble] = []
+        for g, s in zip(self.groups, self.streams):
+            with torch.cuda.stream(s):
+                self.members.append(Ensemble([self.instances[i] for i in g]))
+        self.n = sum(m.n for m in self.members)
+        self.n_elems = sum(m.n_elems for m in self.members)
+        self.nnz = sum(m.nnz for m in self.members)
+        self.lanes = self.members[0].lanes
+        self.device = torch.device("cuda")
+        self._pool = ThreadPoolExecutor(max_workers=len(self.members))
+
+    @staticmethod
+    def split_slowest(picard_iters, n_slow) -> List[List[int]]:
+        """Groups by Picard iterations (e.g. summed over a few probe steps):
+        n_slow an int, the n_slow slowest instances and the rest; a sequence
+        (a, b, ...), the a slowest, the next b, ..., and the rest.  The rest
+        comes first."""
+        sizes = [int(n_slow)] if np.isscalar(n_slow) else [int(k) for k in n_slow]
+        order = [int(i) for i in np.argsort(-np.asarray(picard_iters), kind="stable")]
+        tiers, pos = [], 0
+        for k in sizes:
+            tiers.append(sorted(order[pos:pos + k]))
+            pos += k
+        rest = sorted(order[pos:])
+        return [g for g in [rest] + tiers[::-1] if g]
+
+    def _each(self, fn):
+        def call(k):
+            with torch.cuda.stream(self.streams[k]):
+                return fn(self.members[k])
+        futures = [self._pool.submit(call, k) for k in range(len(self.members))]
+        return [f.result() for f in futures]
+
+    def richards_init(self, *args, **kw):
+        self._each(lambda m: m.richards_init(*args, **kw))
+
+    def step(self, dt: float, per_instance: bool = False):
+        res = self._each(lambda m: m.step(dt, per_instance))
+        errs = [r["err"] for r in res]
+        out = {"picard_iters": max(r["picard_iters"] for r in res),
+               "krylov_iters": sum(r["krylov_iters"] for r in res),
+               "converged": all(r["converged"] for r in res),
+               "err": float("nan") if any(np.isnan(errs)) else max(errs)}
+        if per_instance:
+            for key in ("picard_iters_i", "krylov_iters_i", "converged_i", "err_i"):
+                arr = np.zeros(self.n_inst, dtype=res[0][key].dtype)
+                for g, r in zip(self.groups, res):
+                    arr[g] = r[key]
+                out[key] = arr
+        return out
+
+    def state(self):
+        """(h, theta, theta_n) concatenated in the original instance order."""
+        parts = self._each(lambda m: [m.split(x) for x in m.state()])
+        out = []
+        for q in range(3):
+            pieces = [None] * self.n_inst
+            for g, p in zip(self.groups, parts):
+                for j, i in enumerate(g):
+                    pieces[i] = p[q][j]
+            out.append(torch.as_tensor(np.concatenate(pieces), device=self.device))
+        return tuple(out)
+
+    def split(self, x) -> List[np.ndarray]:
+        x = x.cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+        off = np.concatenate([[0], np.cumsum([inst.points.shape[0] for inst in self.instances])])
+        return [x[off[i]:off[i + 1]] for i in range(self.n_inst)]
+
+    @property
+    def spmv_format(self) -> str:
+        fmts = {m.spmv_format for m in self.members}
+        return fmts.pop() if len(fmts) == 1 else "+".join(sorted(fmts))
+
+    def timing(self, on: bool, fp32: bool = False):
+        self._each(lambda m: m.timing(on, fp32))
+
+    def timing_read(self):
+        res = self._each(lambda m: m.timing_read())
+        return sum(r[0] for r in res), sum(r[1] for r in res), sum(r[2] for r in res)
+
+    def close(self):
+        for m in self.members:
+            m.close()
+        self._pool.shutdown(wait=True)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -139,3 +139,28 @@ def test_mixed_ensemble_within_1e8_of_fp64(solver):
             assert rel_l2(h32[k][i], h64[k][i]) <= 1e-8, (i, k, rel_l2(h32[k][i], h64[k][i]))
             assert rel_l2(th32[k][i], th64[k][i]) <= 1e-8, (i, k)
             assert rel_l2(h32[k][i], o["pressure_head"][k]) <= 1e-8, (i, k)
+
+
+def test_grouped_ensemble_equals_single_context():
+    """EnsembleGroups (instances split over ensemble contexts on separate
+    streams, stepped concurrently by host threads) against one context:
+    every instance's h, theta and per-step Picard / Krylov counts bitwise
+    equal."""
+    from jax_wspm_amd.ensemble import Ensemble, EnsembleGroups, vg_ensemble
+    insts = vg_ensemble(n_inst=7, sides=(16, 40, 24, 32, 20, 36), seed=9)
+    print("[case] grouped ensemble", flush=True)
+    one = Ensemble(insts)
+    one.richards_init("vg", "bicgstab", "jacobi")
+    grp = EnsembleGroups(insts, EnsembleGroups.split_slowest([3, 9, 1, 4, 1, 8, 2], 2))
+    assert grp.groups == [[0, 2, 3, 4, 6], [1, 5]]
+    grp.richards_init("vg", "bicgstab", "jacobi")
+    for _ in range(5):
+        a = one.step(1e-4, per_instance=True)
+        b = grp.step(1e-4, per_instance=True)
+        for key in ("picard_iters_i", "krylov_iters_i"):
+            assert np.array_equal(a[key], b[key]), key
+        assert a["picard_iters"] == b["picard_iters"] and a["krylov_iters"] == b["krylov_iters"]
+    for x, y in zip(one.state(), grp.state()):
+        assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
+    one.close()
+    grp.close()
