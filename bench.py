@@ -183,7 +183,8 @@ def launch_note():
 
 def timing_note(steps, el):
     return (f"roofline from {steps} further steps of the same run, every SpMV launched with start/stop HIP "
-            f"events attached to its dispatch (hipExtLaunchKernelGGL; {el / max(steps, 1) * 1e3:.4f} ms/step there)")
+            f"events attached to its dispatch (hipExtLaunchKernelGGL; {el / max(steps, 1) * 1e3:.4f} ms/step there; "
+            "ensemble groups stepped one after another there)")
 
 
 def run_ensemble(args):
@@ -232,7 +233,10 @@ def run_ensemble(args):
             print(f"[bench] c3 step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    launches, spmv_ms, rows, el_ev = spmv_timing_region(ens.timing, ens.timing_read, ens.step, dt,
+    # per-launch timing: groups stepped one after another, so that launches of
+    # different streams do not share the GPU while they are timed
+    step_seq = (lambda d: ens.step(d, concurrent=False)) if groups else ens.step
+    launches, spmv_ms, rows, el_ev = spmv_timing_region(ens.timing, ens.timing_read, step_seq, dt,
                                                          0 if args.no_timing else args.steps, mixed)
     h = ens.state()[0]
     assert bool(torch.isfinite(h).all()), "non-finite pressure head"

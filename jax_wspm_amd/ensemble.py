@@ -255,8 +255,17 @@ class EnsembleGroups:
     def richards_init(self, *args, **kw):
         self._each(lambda m: m.richards_init(*args, **kw))
 
-    def step(self, dt: float, per_instance: bool = False):
-        res = self._each(lambda m: m.step(dt, per_instance))
+    def step(self, dt: float, per_instance: bool = False, concurrent: bool = True):
+        """One time step of every instance; concurrent False steps the groups
+        one after another (same results; kernels of different groups then do
+        not overlap, e.g. for per-launch timing)."""
+        if concurrent:
+            res = self._each(lambda m: m.step(dt, per_instance))
+        else:
+            res = []
+            for m, s in zip(self.members, self.streams):
+                with torch.cuda.stream(s):
+                    res.append(m.step(dt, per_instance))
         errs = [r["err"] for r in res]
         out = {"picard_iters": max(r["picard_iters"] for r in res),
                "krylov_iters": sum(r["krylov_iters"] for r in res),
