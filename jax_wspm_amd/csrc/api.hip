@@ -235,7 +235,8 @@ __device__ inline bool ens_on(const KState& s, int mode) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict__ st, int n_inst,
-                                                       const int* __restrict__ iblk, int* act, int mode) {
+                                                       const int* __restrict__ iblk, int* act, int mode,
+                                                       int* act_zero) {
     const int i = blockIdx.x;
     int c = 0;
     for (int j = threadIdx.x; j < i; j += BLOCK)
@@ -253,6 +254,38 @@ __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict_
     if (on)
         for (int b = threadIdx.x; b < nb; b += BLOCK) act[1 + pos + b] = b0 + b;
     if (i == n_inst - 1 && threadIdx.x == 0) act[0] = pos + (on ? nb : 0);
+    // the other Krylov list's count: zero for the next list built by
+    // k_finish_ens_k (nothing reads that list here)
+    if (act_zero != nullptr && i == 0 && threadIdx.x == 0) *act_zero = 0;
+}
+
+// k_finish_ens that also rebuilds the Krylov active list (mode 0) from the
+// states it has just finished, in place of the k_ens_compact launch that
+// would follow it.  Workgroup i appends instance i's entries at a position
+// taken from the list's counter (act_out[0], zero on entry), so the entry
+// order of the list varies from run to run; the results do not, since every
+// entry is computed on its own and stores its partial rows in its own
+// column.  act_zero: the count of the list being replaced, set to zero for
+// the next rebuild (nothing reads that list in this kernel).
+template <int ND, class Op>
+__global__ __launch_bounds__(BLOCK) void k_finish_ens_k(Op op, const double* __restrict__ partials, int prow,
+                                                        const int* __restrict__ iblk, KState* st0, int* act_out,
+                                                        int* act_zero) {
+    KState* st = st0 + blockIdx.x;
+    if (!(st->pdone || op.skip(st))) {   // uniform over the workgroup
+        double acc[ND];
+        ens_reduce<ND>(partials, prow, iblk[blockIdx.x], iblk[blockIdx.x + 1], acc);
+        if (threadIdx.x == 0) op.fin(acc, st);
+    }
+    __shared__ int pos;
+    const int b0 = iblk[blockIdx.x], nb = iblk[blockIdx.x + 1] - b0;
+    if (threadIdx.x == 0) {   // the state as this thread has just left it
+        pos = ens_on(*st, 0) ? atomicAdd(act_out, nb) : -1;
+        if (blockIdx.x == 0) *act_zero = 0;
+    }
+    __syncthreads();
+    if (pos >= 0)
+        for (int b = threadIdx.x; b < nb; b += BLOCK) act_out[1 + pos + b] = b0 + b;
 }
 
 // Per-instance statistics of the last time step.
@@ -653,6 +686,8 @@ struct wspm_ctx {
     int* hstatus = nullptr;         // pinned mirror
     int* ens_out = nullptr;         // k_ens_collect output: 3 int + 1 double per instance
     int* kact = nullptr;            // [1 + n_tab] entries of instances whose Krylov solve runs
+    int* kact_alt = nullptr;        // the list k_finish_ens_k builds next (count zero between builds)
+    bool ens_list_kernel = false;   // WSPM_ENS_LIST=kernel: Krylov lists rebuilt by k_ens_compact only
     int* pact = nullptr;            // [1 + n_tab] entries of instances still iterating Picard
     int ens_grid = 0;               // workgroups of an ensemble launch
     int ens_compact_every = 1;      // Krylov iterations between rebuilds of the active list
@@ -764,15 +799,28 @@ EnsWalk walk_of(wspm_ctx* ctx, int list) {
 void ens_compact(wspm_ctx* ctx, int list, int mode = -1) {
     if (mode < 0) mode = list == LIST_K ? 0 : 1;
     hipLaunchKernelGGL(k_ens_compact, dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, (const KState*)ctx->states,
-                       ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact, mode);
+                       ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact, mode,
+                       list == LIST_K ? ctx->kact_alt : nullptr);
     HIPCHK(hipGetLastError());
 }
 
+// The Krylov list is rebuilt by the finish kernel of the iteration's last
+// reduction (k_finish_ens_k) instead of a k_ens_compact launch after it,
+// except under graph replay (the captured kernels keep the list pointers of
+// the capture, and this swaps them).
+// WSPM_ENS_LIST=kernel keeps the separate launch.
+bool fused_compact(const wspm_ctx* ctx) { return ctx->tab && !ctx->use_graphs && !ctx->ens_list_kernel; }
+
 // Finish of a reduction kernel: one workgroup, or one per instance (ensemble).
 template <class Op, bool RESET_PEV = false>
-void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
+void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which, bool compact_k = false) {
     if constexpr (Op::ND > 0) {
-        if (ctx->tab) {
+        if (ctx->tab && compact_k) {
+            hipLaunchKernelGGL((k_finish_ens_k<Op::ND, Op>), dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, op,
+                               ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which), ctx->kact_alt,
+                               ctx->kact);
+            std::swap(ctx->kact, ctx->kact_alt);
+        } else if (ctx->tab) {
             hipLaunchKernelGGL((k_finish_ens<Op::ND, Op, RESET_PEV>), dim3(ctx->n_inst), dim3(BLOCK), 0,
                                ctx->stream, op, ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which));
         } else {
@@ -783,13 +831,14 @@ void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which) {
     }
 }
 
+// compact_k: the finish also rebuilds the Krylov list (ensembles)
 template <class Op>
-void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL) {
+void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL, bool compact_k = false) {
     const int grid = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
     hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
                        walk_of(ctx, list));
     HIPCHK(hipGetLastError());
-    launch_finish(ctx, op, grid, which);
+    launch_finish(ctx, op, grid, which, compact_k);
 }
 
 hipEvent_t next_event(wspm_ctx* ctx) {
@@ -1217,6 +1266,11 @@ struct KrylovRun {
             return;
         }
         for (int c = 0; c < count && enq < maxiter; ++c, ++enq) {
+            // drop the instances that converged (every ens_compact_every-th
+            // iteration; in between, their entries are walked and skipped),
+            // by the finish of the iteration's last reduction where it can
+            const bool compact = ctx->tab && (enq + 1) % ctx->ens_compact_every == 0;
+            const bool fc = compact && fused_compact(ctx) && (method == WSPM_BICGSTAB || !pc);
             if (method == WSPM_BICGSTAB) {
                 launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which, LIST_K);
                 if (pc) apply(v.p, v.ph, 1);
@@ -1224,10 +1278,10 @@ struct KrylovRun {
                 launch_vec(ctx, BicgS<T>{v, 0.0}, which, LIST_K);
                 if (pc) apply(v.r, v.sh, 1);      // s lives in r
                 launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true, LIST_K);
-                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K);
+                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K, fc);
             } else if (method == WSPM_CG) {
                 launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true, LIST_K);
-                launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K);
+                launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K, fc);   // decides done; CgP skips those
                 if (pc) {
                     apply(v.r, v.z, 1);
                     launch_vec(ctx, CgPcRZ<T>{v}, which, LIST_K);
@@ -1236,9 +1290,7 @@ struct KrylovRun {
             } else {
                 gmres_cycle();
             }
-            // drop the instances that converged (every ens_compact_every-th
-            // iteration; in between, their entries are walked and skipped)
-            if (ctx->tab && (enq + 1) % ctx->ens_compact_every == 0) ens_compact(ctx, LIST_K);
+            if (compact && !fc) ens_compact(ctx, LIST_K);
         }
     }
 
@@ -2732,6 +2784,9 @@ int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
         ctx->ens_status = dalloc<int>(ctx, 4);
         ctx->ens_out = dalloc<int>(ctx, 5 * (size_t)n_inst + 2);  // 3 int + 1 double each
         ctx->kact = dalloc<int>(ctx, 1 + (size_t)ctx->n_tab);
+        ctx->kact_alt = dalloc<int>(ctx, 1 + (size_t)ctx->n_tab);
+        HIPCHK(hipMemset(ctx->kact_alt, 0, sizeof(int)));
+        ctx->ens_list_kernel = env_is("WSPM_ENS_LIST", "kernel");
         ctx->pact = dalloc<int>(ctx, 1 + (size_t)ctx->n_tab);
         HIPCHK(hipMemset(ctx->kact, 0, sizeof(int)));
         HIPCHK(hipMemset(ctx->pact, 0, sizeof(int)));

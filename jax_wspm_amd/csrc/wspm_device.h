@@ -157,6 +157,21 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restric
     if (threadIdx.x == 0) op.fin(acc, st);
 }
 
+// Sum of the partial columns [b0, b1) per dot in the order every ensemble
+// finish uses (thread-strided columns, then block_sum); the total is in
+// thread 0's acc.
+template <int ND>
+__device__ inline void ens_reduce(const double* __restrict__ partials, int prow, int b0, int b1, double (&acc)[ND]) {
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += BLOCK) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
+    }
+    __shared__ double sh[ND * NWAVE];
+    block_sum<ND>(acc, sh);
+}
+
 // Ensemble finish: workgroup i adds the partials of instance i's workgroups
 // [iblk[i], iblk[i+1]) in order and runs op.fin on state st[i].  Instances
 // whose Picard iteration has converged are left alone.  RESET_PEV: this
@@ -166,16 +181,8 @@ __global__ __launch_bounds__(BLOCK) void k_finish_ens(Op op, const double* __res
                                                        const int* __restrict__ iblk, KState* st0) {
     KState* st = st0 + blockIdx.x;
     if (st->pdone || op.skip(st)) return;
-    const int b0 = iblk[blockIdx.x], b1 = iblk[blockIdx.x + 1];
     double acc[ND];
-#pragma unroll
-    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += BLOCK) {
-#pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
-    }
-    __shared__ double sh[ND * NWAVE];
-    block_sum<ND>(acc, sh);
+    ens_reduce<ND>(partials, prow, iblk[blockIdx.x], iblk[blockIdx.x + 1], acc);
     if (threadIdx.x == 0) {
         op.fin(acc, st);
         if (RESET_PEV) st->pev = 0;

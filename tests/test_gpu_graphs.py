@@ -5,7 +5,10 @@ The time step replays each Picard iteration (assembly + solver start, the
 expected Krylov iterations, the Picard error, the state publication) from a
 captured HIP graph and reads the device state back through a sequence word in
 pinned host memory.  WSPM_GRAPHS=0 launches every kernel on the stream and
-WSPM_POLL=sync reads back by copy + stream synchronisation.  Every kernel is
+WSPM_POLL=sync reads back by copy + stream synchronisation.  Ensemble
+Krylov lists are rebuilt by the finish kernel of each iteration's last
+reduction (entries in arrival order); WSPM_ENS_LIST=kernel and graph replay
+rebuild them with a separate kernel, in instance order.  Every kernel is
 deterministic, so all modes must give bitwise identical trajectories and
 iteration counts; each graph is replayed many times, with the host's own
 launches (Krylov follow-ups, the theta_n copy) and synchronisations between
@@ -70,13 +73,14 @@ def test_finish_free_krylov_bitwise(monkeypatch, solver, precond, precision):
         assert np.array_equal(h[k], ref_h[k]), k
 
 
-def _ensemble(monkeypatch, graphs, poll, solver, steps):
+def _ensemble(monkeypatch, graphs, poll, solver, steps, lists="fused", precision="fp64"):
     from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
     monkeypatch.setenv("WSPM_GRAPHS", graphs)
     monkeypatch.setenv("WSPM_POLL", poll)
+    monkeypatch.setenv("WSPM_ENS_LIST", lists)
     insts = vg_ensemble(n_inst=6, sides=(16, 32, 24, 40, 20, 36), seed=11)
     ens = Ensemble(insts)
-    ens.richards_init("vg", solver, "jacobi")
+    ens.richards_init("vg", solver, "jacobi", precision=precision)
     hs, its = [], []
     for _ in range(steps):
         st = ens.step(1e-4, per_instance=True)
@@ -95,3 +99,19 @@ def test_ensemble_modes_bitwise(monkeypatch, solver):
         for k in range(len(h)):
             assert np.array_equal(it[k][0], ref_it[k][0]) and np.array_equal(it[k][1], ref_it[k][1]), (graphs, k)
             assert np.array_equal(h[k], ref_h[k]), (graphs, poll, k)
+
+
+@pytest.mark.parametrize("solver,precision", [("bicgstab", "fp64"), ("cg", "fp64"), ("bicgstab", "mixed"),
+                                              ("cg", "mixed")])
+def test_ensemble_list_rebuild_bitwise(monkeypatch, solver, precision):
+    """Krylov lists rebuilt inside the finish kernel (entry order varies from
+    run to run) against a separate compaction kernel (WSPM_ENS_LIST=kernel):
+    every entry is computed on its own, so bitwise the same trajectories and
+    per-instance iteration counts, run after run."""
+    print(f"[case] ensemble list rebuild {solver} {precision}", flush=True)
+    ref_h, ref_it = _ensemble(monkeypatch, "0", "spin", solver, 6, "kernel", precision)
+    for _ in range(2):
+        h, it = _ensemble(monkeypatch, "0", "spin", solver, 6, "fused", precision)
+        for k in range(len(h)):
+            assert np.array_equal(it[k][0], ref_it[k][0]) and np.array_equal(it[k][1], ref_it[k][1]), k
+            assert np.array_equal(h[k], ref_h[k]), k
