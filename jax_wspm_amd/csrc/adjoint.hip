@@ -184,14 +184,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_jacobian(AdjArgs a, double* J, do
 #pragma unroll
             for (int c = 0; c < 3; ++c) gg[c] = Gx[la] * Gx[c] + Gz[la] * Gz[c];
             const double sa = gg[0] * hv[0] + gg[1] * hv[1] + gg[2] * hv[2] + Gz[la];
-            const uint32_t w = (uint32_t)ic.y;
+            double jac[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double jac = det * (kbar * gg[c] + ks * a.qc.ck[c] * dkr[c] * sa);
-                const uint32_t sl = (w >> (8 * c)) & 0xffu;
-#pragma unroll
-                for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? jac : 0.0;
-            }
+            for (int c = 0; c < 3; ++c) jac[c] = det * (kbar * gg[c] + ks * a.qc.ck[c] * dkr[c] * sa);
+            add_slots<3, MAXR>(v, (uint32_t)ic.y, jac);
             msum += det * a.qc.ml[la];
         }
         const double md = a.inv_dt * msum * a.nd.dth_dh[i];

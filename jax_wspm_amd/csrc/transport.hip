@@ -154,6 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_transport(TrArgs a) {
             Tqz += Q.T[la][c] * qz[c];
         }
         const uint32_t w = (uint32_t)ic.y;
+        double adds[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double uth = 0.0;
@@ -170,10 +171,9 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_transport(TrArgs a) {
                 add += a.inv_dt * m;
                 bm += m * cnv[c];
             }
-            const uint32_t sl = (w >> (8 * c)) & 0xffu;
-#pragma unroll
-            for (int s2 = 0; s2 < MAXR; ++s2) v[s2] += (sl == (uint32_t)s2) ? add : 0.0;
+            adds[c] = add;
         }
+        add_slots<3, MAXR>(v, w, adds);
         if (a.c_src) {
             double s = 0.0;
 #pragma unroll
