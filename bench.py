@@ -93,8 +93,10 @@ def parse():
     ap.add_argument("--inner-tol", type=float, default=1e-4,
                     help="c3-mixed: relative tolerance of the FP32 inner solves (each refinement sweep)")
     ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
-    ap.add_argument("--fd-order", type=int, default=4, choices=(2, 4),
-                    help="c4: order of the central-difference stencil (4: four forward runs per direction)")
+    ap.add_argument("--fd-eps", type=float, default=None,
+                    help="c4: step of the difference quotients (default 5e-4 at order 6, 1e-4 at orders 2 and 4)")
+    ap.add_argument("--fd-order", type=int, default=6, choices=(2, 4, 6),
+                    help="c4: order p of the central-difference stencil (p forward runs per direction)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: steps per instance of the plain single-instance path (0 = skip)")
     args = ap.parse_args()
@@ -430,15 +432,24 @@ def run_adjoint(args):
     drng = np.random.default_rng(1)
     for k in range(args.fd_dirs):
         d = drng.normal(size=p0.size) * scale
-        # fourth-order central differences, eps 1e-4: truncation ~eps^4 (the
-        # second-order stencil's eps^2 term was up to 1.6e-6 of the derivative
-        # on these directions); the forward's solver noise is divided by eps
-        eps = 1e-4
+        # error budget of the difference quotient: truncation ~eps^p for the
+        # order-p stencil plus the forward's converged-tolerance noise (about
+        # 1e-11 in the loss) divided by eps.  Measured on these directions:
+        # order 2 at eps 1e-4, up to 1.6e-6 truncation; order 4 at 1e-4,
+        # noise of up to 1e-6 in the direction with the smallest derivative
+        # (g.d 0.41); order 4 at 5e-4, 8.9e-7 truncation in the most
+        # nonlinear direction.  Order 6 at 5e-4 keeps both terms below 1e-7.
+        eps = args.fd_eps if args.fd_eps else {2: 1e-4, 4: 1e-4, 6: 5e-4}[args.fd_order]
+
+        def dl(k):
+            return loss(p0 + k * eps * d) - loss(p0 - k * eps * d)
+
         if args.fd_order == 2:
-            fd = (loss(p0 + eps * d) - loss(p0 - eps * d)) / (2 * eps)
+            fd = dl(1) / (2 * eps)
+        elif args.fd_order == 4:
+            fd = (8.0 * dl(1) - dl(2)) / (12 * eps)
         else:
-            fd = (8.0 * (loss(p0 + eps * d) - loss(p0 - eps * d))
-                  - (loss(p0 + 2 * eps * d) - loss(p0 - 2 * eps * d))) / (12 * eps)
+            fd = (45.0 * dl(1) - 9.0 * dl(2) + dl(3)) / (60 * eps)
         errs.append(abs(g @ d - fd) / max(abs(fd), 1e-300))
         nerrs.append(abs(g @ d - fd) / (np.linalg.norm(g) * np.linalg.norm(d)))
         print(f"[bench] c4 direction {k + 1}/{args.fd_dirs}: g.d {g @ d:.6e} fd {fd:.6e} rel err {errs[-1]:.2e}",
@@ -459,7 +470,7 @@ def run_adjoint(args):
                    "flow_adjoint_bicgstab_iters_per_step": round(float(np.mean(model.adj_iters)), 2),
                    "species_adjoint_bicgstab_iters_per_step": round(float(np.mean(model.adj_tr_iters)), 2),
                    "fd_directions": args.fd_dirs,
-                   "fd_stencil": f"central, order {args.fd_order}, eps 1e-4",
+                   "fd_stencil": f"central, order {args.fd_order}, eps {eps:g}",
                    "fd_max_rel_err": float(max(errs)) if errs else None,
                    "fd_median_rel_err": float(np.median(errs)) if errs else None,
                    "fd_max_err_over_norms": float(max(nerrs)) if nerrs else None,
