@@ -100,7 +100,10 @@ struct PicardOp {
     double* krec;
     int rel;
     int ens;
-    __device__ bool skip(const KState* s) const { return ens && (s->pdone || !s->done || s->pev); }
+    const int* gate = nullptr;   // one system: *gate == 0 skips this update (an iteration enqueued ahead)
+    __device__ bool skip(const KState* s) const {
+        return (gate != nullptr && *gate == 0) || (ens && (s->pdone || !s->done || s->pev));
+    }
     __device__ void load(const KState* s) {
         if (ens) sp = s->sp;
     }
@@ -169,9 +172,17 @@ __global__ __launch_bounds__(BLOCK) void k_publish(const uint32_t* __restrict__ 
 
 // k_finish (one workgroup) that then publishes the whole state as k_publish
 // does: the Picard error's reduction and the host read-back in one kernel.
+// gate_in non-null and *gate_in == 0: this iteration was enqueued ahead and
+// is not needed; nothing is reduced or published.  gate_out non-null: opens
+// the next Picard iteration, enqueued ahead (*gate_out = 1), when this one
+// has not converged (err >= gtol, not NaN), its Krylov solve is done and
+// gallow (another iteration is allowed); the host evaluates the same test on
+// the published state.
 template <int ND, class Op>
 __global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __restrict__ partials, int prow,
-                                                      int nblk, KState* st, uint32_t* hdst, int* dseq, int* hseq) {
+                                                      int nblk, KState* st, uint32_t* hdst, int* dseq, int* hseq,
+                                                      const int* gate_in, int* gate_out, double gtol, int gallow) {
+    if (gate_in != nullptr && *gate_in == 0) return;   // uniform
     if (!op.skip(st)) {   // uniform over the workgroup
         double acc[ND];
 #pragma unroll
@@ -182,7 +193,10 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __res
         }
         __shared__ double sh[ND * NWAVE];
         block_sum<ND>(acc, sh);
-        if (threadIdx.x == 0) op.fin(acc, st);
+        if (threadIdx.x == 0) {
+            op.fin(acc, st);
+            if (gate_out != nullptr) *gate_out = st->picard_err >= gtol && st->done && gallow;
+        }
     }
     // thread 0's state stores, seen by the whole workgroup (one CU)
     __threadfence_block();
@@ -634,6 +648,13 @@ struct wspm_ctx {
     double* partials2 = nullptr;
     KHist* khist[2] = {nullptr, nullptr};
     int khist_cap[2] = {0, 0};
+    // a second Picard iteration enqueued ahead of the first one's read-back:
+    // its gate (opened by the first iteration's finish), its Krylov history
+    // and its pinned state mirror
+    int* gate = nullptr;
+    KHist* khist_ahead = nullptr;
+    int khist_ahead_cap = 0;
+    KState* hstate_ahead = nullptr;
     bool ell = false;
     bool spmv_ell = false;
     bool ell_faster = true;         // ensembles: the autotuner timed ELL against the best CSR width
@@ -966,7 +987,10 @@ void poll_ens(wspm_ctx* ctx, int which) {
 // the state published for the host when fast_poll: the error's finish kernel
 // does the publication (one kernel less per Picard iteration).  picard_poll
 // also waits for it.
-void picard_enqueue(wspm_ctx* ctx, const PicardOp& pop) {
+// hdst: the pinned mirror to publish into (default ctx->hstate); gate_out /
+// gtol / gallow and pop.gate as k_finish_pub's gate_out and gate_in.
+void picard_enqueue(wspm_ctx* ctx, const PicardOp& pop, KState* hdst = nullptr, int* gate_out = nullptr,
+                    double gtol = 0.0, int gallow = 0) {
     if (!ctx->fast_poll) {
         launch_vec(ctx, pop, 0);
         return;
@@ -977,7 +1001,8 @@ void picard_enqueue(wspm_ctx* ctx, const PicardOp& pop) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_finish_pub<PicardOp::ND, PicardOp>), dim3(1), dim3(BLOCK), 0, ctx->stream, pop,
                        ctx->partials, ctx->prow, grid, states_of(ctx, 0),
-                       reinterpret_cast<uint32_t*>(ctx->hstate), ctx->dseq, ctx->hseq);
+                       reinterpret_cast<uint32_t*>(hdst ? hdst : ctx->hstate), ctx->dseq, ctx->hseq, pop.gate,
+                       gate_out, gtol, gallow);
     HIPCHK(hipGetLastError());
 }
 
@@ -1143,6 +1168,8 @@ struct KrylovRun {
     // folded BiCGSTAB started by the assembly with its finish deferred: the
     // start's partial rows, reduced by the first iteration's p kernel
     FoldIn start_in{nullptr, 0, 0};
+    // folded BiCGSTAB enqueued ahead: its first p kernel ends the solve while *gate == 0
+    const int* gate = nullptr;
     double tol0 = 0.0, atol0 = 0.0;
 
     // a folded kernel: no finish, partial rows into part
@@ -1239,7 +1266,10 @@ struct KrylovRun {
             if (method == WSPM_BICGSTAB) {
                 FBicgP<T> P{v, hist, stw, FoldIn{PB, pr, vb}, k, 0.0, 0.0, 0, 0, FoldIn{nullptr, 0, 0}, tol0, atol0,
                             maxiter};
-                if (k == 0) P.start = start_in;
+                if (k == 0) {
+                    P.start = start_in;
+                    P.gate = gate;
+                }
                 vec_nf(P, nullptr, ctx->n, vb);
                 launch_spmv<T>(ctx, A, FBicgQ<T>{{v}, hist, k}, which, true, LIST_ALL, PA, false);
                 vec_nf(FBicgS<T>{{v, 0.0}, hist, FoldIn{PA, pr, sb}, k}, PB, ctx->n, vb);
@@ -1627,8 +1657,10 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
                         const double* K, const double* theta_m, const double* theta_n, double dt,
                         double* A, double* b, double* dinv, float* A32, float* dinv32,
                         int mode = ASM_INIT_NONE, double* x = nullptr, double tol = 0.0, double atol = 0.0,
-                        int maxiter = 0, int restart = 0, bool ell = false, int* start_grid = nullptr) {
+                        int maxiter = 0, int restart = 0, bool ell = false, int* start_grid = nullptr,
+                        const int* gate = nullptr) {
     AsmArgs a{};
+    a.gate = gate;
     a.n = ctx->n;
     a.win_cap = ctx->asm_win;
     a.inc_cap = ctx->asm_inc;
@@ -2097,6 +2129,30 @@ int32_t wspm_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, const
 
 static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats);
 
+// Gate, pinned mirror and Krylov history of a Picard iteration enqueued ahead.
+static void ensure_ahead(wspm_ctx* ctx) {
+    if (!ctx->gate) {
+        ctx->gate = dalloc<int>(ctx, 1);
+        HIPCHK(hipMemset(ctx->gate, 0, sizeof(int)));
+    }
+    if (!ctx->hstate_ahead) {
+        void* hp = nullptr;
+        HIPCHK(hipHostMalloc(&hp, sizeof(KState), hipHostMallocDefault));
+        ctx->host_allocs.push_back(hp);
+        ctx->hstate_ahead = static_cast<KState*>(hp);
+        std::memset(ctx->hstate_ahead, 0, sizeof(KState));
+    }
+}
+
+static KHist* khist_ahead(wspm_ctx* ctx, int maxiter) {
+    const int need = std::max(maxiter, 0) + 2;
+    if (ctx->khist_ahead_cap < need) {
+        ctx->khist_ahead = dalloc<KHist>(ctx, need);
+        ctx->khist_ahead_cap = need;
+    }
+    return ctx->khist_ahead;
+}
+
 int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     if (!ctx) return WSPM_EINVAL;
     if (!ctx->rich_init) return fail(ctx, WSPM_ESTATE, "wspm_richards_init not called");
@@ -2130,6 +2186,13 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const bool graphs = ctx->use_graphs && ctx->fast_poll && !own_start && !mixed && !ctx->timing &&
                         (P.method == WSPM_BICGSTAB || P.method == WSPM_CG);
     const EllScope ell_scope(ctx, ctx->ell);
+    // the second Picard iteration enqueued ahead of the first one's read-back:
+    // folded BiCGSTAB started by the register assembly, one system, spin-wait
+    // read-back (WSPM_AHEAD=0: off)
+    const bool ahead = !graphs && ctx->fast_poll && !own_start && !mixed && !ctx->tab && ctx->asm_reg &&
+                       P.method == WSPM_BICGSTAB && P.picard_maxit >= 2 && !env_is("WSPM_KRYLOV", "unfolded") &&
+                       !env_is("WSPM_AHEAD", "0");
+    if (ahead) ensure_ahead(ctx);
     while (err >= P.picard_tol && it < P.picard_maxit) {
         const int mode = own_start ? ASM_INIT_NONE
                          : mixed ? ASM_INIT_RESID
@@ -2165,6 +2228,80 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                 picard_poll(ctx, pop);
             }
             run.stats(&st);
+            kry += st.iters;
+            err = ctx->hstate[0].picard_err;
+            std::swap(ctx->hA, ctx->hB);
+            ++it;
+            if (std::isnan(err)) break;
+            continue;
+        }
+        if (it == 0 && ahead) {
+            // The first two Picard iterations, the second enqueued before the
+            // first one's read-back, so the device does not wait for the
+            // host between them.  The first iteration's finish opens the
+            // second's gate when it has not converged and its Krylov solve is
+            // done; otherwise every kernel of the second returns at once and
+            // the host goes on as without it.  The second iteration writes
+            // nothing the first one's continuation reads: its own Krylov
+            // history, its own pinned state mirror, and the state and work
+            // vectors only once the gate is open.
+            const bool rel = P.picard_mode == WSPM_PICARD_REL;
+            double* dv = jac ? ctx->dinv : nullptr;
+            KrylovRun<double> run1(ctx, P.method, ctx->Av, dv, ctx->bv, ctx->hA, ctx->hB, P.tol, P.atol, P.maxiter,
+                                   P.restart, 0, true, P.precond);
+            int sg = 0;
+            do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv, dv, nullptr,
+                        nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, ctx->ell, &sg);
+            run1.start_in = FoldIn{ctx->partials, ctx->prow, sg};
+            run1.enqueue(run1.first_chunk());
+            const PicardOp pop1{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), rel};
+            picard_enqueue(ctx, pop1, ctx->hstate, ctx->gate, P.picard_tol, 1);
+            // the second iteration: h_m = hB, its iterate into hA
+            KrylovRun<double> run2(ctx, P.method, ctx->Av, dv, ctx->bv, ctx->hB, ctx->hA, P.tol, P.atol, P.maxiter,
+                                   P.restart, 0, true, P.precond);
+            run2.hist = khist_ahead(ctx, P.maxiter);
+            run2.gate = ctx->gate;
+            do_assemble(ctx, Ks, ctx->hB, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv, dv, nullptr,
+                        nullptr, mode, ctx->hA, P.tol, P.atol, P.maxiter, P.restart, ctx->ell, &sg, ctx->gate);
+            run2.start_in = FoldIn{ctx->partials, ctx->prow, sg};
+            run2.enqueue(run2.first_chunk());
+            PicardOp pop2{ctx->sp, ctx->hA, ctx->hB, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), rel};
+            pop2.gate = ctx->gate;
+            picard_enqueue(ctx, pop2, ctx->hstate_ahead);
+            wait_published(ctx);   // the first iteration, in ctx->hstate
+            const KState& s1 = ctx->hstate[0];
+            const bool open = s1.picard_err >= P.picard_tol && s1.done;   // the finish's test (2 iterations allowed)
+            wspm_krylov_stats st{};
+            if (!open) {
+                if (run1.more()) {
+                    while (run1.more()) {
+                        run1.enqueue(1);
+                        poll(ctx, 0);
+                    }
+                    picard_poll(ctx, pop1);
+                }
+            }
+            run1.stats(&st);
+            kry += st.iters;
+            err = ctx->hstate[0].picard_err;
+            std::swap(ctx->hA, ctx->hB);
+            ++it;
+            if (!open) {
+                if (std::isnan(err)) break;
+                continue;
+            }
+            // the second iteration ran: its state, then on as usual
+            wait_published(ctx);
+            ctx->hstate[0] = ctx->hstate_ahead[0];
+            if (run2.more()) {
+                pop2.gate = nullptr;
+                while (run2.more()) {
+                    run2.enqueue(1);
+                    poll(ctx, 0);
+                }
+                picard_poll(ctx, pop2);
+            }
+            run2.stats(&st);
             kry += st.iters;
             err = ctx->hstate[0].picard_err;
             std::swap(ctx->hA, ctx->hB);

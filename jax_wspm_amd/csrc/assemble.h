@@ -73,6 +73,7 @@ struct AsmArgs {
     double* A;
     float* A32;              // nullable: also write an FP32 copy of A
     int ell_n;               // 0: A (and A32) in CSR order; else ELL, entry s of row i at s * ell_n + i
+    const int* gate;         // non-null (one system): the register kernels return at once while *gate == 0
                              // (register kernels only)
     double* b;
     double* dinv;            // nullable

@@ -535,6 +535,7 @@ __device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, 
 template <int D, int MAXR, int MODE, int NBR>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     if (a.tab == nullptr) {
+        if (a.gate != nullptr && *a.gate == 0) return;   // uniform: a Picard iteration enqueued ahead, not needed
         const int i = blockIdx.x * BLOCK + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
         if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);

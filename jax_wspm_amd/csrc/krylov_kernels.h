@@ -469,9 +469,17 @@ struct FBicgP {
     FoldIn start;
     double tol, atol;
     int maxiter;
+    const int* gate = nullptr;   // k == 0: *gate == 0 ends the solve before it starts (state untouched)
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState* s) {
         first = k == 0;
+        if (k == 0 && gate != nullptr && *gate == 0) {
+            go = 0;
+            beta = 0.0;
+            omega = 1.0;
+            if (lead()) H[0].done = 1;
+            return;
+        }
         if (k == 0 && start.p != nullptr) {
             double t[2];
             reduce_all<2>(start, t);

@@ -4,7 +4,9 @@ iterations, the spin-wait read-back and the finish-free Krylov iterations.
 The time step replays each Picard iteration (assembly + solver start, the
 expected Krylov iterations, the Picard error, the state publication) from a
 captured HIP graph and reads the device state back through a sequence word in
-pinned host memory.  WSPM_GRAPHS=0 launches every kernel on the stream and
+pinned host memory.  Without graphs, the second Picard iteration of a step
+is enqueued before the first one's read-back, behind a device-side gate
+(WSPM_AHEAD=0: enqueued after it).  WSPM_GRAPHS=0 launches every kernel on the stream and
 WSPM_POLL=sync reads back by copy + stream synchronisation.  Ensemble
 Krylov lists are rebuilt by the finish kernel of each iteration's last
 reduction (entries in arrival order); WSPM_ENS_LIST=kernel and graph replay
@@ -115,3 +117,43 @@ def test_ensemble_list_rebuild_bitwise(monkeypatch, solver, precision):
         for k in range(len(h)):
             assert np.array_equal(it[k][0], ref_it[k][0]) and np.array_equal(it[k][1], ref_it[k][1]), k
             assert np.array_equal(h[k], ref_h[k]), k
+
+
+def _single_ahead(monkeypatch, ahead, dts, **kw):
+    from jax_wspm_amd.device import Context
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    monkeypatch.setenv("WSPM_GRAPHS", "0")
+    monkeypatch.setenv("WSPM_POLL", "spin")
+    monkeypatch.setenv("WSPM_AHEAD", ahead)
+    pb = from_config(baseline_vg_config(40, 2, solver="bicgstab", precond="jacobi", nsteps=1, dt=1e-4,
+                                        ks_field={"sigma": 0.5, "length": 0.05, "seed": 3}))
+    ctx = Context(pb.points, pb.elements)
+    ctx.set_dirichlet(pb.bc_lists)
+    ctx.richards_init("vg", pb.phi, "bicgstab", "jacobi", pb.h0, Ks_elem=pb.Ks_elem, **kw)
+    hs, its = [], []
+    for dt in dts:
+        st = ctx.richards_step(dt)
+        its.append((st["picard_iters"], st["krylov_iters"], st["converged"]))
+        hs.append(ctx.richards_state()[0].cpu().numpy().copy())
+    ctx.close()
+    return hs, its
+
+
+@pytest.mark.parametrize("case,kw", [("default", {}),
+                                     ("converges_in_one", {"picard_tol": 1.0}),
+                                     ("two_allowed", {"picard_maxit": 2}),
+                                     ("krylov_cut", {"maxiter": 1}),
+                                     ("absolute", {"picard_mode": "abs", "picard_tol": 1e-3})])
+def test_picard_iteration_ahead_bitwise(monkeypatch, case, kw):
+    """The second Picard iteration enqueued ahead of the first one's
+    read-back, behind a gate the first one's finish opens, against the plain
+    order (WSPM_AHEAD=0): bitwise the same trajectories and counts, also where
+    the gate stays shut (converged after one iteration, Krylov solve cut
+    short, a limit of two iterations)."""
+    dts = [1e-4] * 3 + [5e-4, 1e-4, 2e-3, 1e-4]
+    print(f"[case] picard ahead {case}", flush=True)
+    ref_h, ref_it = _single_ahead(monkeypatch, "0", dts, **kw)
+    h, it = _single_ahead(monkeypatch, "1", dts, **kw)
+    assert it == ref_it
+    for k in range(len(dts)):
+        assert np.array_equal(h[k], ref_h[k]), k
