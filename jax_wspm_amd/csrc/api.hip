@@ -273,8 +273,8 @@ __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict_
     if (act_zero != nullptr && i == 0 && threadIdx.x == 0) *act_zero = 0;
 }
 
-// k_finish_ens that also rebuilds the Krylov active list (mode 0) from the
-// states it has just finished, in place of the k_ens_compact launch that
+// k_finish_ens that also rebuilds the Krylov active list (mode as ens_on) from
+// the states it has just finished, in place of the k_ens_compact launch that
 // would follow it.  Workgroup i appends instance i's entries at a position
 // taken from the list's counter (act_out[0], zero on entry), so the entry
 // order of the list varies from run to run; the results do not, since every
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict_
 template <int ND, class Op>
 __global__ __launch_bounds__(BLOCK) void k_finish_ens_k(Op op, const double* __restrict__ partials, int prow,
                                                         const int* __restrict__ iblk, KState* st0, int* act_out,
-                                                        int* act_zero) {
+                                                        int* act_zero, int mode) {
     KState* st = st0 + blockIdx.x;
     if (!(st->pdone || op.skip(st))) {   // uniform over the workgroup
         double acc[ND];
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish_ens_k(Op op, const double* __r
     __shared__ int pos;
     const int b0 = iblk[blockIdx.x], nb = iblk[blockIdx.x + 1] - b0;
     if (threadIdx.x == 0) {   // the state as this thread has just left it
-        pos = ens_on(*st, 0) ? atomicAdd(act_out, nb) : -1;
+        pos = ens_on(*st, mode) ? atomicAdd(act_out, nb) : -1;
         if (blockIdx.x == 0) *act_zero = 0;
     }
     __syncthreads();
@@ -834,12 +834,12 @@ bool fused_compact(const wspm_ctx* ctx) { return ctx->tab && !ctx->use_graphs &&
 
 // Finish of a reduction kernel: one workgroup, or one per instance (ensemble).
 template <class Op, bool RESET_PEV = false>
-void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which, bool compact_k = false) {
+void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which, bool compact_k = false, int cmode = 0) {
     if constexpr (Op::ND > 0) {
         if (ctx->tab && compact_k) {
             hipLaunchKernelGGL((k_finish_ens_k<Op::ND, Op>), dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, op,
                                ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which), ctx->kact_alt,
-                               ctx->kact);
+                               ctx->kact, cmode);
             std::swap(ctx->kact, ctx->kact_alt);
         } else if (ctx->tab) {
             hipLaunchKernelGGL((k_finish_ens<Op::ND, Op, RESET_PEV>), dim3(ctx->n_inst), dim3(BLOCK), 0,
@@ -852,14 +852,15 @@ void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which, bool compac
     }
 }
 
-// compact_k: the finish also rebuilds the Krylov list (ensembles)
+// compact_k: the finish also rebuilds the Krylov list (ensembles; cmode as ens_on)
 template <class Op>
-void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL, bool compact_k = false) {
+void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL, bool compact_k = false,
+                int cmode = 0) {
     const int grid = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
     hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
                        walk_of(ctx, list));
     HIPCHK(hipGetLastError());
-    launch_finish(ctx, op, grid, which, compact_k);
+    launch_finish(ctx, op, grid, which, compact_k, cmode);
 }
 
 hipEvent_t next_event(wspm_ctx* ctx) {
@@ -881,7 +882,7 @@ constexpr size_t TIMING_SLOTS = (size_t)1 << 18;
 // finish kernel (a folded Krylov kernel's consumer reduces the rows itself).
 template <class T, class Epi>
 void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool timed = false, int list = LIST_ALL,
-                 double* part = nullptr, bool finish = true) {
+                 double* part = nullptr, bool finish = true, bool compact_k = false) {
     const bool t = timed && ctx->timing && ctx->timing_fp32 == std::is_same<T, float>::value &&
                    ctx->ev_used / 2 < TIMING_SLOTS;
     int* tflag = t ? ctx->tflags + ctx->ev_used / 2 : nullptr;
@@ -902,7 +903,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         const int eg = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
         if (ctx->ell_w == 7) go(k_spmv_ell<7, T, Epi>, dim3(eg), ctx->n, (const int32_t*)ctx->ell_col, av, epi, red, tflag, tab);
         else go(k_spmv_ell<8, T, Epi>, dim3(eg), ctx->n, (const int32_t*)ctx->ell_col, av, epi, red, tflag, tab);
-        if (finish) launch_finish(ctx, epi, eg, which);
+        if (finish) launch_finish(ctx, epi, eg, which, compact_k);
         return;
     }
     const int grid = ctx->tab ? ctx->ens_grid : ctx->spmv_grid;
@@ -917,7 +918,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         case 32: go(k_spmv<32, T, Epi>, g, ctx->n, rp, ci, av, epi, red, tflag, tab); break;
         default: go(k_spmv<64, T, Epi>, g, ctx->n, rp, ci, av, epi, red, tflag, tab); break;
     }
-    if (finish) launch_finish(ctx, epi, grid, which);
+    if (finish) launch_finish(ctx, epi, grid, which, compact_k);
 }
 
 // Wait for the next publication (k_publish / k_ens_status with a sequence
@@ -1170,6 +1171,10 @@ struct KrylovRun {
     FoldIn start_in{nullptr, 0, 0};
     // folded BiCGSTAB enqueued ahead: its first p kernel ends the solve while *gate == 0
     const int* gate = nullptr;
+    // ensembles, >= 0: the last iteration of each enqueue() call rebuilds the
+    // Krylov list with this ens_on mode in its last finish (where that finish
+    // rebuilds lists)
+    int last_mode = -1;
     double tol0 = 0.0, atol0 = 0.0;
 
     // a folded kernel: no finish, partial rows into part
@@ -1299,8 +1304,10 @@ struct KrylovRun {
             // drop the instances that converged (every ens_compact_every-th
             // iteration; in between, their entries are walked and skipped),
             // by the finish of the iteration's last reduction where it can
-            const bool compact = ctx->tab && (enq + 1) % ctx->ens_compact_every == 0;
+            const bool last = last_mode >= 0 && (c == count - 1 || enq + 1 == maxiter);
+            const bool compact = ctx->tab && ((enq + 1) % ctx->ens_compact_every == 0 || last);
             const bool fc = compact && fused_compact(ctx) && (method == WSPM_BICGSTAB || !pc);
+            const int cm = last ? last_mode : 0;
             if (method == WSPM_BICGSTAB) {
                 launch_vec(ctx, BicgP<T>{v, 0.0, 0.0, 0}, which, LIST_K);
                 if (pc) apply(v.p, v.ph, 1);
@@ -1308,10 +1315,10 @@ struct KrylovRun {
                 launch_vec(ctx, BicgS<T>{v, 0.0}, which, LIST_K);
                 if (pc) apply(v.r, v.sh, 1);      // s lives in r
                 launch_spmv<T>(ctx, A, BicgT<T>{v}, which, true, LIST_K);
-                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K, fc);
+                launch_vec(ctx, BicgXR<T>{v, 0.0, 0.0, 0}, which, LIST_K, fc, cm);
             } else if (method == WSPM_CG) {
                 launch_spmv<T>(ctx, A, CgAp<T>{v}, which, true, LIST_K);
-                launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K, fc);   // decides done; CgP skips those
+                launch_vec(ctx, CgXR<T>{v, 0.0}, which, LIST_K, fc, cm);   // decides done; CgP skips those
                 if (pc) {
                     apply(v.r, v.z, 1);
                     launch_vec(ctx, CgPcRZ<T>{v}, which, LIST_K);
@@ -1320,7 +1327,7 @@ struct KrylovRun {
             } else {
                 gmres_cycle();
             }
-            if (compact && !fc) ens_compact(ctx, LIST_K);
+            if (compact && !fc) ens_compact(ctx, LIST_K, cm);
         }
     }
 
@@ -3020,24 +3027,30 @@ static void ens_mixed_solve(wspm_ctx* ctx, bool jac) {
     const wspm_richards_params& P = ctx->prm;
     while (true) {
         const int sweeps = env_int("WSPM_ENS_SWEEPS", ENS_SWEEPS), inner = env_int("WSPM_ENS_INNER", ENS_INNER);
+        // fc: the lists are rebuilt by the finishes before them (the
+        // assembly's finish and every residual finish leave the instances
+        // still refining; the start's finish those whose inner solve runs;
+        // the inner solve's last finish those refined in this sweep)
+        const bool fc = fused_compact(ctx);
         for (int sw = 0; sw < sweeps; ++sw) {
-            ens_compact(ctx, LIST_K);                 // instances still refining (done clear)
+            if (!fc) ens_compact(ctx, LIST_K);        // instances still refining (done clear)
             // the inner solve starts from zero: scaling and solver start in one
             // pass, without the start's SpMV
             KrylovRun<float> run(ctx, P.method, ctx->A32, jac ? ctx->dinv32 : nullptr, ctx->ref_r32, ctx->ref_d32,
                                  ctx->ref_d32, P.inner_tol, 0.0, P.maxiter, P.restart, 0, true);
+            if (fc) run.last_mode = 2;
             if (P.method == WSPM_BICGSTAB)
                 launch_vec(ctx, RefStartE<BicgInit<float>>{{run.v, P.inner_tol, 0.0, P.maxiter}, ctx->ref_r,
-                                                           ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
+                                                           ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K, fc);
             else
                 launch_vec(ctx, RefStartE<CgInit<float>>{{run.v, P.inner_tol, 0.0, P.maxiter}, ctx->ref_r,
-                                                         ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K);
-            ens_compact(ctx, LIST_K);
+                                                         ctx->ref_r32, ctx->ref_d32, 0.0}, 0, LIST_K, fc);
+            if (!fc) ens_compact(ctx, LIST_K);
             run.enqueue(std::min(inner, P.maxiter));
-            ens_compact(ctx, LIST_K, 2);              // the instances refined in this sweep
+            if (!fc) ens_compact(ctx, LIST_K, 2);     // the instances refined in this sweep
             launch_vec(ctx, RefCorrectE{ctx->hB, ctx->ref_d32, 0.0}, 0, LIST_K);
             launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 0}, 0, true,
-                                LIST_K);
+                                LIST_K, nullptr, true, fc);
         }
         poll_ens(ctx, 0);                             // hstatus[0]: instances whose refinement goes on
         if (ctx->hstatus[0] == 0) break;
