@@ -3011,12 +3011,12 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
 // once per ENS_SWEEPS sweeps: the corrections use the true FP64 residual, so
 // an inner solve cut at ENS_INNER iterations only costs another sweep, and
 // instances that are done drop out of the lists on the device.  One sweep of
-// up to three inner iterations per read-back measured fastest on the
-// 256-instance ensemble (with inner tolerance 1e-4: 59.3k instance-steps/s,
-// against 54.3k for two sweeps of two at 1e-6); WSPM_ENS_SWEEPS and
-// WSPM_ENS_INNER override them.
+// up to two inner iterations per read-back measured fastest on the
+// 256-instance ensemble at inner tolerance 1e-4 (64.9k-65.4k instance-steps/s,
+// against 63.9k-64.4k for three, 62.3k for one and 63.6k for four);
+// WSPM_ENS_SWEEPS and WSPM_ENS_INNER override them.
 static constexpr int ENS_SWEEPS = 1;
-static constexpr int ENS_INNER = 3;
+static constexpr int ENS_INNER = 2;
 
 static int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
