@@ -203,7 +203,8 @@ class EnsembleGroups:
     GIL), so that the kernels of different groups run concurrently.
 
     Instances are independent, so every instance's results are bitwise those
-    of any other grouping.  The point of a split is the shape of a time step:
+    of any other grouping whose contexts use the same SpMV layout (each
+    context's autotuner picks its own; WSPM_LANES fixes it).  The point of a split is the shape of a time step:
     the few instances whose Picard loops take the most iterations (stiff, dry
     soils) keep a step going for many short, latency-bound rounds after the
     others have converged.  In a group of their own (``split_slowest``) those

@@ -80,6 +80,9 @@ def _ensemble(monkeypatch, graphs, poll, solver, steps, lists="fused", precision
     monkeypatch.setenv("WSPM_GRAPHS", graphs)
     monkeypatch.setenv("WSPM_POLL", poll)
     monkeypatch.setenv("WSPM_ENS_LIST", lists)
+    # one SpMV layout for every context compared (the autotuner's timed choice
+    # can differ between contexts, and with it the row sum order)
+    monkeypatch.setenv("WSPM_LANES", "8")
     insts = vg_ensemble(n_inst=6, sides=(16, 32, 24, 40, 20, 36), seed=11)
     ens = Ensemble(insts)
     ens.richards_init("vg", solver, "jacobi", precision=precision)
