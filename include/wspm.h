@@ -132,6 +132,18 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems,
                         const int32_t* conn, const double* coords, void* stream,
                         wspm_ctx** out);
 void wspm_ctx_destroy(wspm_ctx* ctx);
+/* Reverse Cuthill-McKee renumbering of a mesh (host only, no GPU call; no
+ * counterpart in the reference, whose unstructured solute meshes,
+ * data/mesh/solute read by src/mesh/loader.py:33-36, run in their file order).
+ * conn as wspm_ctx_create.  Out (host int32): node_new[n_nodes], the new index
+ * of every node (each connected component from a pseudo-peripheral node of
+ * least degree, neighbours by increasing degree, the order reversed);
+ * elem_old[n_elems], the original element at each new position (elements by
+ * their smallest new vertex index, ties in file order); band (int32[4] or
+ * NULL): lower and upper bandwidth of the node graph before and after.
+ * Errors: wspm_last_error(NULL). */
+int32_t wspm_mesh_rcm(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32_t* conn, int32_t* node_new,
+                      int32_t* elem_old, int32_t* band);
 int32_t wspm_set_stream(wspm_ctx* ctx, void* stream);
 /* info (host int64[8]): n_nodes, n_elems, nnz, dim, max_row_len, n_bc, the
  * CSR SpMV's lanes per row, and 1 if the time-step solves of the last
@@ -307,8 +319,9 @@ int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, co
  * wspm_bc_set / wspm_spmv / wspm_soil with global numbering, and
  * wspm_richards_step / wspm_richards_state (concatenated arrays);
  * wspm_assemble_richards, wspm_krylov, wspm_krylov_mixed and
- * wspm_richards_init return WSPM_EINVAL.  The SpMV row width is chosen at
- * creation by timing the candidates (info[6]). */
+ * wspm_richards_init return WSPM_EINVAL.  The SpMV layout follows from the
+ * instances' row lengths (ELL when they allow it); an autotuner times the
+ * launch grid at creation (wspm_ens_shape). */
 int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
                         const int32_t* elem_counts, const int32_t* conn,
                         const double* coords, void* stream, wspm_ctx** out);
@@ -326,6 +339,15 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm,
  * Picard iterations, summed Krylov iterations, all converged, max error. */
 int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylov_iters,
                             int32_t* converged, double* err);
+
+/* Launch shape of an ensemble context (host int32[8]): [0] 1 if the
+ * time-step operator is ELL (0: CSR), [1] its width (ELL) or lanes per row
+ * (CSR), [2] workgroups walking the entry lists, [3] rows per entry, [4]
+ * entries, [5] grids the autotuner timed (0: fixed by WSPM_ENS_GRID), [6]
+ * the chosen grid's SpMV time over the whole operator in nanoseconds, [7] 0.
+ * The format follows from the meshes alone, so results are repeatable; the
+ * tuned grid changes no bit of them. */
+int32_t wspm_ens_shape(const wspm_ctx* ctx, int32_t* out);
 
 /* ---- Direct solve -------------------------------------------------------------
  * x = A^{-1} b for CSR values A on the context's pattern (all dev): the

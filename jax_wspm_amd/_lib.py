@@ -74,6 +74,7 @@ PROTOTYPES = {
     "wspm_last_error": (C.c_char_p, [_P]),
     "wspm_ctx_create": (_I, [_I, _I, _I, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ctx_destroy": (None, [_P]),
+    "wspm_mesh_rcm": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "wspm_set_stream": (_I, [_P, _P]),
     "wspm_ctx_info": (_I, [_P, C.POINTER(C.c_int64)]),
     "wspm_get_pattern": (_I, [_P, _P, _P]),
@@ -102,6 +103,7 @@ PROTOTYPES = {
     "wspm_ens_create": (_I, [_I, _I, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "wspm_ens_richards_init": (_I, [_P, C.POINTER(RichardsParams), _P, _P, _P]),
     "wspm_ens_step_stats": (_I, [_P, _P, _P, _P, _P]),
+    "wspm_ens_shape": (_I, [_P, C.POINTER(_I)]),
     "wspm_domain_integral": (_I, [_P, _P, C.POINTER(_D)]),
     "wspm_direct": (_I, [_P, _P, _P, _P, C.POINTER(_I)]),
     "wspm_direct_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),

@@ -66,6 +66,9 @@ def parse_args(argv=None):
                    help="Test1-3: record the domain integrals of theta and of the exact theta every step")
     p.add_argument("--diagnostics", action="store_true",
                    help="SoluteTest/NitrogenTest: record the largest element Peclet and Courant numbers every step")
+    p.add_argument("--reorder", choices=["none", "rcm"], default="none",
+                   help="renumber the mesh's nodes (reverse Cuthill-McKee) for the GPU; results stay in the "
+                        "mesh's own numbering")
     return p.parse_args(argv)
 
 
@@ -117,6 +120,8 @@ def run(args):
     else:
         cls = RichardsSolver3D if args.test_case == "Test3D" else RichardsSolver
     extra = {}
+    if getattr(args, "reorder", "none") != "none":
+        extra["reorder"] = args.reorder
     if args.mass_balance:
         if args.test_case not in ("Test1", "Test2", "Test3"):
             raise ValueError("--mass-balance needs Test1, Test2 or Test3")

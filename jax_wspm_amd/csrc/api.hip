@@ -657,7 +657,6 @@ struct wspm_ctx {
     KState* hstate_ahead = nullptr;
     bool ell = false;
     bool spmv_ell = false;
-    bool ell_faster = true;         // ensembles: the autotuner timed ELL against the best CSR width
     int asm_kernel = ASM_K_LDS;                   // ASM_K_*
     int prow = MAX_BLOCKS;                        // partials rows
     int lanes = 8;
@@ -711,11 +710,14 @@ struct wspm_ctx {
     bool ens_list_kernel = false;   // WSPM_ENS_LIST=kernel: Krylov lists rebuilt by k_ens_compact only
     int* pact = nullptr;            // [1 + n_tab] entries of instances still iterating Picard
     int ens_grid = 0;               // workgroups of an ensemble launch
+    int ens_grid_cands = 0;         // grids the autotuner timed (0: fixed by WSPM_ENS_GRID)
+    float ens_tune_us = 0.0f;       // the chosen grid's SpMV time over the whole operator
     int ens_compact_every = 1;      // Krylov iterations between rebuilds of the active list
     // boundary node lists for edge terms: [0] inlet (top), [1] outlet (bottom)
     struct EdgeList {
         int cnt = 0;
         int32_t *nodes = nullptr, *slot_prev = nullptr, *slot_next = nullptr;
+        std::string unjoined;   // the first consecutive pair not joined by a mesh edge ("" if none)
     } edges[2];
     // discrete adjoint workspace (allocated on first use)
     struct {
@@ -751,6 +753,7 @@ struct wspm_ctx {
     int* hseq = nullptr;
     int seq = 0;
     bool fast_poll = true;
+    int spin_us = 1000;         // spin on the word this long before blocking on the stream (WSPM_SPIN_US)
     // HIP graphs of whole Picard iterations, replayed while the step's
     // parameters repeat (WSPM_GRAPHS=1; by default every kernel is launched
     // on the stream)
@@ -922,19 +925,24 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
 }
 
 // Wait for the next publication (k_publish / k_ens_status with a sequence
-// word): spin on the pinned word; if the stream drains or a millisecond
+// word): spin on the pinned word; if the stream drains or ctx->spin_us
 // passes, block on the stream instead, after which the word must be there.
+// A word past the one awaited is accepted: a publication enqueued behind the
+// awaited one (a Picard iteration enqueued ahead) may already have run when
+// the host falls back to synchronising the stream, and it writes its state
+// into a mirror of its own.
 void wait_published(wspm_ctx* ctx) {
     const int want = ++ctx->seq;
     volatile int* f = ctx->hseq;
     const auto t0 = std::chrono::steady_clock::now();
-    for (unsigned spin = 1; *f != want; ++spin) {
-        if ((spin & 1023u) != 0) continue;
-        const hipError_t q = hipStreamQuery(ctx->stream);
+    const auto limit = std::chrono::microseconds(ctx->spin_us);
+    for (unsigned spin = 1; (int)((unsigned)*f - (unsigned)want) < 0; ++spin) {
+        if (ctx->spin_us > 0 && (spin & 1023u) != 0) continue;
+        const hipError_t q = ctx->spin_us > 0 ? hipStreamQuery(ctx->stream) : hipSuccess;
         if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
-        if (q == hipSuccess || std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(1)) {
+        if (q == hipSuccess || std::chrono::steady_clock::now() - t0 > limit) {
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            if (*f != want) {
+            if ((int)((unsigned)*f - (unsigned)want) < 0) {
                 ctx->err = "status read-back: sequence word " + std::to_string(*f) + ", expected " +
                            std::to_string(want);
                 ctx->seq = *f;
@@ -1449,6 +1457,45 @@ static void free_ctx(wspm_ctx* ctx) {
     delete ctx;
 }
 
+int32_t wspm_mesh_rcm(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32_t* conn, int32_t* node_new,
+                      int32_t* elem_old, int32_t* band) {
+    if (!conn || !node_new || !elem_old) {
+        g_create_error = "null argument";
+        return WSPM_EINVAL;
+    }
+    try {
+        HostPattern P;
+        std::string err;
+        const int rc = build_pattern(dim, n_nodes, n_elems, conn, P, err);
+        if (rc != WSPM_OK) {
+            g_create_error = err;
+            return rc;
+        }
+        const std::vector<int32_t> perm = rcm_order(P);
+        std::copy(perm.begin(), perm.end(), node_new);
+        // elements by their first vertex in the new order (ties: original order)
+        const int K = dim + 1;
+        std::vector<int32_t> key(n_elems);
+        for (int e = 0; e < n_elems; ++e) {
+            int m = INT32_MAX;
+            for (int c = 0; c < K; ++c) m = std::min(m, perm[conn[(size_t)e * K + c]]);
+            key[e] = m;
+        }
+        std::vector<int32_t> ord(n_elems);
+        for (int e = 0; e < n_elems; ++e) ord[e] = e;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+        std::copy(ord.begin(), ord.end(), elem_old);
+        if (band) {
+            bandwidth(P, {}, band[0], band[1]);
+            bandwidth(P, perm, band[2], band[3]);
+        }
+    } catch (const std::exception& ex) {
+        g_create_error = ex.what();
+        return WSPM_ENOMEM;
+    }
+    return WSPM_OK;
+}
+
 int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32_t* conn,
                         const double* coords, void* stream, wspm_ctx** out) {
     if (!out || !conn || !coords) {
@@ -1547,6 +1594,8 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         {
             const char* pe = std::getenv("WSPM_POLL");
             ctx->fast_poll = !(pe && std::string(pe) == "sync");
+            const char* su = std::getenv("WSPM_SPIN_US");
+            if (su) ctx->spin_us = std::max(0, std::atoi(su));
         }
         ctx->H = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
         ctx->H32 = dalloc<double>(ctx, (size_t)MAX_RESTART * (MAX_RESTART + 1));
@@ -1638,6 +1687,9 @@ static void launch_edge_list(wspm_ctx* ctx, int list, int kind, double flux, dou
                              int kstride, const double* th, double* A, double* b, double* dinv) {
     const auto& L = ctx->edges[list];
     if (L.cnt < 2) return;
+    if ((kind == EDGE_CAUCHY || kind == EDGE_FREEDRAIN_N) && !L.unjoined.empty())
+        throw ApiFail{WSPM_EINVAL, "consecutive edge nodes " + L.unjoined +
+                                       " are not joined by a mesh edge (matrix edge terms need one)"};
     EdgeArgs e{};
     e.cnt = L.cnt;
     e.nodes = L.nodes;
@@ -2018,8 +2070,8 @@ static size_t op_cap(const wspm_ctx* ctx) {
 
 // ELL for the time steps: 2D register assembly, rows of 7 or 8 entries, at
 // most 25 % padding, and a Krylov solve with no preconditioner or Jacobi
-// (the general preconditioners and the direct solver read CSR); on an
-// ensemble, only if the autotuner measured it faster.  WSPM_SPMV=csr keeps CSR.
+// (the general preconditioners and the direct solver read CSR).
+// WSPM_SPMV=csr keeps CSR.
 static bool ell_shape_ok(const wspm_ctx* ctx) {
     const char* env = std::getenv("WSPM_SPMV");
     if (env && std::string(env) == "csr") return false;
@@ -2029,7 +2081,7 @@ static bool ell_shape_ok(const wspm_ctx* ctx) {
 }
 
 static bool ell_ok(const wspm_ctx* ctx, const wspm_richards_params* prm) {
-    if (!ell_shape_ok(ctx) || (ctx->tab && !ctx->ell_faster)) return false;
+    if (!ell_shape_ok(ctx)) return false;
     return prm->method != WSPM_DIRECT && prm->precond <= WSPM_PRECOND_JACOBI;
 }
 
@@ -2464,12 +2516,16 @@ int32_t wspm_edge_set(wspm_ctx* ctx, int32_t list, int32_t n, const int32_t* nod
         if (seen[nd[k]]) return fail(ctx, WSPM_EINVAL, "edge node listed twice");
         seen[nd[k]] = 1;
     }
+    // a pair not joined by a mesh edge has no matrix entries in the pattern:
+    // the list then takes right-hand-side terms only (Neumann, free drainage
+    // of water), and the matrix terms (Cauchy, free drainage of nitrogen)
+    // refuse it
+    std::string unjoined;
     for (int k = 0; k < n; ++k) {
         if (k > 0) sp[k] = pos(nd[k], nd[k - 1]);
         if (k + 1 < n) sn[k] = pos(nd[k], nd[k + 1]);
-        if ((k > 0 && sp[k] < 0) || (k + 1 < n && sn[k] < 0))
-            return fail(ctx, WSPM_EINVAL, "consecutive edge nodes " + std::to_string(nd[k]) +
-                                              " are not joined by a mesh edge");
+        if (unjoined.empty() && k + 1 < n && sn[k] < 0)
+            unjoined = std::to_string(nd[k]) + " and " + std::to_string(nd[k + 1]);
     }
     API_BEGIN
     auto& L = ctx->edges[list];
@@ -2484,6 +2540,7 @@ int32_t wspm_edge_set(wspm_ctx* ctx, int32_t list, int32_t n, const int32_t* nod
         HIPCHK(hipMemcpy(L.slot_next, sn.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     L.cnt = n;
+    L.unjoined = unjoined;
     return WSPM_OK;
     API_END
 }
@@ -2784,16 +2841,30 @@ static constexpr int ENS_ROWS = BLOCK;
 // Workgroups of an ensemble launch (they stride over the listed entries).
 static constexpr int ENS_GRID = 2048;
 
-// Time rep launches of the plain SpMV at each candidate row width and keep
-// the fastest (ties: the narrower).  The values are zeros: timing only.
-static void ens_tune_lanes(wspm_ctx* ctx) {
+// Ensemble launch shapes.  The operator format follows from the meshes
+// alone: ELL when the rows allow it (ell_shape_ok), else CSR at the row
+// width wspm_ctx_create chose (WSPM_LANES overrides it).  Every context that
+// holds the same instances therefore sums each row in the same order, and
+// results are repeatable run after run and across groupings of the
+// instances.  The tuner times only what changes no bit of the results: the
+// number of workgroups that walk the entry lists.  Every listed entry (256
+// rows of one instance) is worked by one workgroup and writes its own
+// partial-row column, so the grid decides who works an entry, not what is
+// summed.  Candidates 256 ... 8192 workgroups (at most one per entry),
+// timed on the whole operator (zero values) with the time-step format: 2
+// warm-up and 10 timed launches each, the fastest kept (ties: the smaller
+// grid).  WSPM_ENS_GRID fixes the grid instead.
+static void ens_autotune(wspm_ctx* ctx) {
     const char* env = std::getenv("WSPM_LANES");
     if (env) {
         const int l = std::atoi(env);
-        if (l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64) {
-            ctx->lanes = l;
-            return;
-        }
+        if (l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64) ctx->lanes = l;
+    }
+    const char* g = std::getenv("WSPM_ENS_GRID");
+    if (g && std::atoi(g) > 0) {
+        ctx->ens_grid = std::max(1, std::min(std::atoi(g), ctx->n_tab));
+        ctx->ens_grid_cands = 0;
+        return;
     }
     double* av = nullptr;
     double* xy = nullptr;
@@ -2810,38 +2881,32 @@ static void ens_tune_lanes(wspm_ctx* ctx) {
         HIPCHK(hipMemsetAsync(xy, 0, 2 * (size_t)ctx->n * sizeof(double), ctx->stream));
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
+        const bool ell = ell_shape_ok(ctx);
+        if (ell) ensure_ell(ctx);
+        const EllScope scope(ctx, ell);
         const EpiPlain<double> epi{xy, xy + ctx->n};
-        const int cands[3] = {4, 8, 16};
-        int best = ctx->lanes;
+        int best = std::min(ENS_GRID, ctx->n_tab), cands = 0;
         float best_ms = 1e30f;
-        for (int c : cands) {
-            ctx->lanes = c;
-            for (int r = 0; r < 3; ++r) launch_spmv<double>(ctx, av, epi, 0);
+        for (int grid = 256; grid <= 8192; grid *= 2) {
+            const int gg = std::min(grid, ctx->n_tab);
+            if (grid > 256 && gg != grid) break;   // the table has fewer entries
+            ctx->ens_grid = gg;
+            for (int r = 0; r < 2; ++r) launch_spmv<double>(ctx, av, epi, 0);
             HIPCHK(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < 20; ++r) launch_spmv<double>(ctx, av, epi, 0);
+            for (int r = 0; r < 10; ++r) launch_spmv<double>(ctx, av, epi, 0);
             HIPCHK(hipEventRecord(e1, ctx->stream));
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0.0f;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ++cands;
             if (ms < best_ms * 0.98f) {
                 best_ms = ms;
-                best = c;
+                best = gg;
             }
         }
-        ctx->lanes = best;
-        // the ELL layout against the best CSR width (same zero operator)
-        if (ell_shape_ok(ctx)) {
-            ensure_ell(ctx);
-            const EllScope scope(ctx, true);
-            for (int r = 0; r < 3; ++r) launch_spmv<double>(ctx, av, epi, 0);
-            HIPCHK(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < 20; ++r) launch_spmv<double>(ctx, av, epi, 0);
-            HIPCHK(hipEventRecord(e1, ctx->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            ctx->ell_faster = ms < best_ms;
-        }
+        ctx->ens_grid = best;
+        ctx->ens_grid_cands = cands;
+        ctx->ens_tune_us = best_ms * 100.0f;   // per launch, microseconds
     } catch (const Err&) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -2937,16 +3002,13 @@ int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
         {
             const char* ce = std::getenv("WSPM_ENS_COMPACT");
             ctx->ens_compact_every = std::max(1, ce ? std::atoi(ce) : 1);
-            const char* g = std::getenv("WSPM_ENS_GRID");
-            const int want = g ? std::atoi(g) : ENS_GRID;
-            ctx->ens_grid = std::max(1, std::min(want > 0 ? want : ENS_GRID, ctx->n_tab));
         }
         void* hp = nullptr;
         HIPCHK(hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault));
         ctx->host_allocs.push_back(hp);
         ctx->hstatus = static_cast<int*>(hp);
         std::memset(ctx->hstatus, 0, 4 * sizeof(int));
-        ens_tune_lanes(ctx);
+        ens_autotune(ctx);
         HIPCHK(hipStreamSynchronize(ctx->stream));
     } catch (const Err&) {
         g_create_error = ctx->err;
@@ -3192,6 +3254,20 @@ int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylo
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WSPM_OK;
     API_END
+}
+
+int32_t wspm_ens_shape(const wspm_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return WSPM_EINVAL;
+    if (!ctx->tab) return fail(const_cast<wspm_ctx*>(ctx), WSPM_EINVAL, "not an ensemble context");
+    out[0] = ell_shape_ok(ctx) ? 1 : 0;
+    out[1] = ell_shape_ok(ctx) ? ctx->max_row : ctx->lanes;
+    out[2] = ctx->ens_grid;
+    out[3] = ENS_ROWS;
+    out[4] = ctx->n_tab;
+    out[5] = ctx->ens_grid_cands;
+    out[6] = (int32_t)std::lround(ctx->ens_tune_us * 1000.0f);
+    out[7] = 0;
+    return WSPM_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -183,6 +183,18 @@ class Ensemble:
         check(self.lib.wspm_ctx_info(self.h, info), self.h)
         return "ell" if info[7] else "csr"
 
+    @property
+    def shape(self) -> dict:
+        """Launch shape (wspm_ens_shape): operator format and width, the
+        tuned number of workgroups walking the entry lists, rows per entry,
+        entries, grids timed by the autotuner and the chosen grid's SpMV time
+        over the whole operator (us)."""
+        out = (C.c_int32 * 8)()
+        check(self.lib.wspm_ens_shape(self.h, out), self.h)
+        return {"format": "ell" if out[0] else "csr", "width": int(out[1]), "grid": int(out[2]),
+                "rows_per_entry": int(out[3]), "entries": int(out[4]), "grids_timed": int(out[5]),
+                "tuned_spmv_us": out[6] / 1000.0}
+
     def timing(self, on: bool, fp32: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
         mixed-precision inner solves)."""
@@ -202,9 +214,10 @@ class EnsembleGroups:
     stream and stepped by its own host thread (the library's calls release the
     GIL), so that the kernels of different groups run concurrently.
 
-    Instances are independent, so every instance's results are bitwise those
-    of any other grouping whose contexts use the same SpMV layout (each
-    context's autotuner picks its own; WSPM_LANES fixes it).  The point of a split is the shape of a time step:
+    Instances are independent, and every context derives its SpMV layout from
+    the instances' row lengths (its autotuner only times the launch grid,
+    which changes no bit), so every instance's results are bitwise those of
+    any other grouping.  The point of a split is the shape of a time step:
     the few instances whose Picard loops take the most iterations (stiff, dry
     soils) keep a step going for many short, latency-bound rounds after the
     others have converged.  In a group of their own (``split_slowest``) those

@@ -18,6 +18,7 @@ exact Tracy theta at the step's recorded time (``wspm_domain_integral``), and
 from __future__ import annotations
 
 import time
+from dataclasses import replace
 from typing import Any, Dict, Optional, Union
 
 import numpy as np
@@ -26,6 +27,14 @@ import torch
 from .config import SimulationConfig
 from .device import Context
 from .problems import Problem, from_config
+from .reorder import renumbering
+
+
+def _renumbered(pb: Problem, rn) -> Problem:
+    """The problem on the renumbered mesh (boundary lists in their order)."""
+    return replace(pb, points=rn.points(pb.points), elements=rn.elements(pb.elements),
+                   bc_lists=[(rn.node_ids(nd), np.asarray(v)) for nd, v in pb.bc_lists],
+                   h0=rn.nodal(pb.h0), Ks_elem=rn.elemental(pb.Ks_elem))
 
 
 def adapt_time_step(dt: float, iters: int, m_it=5, M_it=7, amp=2.0, red=0.5) -> float:
@@ -44,14 +53,20 @@ class RichardsSolver:
     step plus the last."""
 
     def __init__(self, config: Union[SimulationConfig, Problem], mesh_dir: Optional[str] = None,
-                 history: Union[str, int] = "all", mass_balance: bool = False):
-        self.problem = config if isinstance(config, Problem) else from_config(config, mesh_dir)
+                 history: Union[str, int] = "all", mass_balance: bool = False, reorder: Optional[str] = None):
+        """reorder "rcm": the GPU works on the reverse Cuthill-McKee
+        renumbering of the mesh (jax_wspm_amd.reorder); inputs and results
+        stay in the problem's own numbering."""
+        pb0 = config if isinstance(config, Problem) else from_config(config, mesh_dir)
+        self.renum = renumbering(reorder, pb0.elements, len(pb0.points))
+        self.problem = pb0 if self.renum is None else _renumbered(pb0, self.renum)
         self.config = config
         self.history = history
         pb = self.problem
         self.ctx = Context(pb.points, pb.elements)
         self.ctx.set_dirichlet(pb.bc_lists)
-        self.points = pb.points
+        self.points = pb0.points
+        self.elements = pb0.elements
         self.mesh_info = pb.mesh_info
         self.mass_balance = bool(mass_balance)
         if self.mass_balance and not (pb.name in ("Test1", "Test2", "Test3") and pb.soil_model == "gardner"):
@@ -68,6 +83,11 @@ class RichardsSolver:
         d = alpha0 * (ts - tr) / ks
         _, S = exact_solution(pb.points[:, 0], pb.points[:, 1], t, int(pb.name[-1]), -hd, alpha0, eps0, d)
         return torch.as_tensor((ts - tr) * S + tr, dtype=torch.float64, device=self.ctx.device)
+
+    def _host(self, x: torch.Tensor) -> np.ndarray:
+        """A nodal GPU array on the host, in the problem's own numbering."""
+        a = x.cpu().numpy()
+        return a if self.renum is None else self.renum.nodal_back(a)
 
     def init_state(self):
         pb = self.problem
@@ -118,22 +138,22 @@ class RichardsSolver:
                 rec["mass_balance_exact"].append(self.ctx.domain_integral(self._exact_theta(t)))
             if self._keep(k):
                 h, th, thn = self.ctx.richards_state()
-                rec["pressure_head"].append(h.cpu().numpy())
-                rec["theta"].append((thn if pb.theta_at_start else th).cpu().numpy())
+                rec["pressure_head"].append(self._host(h))
+                rec["theta"].append(self._host(thn if pb.theta_at_start else th))
                 last = k
             t += dt_new
             dt = dt_new
             k += 1
         if k > 0 and last != k - 1:
             h, th, thn = self.ctx.richards_state()
-            rec["pressure_head"].append(h.cpu().numpy())
-            rec["theta"].append((thn if pb.theta_at_start else th).cpu().numpy())
+            rec["pressure_head"].append(self._host(h))
+            rec["theta"].append(self._host(thn if pb.theta_at_start else th))
         torch.cuda.synchronize()
         sim = time.perf_counter() - t0
         out: Dict[str, Any] = {key: np.array(v) for key, v in rec.items()}
         out.update({
-            "points": pb.points,
-            ("tetrahedra" if pb.dim == 3 else "triangles"): pb.elements,
+            "points": self.points,
+            ("tetrahedra" if pb.dim == 3 else "triangles"): self.elements,
             "simulation_time": sim,
             "mesh_info": pb.mesh_info,
             "config": self.config,

@@ -37,6 +37,7 @@ import torch
 from .config import SimulationConfig
 from .device import Context
 from .mesh import load_and_validate_mesh
+from .reorder import renumbering
 
 
 def transport_boundary_nodes(points: np.ndarray, test_case: str):
@@ -73,15 +74,26 @@ class _TransportBase:
     freedrain = True
 
     def __init__(self, config: SimulationConfig, mesh_dir: Optional[str] = None, points=None, triangles=None,
-                 history: Any = "all", diagnostics: bool = False):
+                 history: Any = "all", diagnostics: bool = False, reorder: Optional[str] = None):
+        """reorder "rcm": the GPU works on the reverse Cuthill-McKee
+        renumbering of the mesh (jax_wspm_amd.reorder); the boundary lists
+        keep the reference's order and results are in the mesh's own
+        numbering."""
         self.config = config
         self.history = history
         self.diagnostics = bool(diagnostics)
         self.points, self.triangles, self.mesh_info = _mesh(config, mesh_dir, points, triangles)
         self.inlet, self.outlet = transport_boundary_nodes(self.points, config.test_case)
-        self.ctx = Context(self.points, self.triangles)
-        self.ctx.edge_set("inlet", self.inlet)
-        self.ctx.edge_set("outlet", self.outlet)
+        self.renum = renumbering(reorder, self.triangles, len(self.points))
+        rn = self.renum
+        if rn is None:
+            self.ctx = Context(self.points, self.triangles)
+            self.ctx.edge_set("inlet", self.inlet)
+            self.ctx.edge_set("outlet", self.outlet)
+        else:
+            self.ctx = Context(rn.points(self.points), rn.elements(self.triangles))
+            self.ctx.edge_set("inlet", rn.node_ids(self.inlet))
+            self.ctx.edge_set("outlet", rn.node_ids(self.outlet))
         self.ctx.richards_edges(self.neumann_flux, freedrain=self.freedrain)
         self.phi = np.asarray(config.van_genuchten.to_array(), dtype=np.float64)
         s = config.solver
@@ -102,6 +114,11 @@ class _TransportBase:
         x, _ = self.ctx.krylov(s.solver_type, A, b, x0, dinv=dinv if s.precond_type == "jacobi" else None,
                                tol=s.tol, maxiter=s.maxiter, restart=s.restart)
         return x
+
+    def _host(self, x) -> np.ndarray:
+        """A nodal GPU array on the host, in the mesh's own numbering."""
+        a = x.cpu().numpy()
+        return a if self.renum is None else self.renum.nodal_back(a)
 
     def _keep(self, k: int, last: bool) -> bool:
         if self.history == "all":
@@ -160,7 +177,7 @@ class NitrogenSolver(_TransportBase):
             last = self._done(t + dt, k + 1) or (max_steps is not None and k + 1 >= max_steps)
             if self._keep(k, last):
                 for key, v in zip(keys, (h, fields[6], new[0], new[1], new[2])):
-                    out[key].append(v.cpu().numpy())
+                    out[key].append(self._host(v))
             out["times"].append(t)
             out["iterations"].append(it)
             out["errors"].append(st["err"])
@@ -205,7 +222,7 @@ class CoupledSolver(_TransportBase):
             last = self._done(t + dt, k + 1) or (max_steps is not None and k + 1 >= max_steps)
             if self._keep(k, last):
                 for key, v in zip(keys, (h, fields[6], c_new)):
-                    out[key].append(v.cpu().numpy())
+                    out[key].append(self._host(v))
             out["times"].append(t)
             out["iterations"].append(it)
             out["errors"].append(st["err"])

@@ -6,7 +6,8 @@ box does not have):
     python tests/golden/make_golden.py /root/reference
 
 Everything written is data the reference itself holds (mesh checksums, rows of
-its published result CSVs, slices of its committed 3D result NPZ files), so
+its published result CSVs, slices of its committed 3D result NPZ files, two of
+its unstructured solute meshes), so
 the tests can pin the oracle without the reference present.
 """
 from __future__ import annotations
@@ -65,6 +66,16 @@ def main(ref: Path):
                                                    "l2_saturation", "linf_saturation",
                                                    "l2_relative_pressure", "l2_relative_saturation")})
     (HERE / "tracy_metrics.json").write_text(json.dumps(rows, indent=1))
+
+    # the unstructured solute meshes of SoluteTest (data/mesh/solute, read by
+    # src/mesh/loader.py:33-36): points, and elements shifted to 0-based as
+    # the loader does; 25 is the CLI's default --mesh-size
+    pinns = {}
+    for tag in ("25", "1024"):
+        pinns[f"points_{tag}"] = _read(mesh_dir / "solute" / f"p_Pinns_{tag}.csv").astype(np.float64)
+        pinns[f"elements_{tag}"] = (_read(mesh_dir / "solute" / f"t_Pinns_{tag}.csv").astype(np.int64) - 1
+                                    ).astype(np.int32)
+    np.savez_compressed(HERE / "pinns_meshes.npz", **pinns)
 
     for solver in ("bicgstab", "gmres"):
         z = np.load(ref / "results3D" / f"numerical_results_Test3D_{solver}_{solver}_none_mesh25.npz",

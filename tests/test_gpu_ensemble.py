@@ -141,15 +141,12 @@ def test_mixed_ensemble_within_1e8_of_fp64(solver):
             assert rel_l2(h32[k][i], o["pressure_head"][k]) <= 1e-8, (i, k)
 
 
-def test_grouped_ensemble_equals_single_context(monkeypatch):
+def test_grouped_ensemble_equals_single_context():
     """EnsembleGroups (instances split over ensemble contexts on separate
-    streams, stepped concurrently by host threads) against one context:
-    every instance's h, theta and per-step Picard / Krylov counts bitwise
-    equal.  Each context's SpMV layout is fixed (CSR, 8 lanes per row): the
-    autotuner times the layouts per context, and on small instances the
-    timings are close enough for contexts to pick different ones, whose row
-    sums run in another order."""
-    monkeypatch.setenv("WSPM_LANES", "8")
+    streams, stepped concurrently by host threads) against one context, on
+    the default (autotuned) launch shapes: every instance's h, theta and
+    per-step Picard / Krylov counts bitwise equal.  Each context derives its
+    SpMV layout from the row lengths and tunes only its launch grid."""
     from jax_wspm_amd.ensemble import Ensemble, EnsembleGroups, vg_ensemble
     insts = vg_ensemble(n_inst=7, sides=(16, 40, 24, 32, 20, 36), seed=9)
     print("[case] grouped ensemble", flush=True)
@@ -168,3 +165,36 @@ def test_grouped_ensemble_equals_single_context(monkeypatch):
         assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
     one.close()
     grp.close()
+
+
+def _ens_run(insts, steps, solver="bicgstab", precision="fp64"):
+    from jax_wspm_amd.ensemble import Ensemble
+    ens = Ensemble(insts)
+    ens.richards_init("vg", solver, "jacobi", precision=precision)
+    its = [ens.step(1e-4, per_instance=True)["picard_iters_i"].copy() for _ in range(steps)]
+    h = ens.state()[0].cpu().numpy().copy()
+    shape = ens.shape
+    ens.close()
+    return h, np.array(its), shape
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ensemble_repeatable_across_runs_and_grids(monkeypatch, precision):
+    """Fresh contexts of the same instances, each autotuned on its own, and
+    contexts with the launch grid forced to 1, 3 and 64 workgroups: bitwise
+    the same trajectories and Picard counts (the grid decides which
+    workgroup works an entry, never what is summed)."""
+    from jax_wspm_amd.ensemble import vg_ensemble
+    insts = vg_ensemble(n_inst=5, sides=(16, 40, 24, 32, 48), seed=4)
+    print(f"[case] ensemble repeatable {precision}", flush=True)
+    monkeypatch.delenv("WSPM_ENS_GRID", raising=False)
+    h0, it0, sh0 = _ens_run(insts, 4, precision=precision)
+    assert sh0["format"] == "ell" and sh0["grids_timed"] >= 1 and 1 <= sh0["grid"] <= sh0["entries"]
+    h1, it1, _ = _ens_run(insts, 4, precision=precision)
+    assert np.array_equal(it0, it1) and np.array_equal(h0, h1)
+    for g in ("1", "3", "64"):
+        monkeypatch.setenv("WSPM_ENS_GRID", g)
+        h2, it2, sh2 = _ens_run(insts, 4, precision=precision)
+        assert sh2["grid"] == min(int(g), sh2["entries"]) and sh2["grids_timed"] == 0
+        assert np.array_equal(it0, it2), g
+        assert np.array_equal(h0, h2), g

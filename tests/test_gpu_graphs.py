@@ -80,9 +80,6 @@ def _ensemble(monkeypatch, graphs, poll, solver, steps, lists="fused", precision
     monkeypatch.setenv("WSPM_GRAPHS", graphs)
     monkeypatch.setenv("WSPM_POLL", poll)
     monkeypatch.setenv("WSPM_ENS_LIST", lists)
-    # one SpMV layout for every context compared (the autotuner's timed choice
-    # can differ between contexts, and with it the row sum order)
-    monkeypatch.setenv("WSPM_LANES", "8")
     insts = vg_ensemble(n_inst=6, sides=(16, 32, 24, 40, 20, 36), seed=11)
     ens = Ensemble(insts)
     ens.richards_init("vg", solver, "jacobi", precision=precision)
@@ -158,5 +155,26 @@ def test_picard_iteration_ahead_bitwise(monkeypatch, case, kw):
     ref_h, ref_it = _single_ahead(monkeypatch, "0", dts, **kw)
     h, it = _single_ahead(monkeypatch, "1", dts, **kw)
     assert it == ref_it
+    for k in range(len(dts)):
+        assert np.array_equal(h[k], ref_h[k]), k
+
+
+@pytest.mark.parametrize("case,kw,dts", [
+    ("default", {}, [1e-4] * 3 + [5e-4, 1e-4]),
+    # many BiCGSTAB iterations per Picard iteration: the chunk the host
+    # enqueues first is too short, so the solve is continued after the read-back
+    ("long_solves", {"tol": 1e-14, "maxiter": 400}, [2e-2, 5e-2, 1e-1])])
+def test_picard_ahead_blocking_readback_bitwise(monkeypatch, case, kw, dts):
+    """The host blocks on the stream instead of spinning (WSPM_SPIN_US=0), so
+    the stream drains past the first Picard iteration's publication and the
+    second iteration, enqueued ahead, has published too by the time the host
+    reads the sequence word: the read-back must take the first iteration's
+    state from its own mirror and go on.  Bitwise equal to the plain order."""
+    print(f"[case] picard ahead, blocking read-back, {case}", flush=True)
+    monkeypatch.setenv("WSPM_SPIN_US", "0")
+    ref_h, ref_it = _single_ahead(monkeypatch, "0", dts, **kw)
+    h, it = _single_ahead(monkeypatch, "1", dts, **kw)
+    assert it == ref_it
+    assert any(p >= 2 for p, _, _ in it)      # the gate opened
     for k in range(len(dts)):
         assert np.array_equal(h[k], ref_h[k]), k

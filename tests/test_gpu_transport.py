@@ -113,14 +113,24 @@ def test_solute_system_consistent_mass_matches_oracle():
     assert np.abs(b.cpu().numpy() - b_o).max() <= 1e-12 * np.abs(b_o).max()
 
 
-def test_edge_list_must_follow_mesh_edges():
+def test_edge_list_matrix_terms_need_mesh_edges():
+    """A boundary list whose consecutive nodes are not joined by a mesh edge
+    (the reference's bottom lists of the unstructured solute meshes, in index
+    order) is accepted for right-hand-side terms; the matrix terms (Cauchy,
+    free drainage of nitrogen) have no pattern entries for it and refuse it.
+    A repeated node is refused at once."""
     from jax_wspm_amd._lib import WspmError
     pts, tri = _mesh(4)
+    n = len(pts)
     ctx = _ctx(pts, tri)
     with pytest.raises(WspmError):
-        ctx.edge_set("inlet", [0, 2])          # not joined by a mesh edge
-    with pytest.raises(WspmError):
         ctx.edge_set("inlet", [0, 1, 0])       # repeated node
+    ctx.edge_set("inlet", [0, 2])              # not joined by a mesh edge
+    h = -1.3 - 0.4 * pts[:, 1]
+    gf = ctx.transport_fields(PHI, h, 0.5, 0.1, 0.0)
+    with pytest.raises(WspmError, match="not joined"):
+        ctx.assemble_transport(gf, np.zeros(n), 1e-2, lumped=False, cauchy=(-0.1, 1.0))
+    ctx.assemble_transport(gf, np.zeros(n), 1e-2, lumped=False)   # no edge term: fine
     ctx.close()
 
 
