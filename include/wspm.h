@@ -205,12 +205,18 @@ int32_t wspm_richards_state(wspm_ctx* ctx, double* h, double* theta, double* the
  * enabled, the SpMV launches of the Krylov solvers are bracketed by HIP events
  * on the context's stream, and each launch records on the device whether it
  * did its work or returned early (solve already converged).  on: 0 off,
- * 1 the FP64 SpMV launches, 2 the FP32 ones (mixed precision inner solves). */
+ * 1 the FP64 SpMV launches, 2 the FP32 ones (mixed precision inner solves),
+ * plus 4: the assembly launches as well (wspm_timing_read_asm). */
 int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
 /* Number of SpMV launches that did their work since enable/last read, their
  * summed event-to-event duration (ms) and the rows they worked on (rows may
  * be NULL; an ensemble launch skips the rows of converged instances). */
 int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
+/* With on & 4 in wspm_timing_enable, the assembly launches (a Picard
+ * iteration's fused assembly + solver start) are bracketed by HIP events
+ * too; this reads them as wspm_timing_read does (launches that assembled no
+ * row, e.g. a Picard iteration enqueued ahead and not needed, are left out). */
+int32_t wspm_timing_read_asm(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
 
 /* Transport (2D) -------------------------------------------------------------
  * Boundary node lists for edge terms, in the reference's list order: an edge

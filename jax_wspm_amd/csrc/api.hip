@@ -694,6 +694,10 @@ struct wspm_ctx {
     // timing
     bool timing = false;
     bool timing_fp32 = false;       // time the FP32 SpMV launches instead of the FP64 ones
+    bool timing_asm = false;        // also time the assembly launches (separate events and slots)
+    int* asm_tflags = nullptr;      // [TIMING_SLOTS] rows each timed assembly launch worked on
+    std::vector<hipEvent_t> asm_ev;
+    size_t asm_ev_used = 0;
     int* tflags = nullptr;          // [TIMING_SLOTS] rows each timed launch worked on (0: skipped)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -1450,6 +1454,7 @@ static void free_ctx(wspm_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     else (void)hipDeviceSynchronize();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->asm_ev) (void)hipEventDestroy(e);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
     if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
     for (void* p : ctx->allocs) (void)hipFree(p);
@@ -1755,8 +1760,25 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
         ki.jacobi = dinv != nullptr;
     }
     int grid = 0;
+    // benchmark timing of the assembly: events recorded around the launch on
+    // the context's stream, and the rows the launch worked on
+    const bool timed = ctx->timing && ctx->timing_asm && ctx->asm_ev_used / 2 < TIMING_SLOTS;
+    hipEvent_t ae0 = nullptr, ae1 = nullptr;
+    if (timed) {
+        while (ctx->asm_ev.size() < ctx->asm_ev_used + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->asm_ev.push_back(e);
+        }
+        ae0 = ctx->asm_ev[ctx->asm_ev_used];
+        ae1 = ctx->asm_ev[ctx->asm_ev_used + 1];
+        a.tflag = ctx->asm_tflags + ctx->asm_ev_used / 2;
+        ctx->asm_ev_used += 2;
+        HIPCHK(hipEventRecord(ae0, ctx->stream));
+    }
     HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream,
                            ctx->asm_kernel));
+    if (timed) HIPCHK(hipEventRecord(ae1, ctx->stream));
     if (mode == ASM_INIT_NONE) return;
     KWork<double>& w = ensure_work<double>(ctx, 0);
     KVec<double> v = make_kvec<double>(ctx, w, dinv, b, h_m, x);
@@ -2459,11 +2481,42 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on) {
     if (!ctx) return WSPM_EINVAL;
     API_BEGIN
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (on < 0 || on > 7) return fail(ctx, WSPM_EINVAL, "timing: 0..7 (1 FP64 SpMV, 2 FP32 SpMV, +4 assembly)");
     if (on && !ctx->tflags) ctx->tflags = dalloc<int>(ctx, TIMING_SLOTS);
     if (on) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
+    if ((on & 4) && !ctx->asm_tflags) ctx->asm_tflags = dalloc<int>(ctx, TIMING_SLOTS);
+    if (on & 4) HIPCHK(hipMemset(ctx->asm_tflags, 0, TIMING_SLOTS * sizeof(int)));
     ctx->timing = on != 0;
-    ctx->timing_fp32 = on == 2;
+    ctx->timing_fp32 = (on & 3) == 2;
+    ctx->timing_asm = (on & 4) != 0;
     ctx->ev_used = 0;
+    ctx->asm_ev_used = 0;
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_timing_read_asm(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows) {
+    if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
+    API_BEGIN
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t pairs = ctx->asm_ev_used / 2;
+    std::vector<int> flags(pairs, 0);
+    if (pairs) HIPCHK(hipMemcpy(flags.data(), ctx->asm_tflags, pairs * sizeof(int), hipMemcpyDeviceToHost));
+    double tot = 0.0;
+    int64_t worked = 0, nrows = 0;
+    for (size_t k = 0; k < pairs; ++k) {
+        if (!flags[k]) continue;
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->asm_ev[2 * k], ctx->asm_ev[2 * k + 1]));
+        tot += ms;
+        ++worked;
+        nrows += flags[k];
+    }
+    *launches = worked;
+    *total_ms = tot;
+    if (rows) *rows = nrows;
+    ctx->asm_ev_used = 0;
+    if (ctx->asm_tflags) HIPCHK(hipMemset(ctx->asm_tflags, 0, TIMING_SLOTS * sizeof(int)));
     return WSPM_OK;
     API_END
 }

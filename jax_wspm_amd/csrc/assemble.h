@@ -99,6 +99,7 @@ struct AsmArgs {
     const KState* ens_st;
     int n_tab;
     int ens_grid;
+    int* tflag;              // nullable: the benchmark's timing slot, receives the rows this launch assembled
 };
 
 // Vectors written by the fused Krylov start (FP64).

@@ -186,6 +186,7 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     const int r0 = blockIdx.x * R;
     const int r1 = min(a.n, r0 + R);
     const int nrows = r1 - r0;
+    if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
     const int base = a.row_ptr[r0];
     const int wlen = a.row_ptr[r1] - base;
     const int q0 = a.inc_ptr[r0];
@@ -536,6 +537,7 @@ template <int D, int MAXR, int MODE, int NBR>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
     if (a.tab == nullptr) {
         if (a.gate != nullptr && *a.gate == 0) return;   // uniform: a Picard iteration enqueued ahead, not needed
+        if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
         const int i = blockIdx.x * BLOCK + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
         if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
@@ -549,6 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
         const int t = a.act ? a.act[1 + v] : v;
         const int4 tb = a.tab[t];
         if (a.ens_st[tb.z].pdone) continue;
+        if (a.tflag != nullptr && threadIdx.x == 0) atomicAdd(a.tflag, tb.y - tb.x);
         const int i = tb.x + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
         if (i < tb.y) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);

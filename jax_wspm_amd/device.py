@@ -286,10 +286,19 @@ class Context:
         check(self.lib.wspm_ctx_info(self.h, info), self.h)
         return "ell" if info[7] else "csr"
 
-    def timing(self, on: bool, fp32: bool = False):
+    def timing(self, on: bool, fp32: bool = False, assembly: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
-        mixed-precision inner solves)."""
-        self._ck(self.lib.wspm_timing_enable(self.h, (2 if fp32 else 1) if on else 0))
+        mixed-precision inner solves); assembly=True: the assembly launches
+        too (timing_read_asm)."""
+        self._ck(self.lib.wspm_timing_enable(self.h, ((2 if fp32 else 1) | (4 if assembly else 0)) if on else 0))
+
+    def timing_read_asm(self):
+        """(working assembly launches, their summed duration in ms, rows they assembled)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)
 
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""

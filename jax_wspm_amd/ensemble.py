@@ -195,10 +195,19 @@ class Ensemble:
                 "rows_per_entry": int(out[3]), "entries": int(out[4]), "grids_timed": int(out[5]),
                 "tuned_spmv_us": out[6] / 1000.0}
 
-    def timing(self, on: bool, fp32: bool = False):
+    def timing(self, on: bool, fp32: bool = False, assembly: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the
-        mixed-precision inner solves)."""
-        self._ck(self.lib.wspm_timing_enable(self.h, (2 if fp32 else 1) if on else 0))
+        mixed-precision inner solves); assembly=True: the assembly launches
+        too (timing_read_asm)."""
+        self._ck(self.lib.wspm_timing_enable(self.h, ((2 if fp32 else 1) | (4 if assembly else 0)) if on else 0))
+
+    def timing_read_asm(self):
+        """(working assembly launches, their summed duration in ms, rows they assembled)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)
 
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""
@@ -315,12 +324,21 @@ class EnsembleGroups:
         fmts = {m.spmv_format for m in self.members}
         return fmts.pop() if len(fmts) == 1 else "+".join(sorted(fmts))
 
-    def timing(self, on: bool, fp32: bool = False):
-        self._each(lambda m: m.timing(on, fp32))
+    def timing(self, on: bool, fp32: bool = False, assembly: bool = False):
+        self._each(lambda m: m.timing(on, fp32, assembly))
 
     def timing_read(self):
         res = self._each(lambda m: m.timing_read())
         return sum(r[0] for r in res), sum(r[1] for r in res), sum(r[2] for r in res)
+
+    def timing_read_asm(self):
+        res = self._each(lambda m: m.timing_read_asm())
+        return sum(r[0] for r in res), sum(r[1] for r in res), sum(r[2] for r in res)
+
+    @property
+    def shape(self) -> dict:
+        """Launch shape of every member context (wspm_ens_shape)."""
+        return {"contexts": [m.shape for m in self.members]}
 
     def close(self):
         for m in self.members:

@@ -66,10 +66,14 @@ def _vg_problem(pts, tri, solver, precond, nsteps=3, dt=1e-4):
 
 @pytest.mark.parametrize("tag", ["25", "1024"])
 @pytest.mark.parametrize("solver,precond,reorder", [("bicgstab", "jacobi", None), ("cg", "jacobi", None),
-                                                    ("gmres", "none", None), ("bicgstab", "jacobi", "rcm")])
+                                                    ("gmres", "jacobi", None), ("bicgstab", "jacobi", "rcm")])
 def test_richards_pinns_matches_oracle(golden, tag, solver, precond, reorder):
     """The van Genuchten time step (richards_solver_v2.py:107-168) on the
-    unstructured meshes: top Dirichlet -75, h0 -1000, 3 fixed steps."""
+    unstructured meshes: top Dirichlet -75, h0 -1000, 3 fixed steps.  The dry
+    interior rows are about 1e-8 the scale of the Dirichlet rows, so without a
+    preconditioner GMRES stops (rtol 1e-10 on the unscaled residual) where the
+    two summation orders leave the solutions 1.8e-10 apart on Pinns_25; the
+    Jacobi-preconditioned solves are compared at the 1e-10 bar."""
     from jax_wspm_amd.richards import RichardsSolver
     from tests._helpers import to_oracle
     pts, tri = _mesh(golden, tag)
