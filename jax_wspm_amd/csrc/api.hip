@@ -643,6 +643,11 @@ struct wspm_ctx {
     int2* ell_inc = nullptr;        // neighbourhood assembly: incidences in ELL layout
     uint32_t* ell_info = nullptr;   // and rl | dg << 4 | nq << 8 per row
     int ell_iw = 0;
+    // the ELL operator in fan order and the fan kernel's tables (build_fan)
+    bool fan = false;
+    int32_t *fan_inc = nullptr, *fan_pred = nullptr;
+    uint32_t* fan_info = nullptr;
+    int fan_iw = 0;
     // finish-free CG / BiCGSTAB: second partial-row region and the iteration
     // histories of the main (0) and FP32 inner (1) solves
     double* partials2 = nullptr;
@@ -1628,7 +1633,7 @@ int32_t wspm_set_stream(wspm_ctx* ctx, void* stream) {
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info) {
     if (!ctx || !info) return WSPM_EINVAL;
     info[0] = ctx->n; info[1] = ctx->E; info[2] = ctx->nnz; info[3] = ctx->dim;
-    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->ell ? 1 : 0;
+    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->ell ? (ctx->fan ? 2 : 1) : 0;
     return WSPM_OK;
 }
 
@@ -1737,7 +1742,16 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
     a.ell_n = ell ? ctx->n : 0;    // callers pass ell only with the register kernels
-    if (ell && ctx->ell_info && asm_is_nbr(ctx->asm_kernel)) {
+    int kernel = ctx->asm_kernel;
+    if (ell && ctx->fan) {
+        a.ell_col = ctx->ell_col;
+        a.ell_w = ctx->ell_w;
+        a.fan_inc = ctx->fan_inc;
+        a.fan_pred = ctx->fan_pred;
+        a.fan_info = ctx->fan_info;
+        a.fan_iw = ctx->fan_iw;
+        kernel = ASM_K_FAN;
+    } else if (ell && ctx->ell_info && asm_is_nbr(ctx->asm_kernel)) {
         a.ell_col = ctx->ell_col;
         a.ell_inc = ctx->ell_inc;
         a.ell_info = ctx->ell_info;
@@ -1777,7 +1791,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
         HIPCHK(hipEventRecord(ae0, ctx->stream));
     }
     HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream,
-                           ctx->asm_kernel));
+                           kernel));
     if (timed) HIPCHK(hipEventRecord(ae1, ctx->stream));
     if (mode == ASM_INIT_NONE) return;
     KWork<double>& w = ensure_work<double>(ctx, 0);
@@ -2108,11 +2122,30 @@ static bool ell_ok(const wspm_ctx* ctx, const wspm_richards_params* prm) {
 }
 
 // Column table of the ELL layout: entry k of row i at k * n + i; padding
-// points at the row itself (its value stays 0).
+// points at the row itself (its value stays 0).  With the neighbourhood
+// assembly, the rows are put in fan order when every row of the mesh is one
+// fan of consistently oriented triangles (build_fan; WSPM_ASM_FAN=0 keeps
+// the sorted order): the fan kernel then assembles into slots known at
+// compile time.  The CSR pattern of the public entry points stays sorted.
 static void ensure_ell(wspm_ctx* ctx) {
     if (ctx->ell_col) return;
     const HostPattern& P = ctx->hp;
     const int n = ctx->n, W = ctx->max_row;
+    if (asm_is_nbr(ctx->asm_kernel) && W <= ASM_REG_ROW_2D && !env_is("WSPM_ASM_FAN", "0")) {
+        std::vector<int32_t> fc, fi, fp;
+        std::vector<uint32_t> finfo;
+        int iw = 0;
+        if (build_fan(P, W, fc, fi, fp, finfo, iw)) {
+            ctx->ell_col = dupload(ctx, fc.data(), fc.size());
+            ctx->ell_w = W;
+            ctx->fan_inc = dupload(ctx, fi.data(), fi.size());
+            ctx->fan_pred = dupload(ctx, fp.data(), fp.size());
+            ctx->fan_info = dupload(ctx, finfo.data(), finfo.size());
+            ctx->fan_iw = iw;
+            ctx->fan = true;
+            return;
+        }
+    }
     std::vector<int32_t> c((size_t)W * n);
     for (int i = 0; i < n; ++i) {
         const int rs = P.row_ptr[i], rl = P.row_ptr[i + 1] - rs;

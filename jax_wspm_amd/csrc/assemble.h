@@ -88,6 +88,14 @@ struct AsmArgs {
     const int2* ell_inc;
     const uint32_t* ell_info;
     int ell_w, ell_iw;
+    // Fan kernel (ASM_K_FAN): ell_col in fan order (slot 0 the node, slots
+    // 1..F its neighbours around it), the element << 2 | local vertex of fan
+    // triangle u at u * n + i (fan_iw planes), the neighbour before slot 1
+    // (closed fans) and F | closed << 4 per row.
+    const int32_t* fan_inc;
+    const int32_t* fan_pred;
+    const uint32_t* fan_info;
+    int fan_iw;
     QuadConst qc;
     // Ensemble mode (register kernel only): a grid of ens_grid workgroups
     // walks the row-table entries listed in act (act[0] = count; null: all
@@ -121,7 +129,9 @@ int asm_reg_max_row(int dim);
 // (2D rows <= 8 entries and <= 7 incidences).
 // ASM_K_NBR stages the neighbourhood in LDS; ASM_K_NBR_BLEND selects from
 // registers (WSPM_ASM_KERNEL=blend).
-enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3 };
+// ASM_K_FAN: the time-step assembly into an ELL operator in fan order
+// (every row a fan of consistently oriented triangles; chosen per launch).
+enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3, ASM_K_FAN = 4 };
 inline bool asm_is_nbr(int k) { return k == ASM_K_NBR || k == ASM_K_NBR_BLEND; }
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, int kernel);

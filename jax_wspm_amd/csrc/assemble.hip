@@ -526,10 +526,130 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
-// NB: 0 incidence gathers; 1, 2 neighbourhood gathers with LDS = NB - 1
+// Fan variant (2D, the time-step operator in fan order).  The row's slots
+// are the node itself (slot 0) and its neighbours in their cyclic order
+// around it, so fan triangle u is (node, slot u, slot u + 1), with the
+// closed fan's triangle 0 taking the neighbour before slot 1 from pred.
+// Every triangle therefore adds to slots known at compile time after the
+// unroll: no slot selects, no staging of the records.  The triangle is
+// taken in the rotated order (node, slot u, slot u + 1) = (v_la, v_la+1,
+// v_la+2) of its own vertices, the same orientation, so det J, G and the
+// stiffness row of the node are those of the element (to rounding), with
+// the quadrature's conductivity and mass weights of vertices la, la + 1,
+// la + 2.  Sums run over the triangles in fan order; a closed fan's
+// triangle 0 contribution to the last neighbour is added after the others.
+template <int MODE>
+__device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
+    constexpr int W = 8;          // the node + up to 7 neighbours
+    constexpr int T = W - 1;      // fan triangles
+    const uint32_t inf = a.fan_info[i];
+    const int F = (int)(inf & 15u);
+    const bool closed = (inf & 16u) != 0;
+    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
+    int col[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) col[s] = s < a.ell_w ? a.ell_col[(size_t)s * a.n + i] : i;
+    int code[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
+    const int pr = a.fan_pred[i];
+    // every gather unconditional: padding columns, pred and padding triangle
+    // codes all name valid nodes / elements (the row's own node, element 0),
+    // and values of absent slots are never used.  (A select between a record
+    // and a zero record compiles into a flat load through a selected pointer.)
+    double4 nr[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) nr[s] = a.nrec[col[s]];
+    const double4 np = a.nrec[pr];
+    double xv[W];
+    if constexpr (MODE != ASM_INIT_NONE) {
+#pragma unroll
+        for (int s = 0; s < W; ++s) xv[s] = ki.x0[col[s]];
+    }
+    double ks[T];
+    if (a.Ks_elem != nullptr) {   // uniform
+#pragma unroll
+        for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[code[u] >> 2];
+    } else {
+#pragma unroll
+        for (int u = 0; u < T; ++u) ks[u] = 1.0;
+    }
+    double v[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) v[s] = 0.0;
+    double bi;
+    if (dir) {
+        v[0] = 1.0;
+        bi = a.bc_val[i];
+    } else {
+        bi = 0.0;
+        double msum = 0.0, vp = 0.0;
+        const double sx = nr[0].x, sy = nr[0].y, sk = nr[0].z;
+        const QuadConst& qc = a.qc;
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            const bool ex = closed ? (u < F) : (u >= 1 && u < F);
+            if (ex) {
+                const double4 P = (u == 0) ? np : nr[u];
+                const double4 Q = nr[u + 1];
+                const int la = code[u] & 3;
+                const double J00 = P.x - sx, J01 = P.y - sy;
+                const double J10 = Q.x - sx, J11 = Q.y - sy;
+                const double det = J00 * J11 - J01 * J10;
+                const double id = 1.0 / det;
+                const double G01 = J11 * id, G11 = -J10 * id;
+                const double G02 = -J01 * id, G12 = J00 * id;
+                const double G00 = -(G01 + G02), G10 = -(G11 + G12);
+                const double ckS = la == 0 ? qc.ck[0] : (la == 1 ? qc.ck[1] : qc.ck[2]);
+                const double ckP = la == 0 ? qc.ck[1] : (la == 1 ? qc.ck[2] : qc.ck[0]);
+                const double ckQ = la == 0 ? qc.ck[2] : (la == 1 ? qc.ck[0] : qc.ck[1]);
+                const double mlS = la == 0 ? qc.ml[0] : (la == 1 ? qc.ml[1] : qc.ml[2]);
+                const double kbar = (ckS * sk + ckP * P.z + ckQ * Q.z) * ks[u];
+                const double kd = kbar * det;
+                v[0] += kd * (G00 * G00 + G10 * G10);
+                const double oP = kd * (G00 * G01 + G10 * G11);
+                if (u == 0) vp += oP;
+                else v[u] += oP;
+                v[u + 1] += kd * (G00 * G02 + G10 * G12);
+                bi += -kd * G10;
+                msum += det * mlS;
+            }
+        }
+        // the closed fan's triangle 0 joins the last neighbour (slot F)
+#pragma unroll
+        for (int s = 1; s < W; ++s) v[s] = (closed && s == F) ? v[s] + vp : v[s];
+        const double ci = a.C[i];
+        const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+        v[0] += a.inv_dt * msum * ci;
+        bi -= a.inv_dt * msum * rhs;
+    }
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        if (s <= F && s < a.ell_w) {
+            const size_t o = (size_t)s * a.ell_n + i;
+            a.A[o] = v[s];
+            if (a.A32 != nullptr) a.A32[o] = (float)v[s];
+        }
+    }
+    a.b[i] = bi;
+    const double dgv = v[0];
+    const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
+    if (a.dinv != nullptr) a.dinv[i] = di;
+    if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
+    if constexpr (MODE != ASM_INIT_NONE) {
+        double ax = 0.0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (s <= F) ax += v[s] * xv[s];
+        krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
+    }
+}
+
+// NB: 0 incidence gathers; 1, 2 neighbourhood gathers with LDS = NB - 1; 3 fan order
 template <int D, int MAXR, int MODE, int NB>
 __device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
-    if constexpr (NB > 0 && D == 2) assemble_row_nbr<MAXR, MODE, NB - 1>(a, ki, i, acc);
+    if constexpr (NB == 3 && D == 2) assemble_row_fan<MODE>(a, ki, i, acc);
+    else if constexpr (NB > 0 && D == 2) assemble_row_nbr<MAXR, MODE, NB - 1>(a, ki, i, acc);
     else assemble_row<D, MAXR, MODE>(a, ki, i, acc);
 }
 
@@ -639,6 +759,7 @@ hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit
     if (kernel != ASM_K_LDS) {
         const int grid = a.tab ? a.ens_grid : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
+        if (kernel == ASM_K_FAN) return launch_reg<2, ASM_REG_ROW_2D, 3>(mode, dim3(grid), s, a, ki, partials, prow);
         if (kernel == ASM_K_NBR) return launch_reg<2, ASM_REG_ROW_2D, 2>(mode, dim3(grid), s, a, ki, partials, prow);
         if (kernel == ASM_K_NBR_BLEND) return launch_reg<2, ASM_REG_ROW_2D, 1>(mode, dim3(grid), s, a, ki, partials, prow);
         return launch_reg<2, ASM_REG_ROW_2D, 0>(mode, dim3(grid), s, a, ki, partials, prow);

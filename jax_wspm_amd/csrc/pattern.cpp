@@ -96,7 +96,92 @@ int build_pattern(int dim, int n, int E, const int32_t* conn, HostPattern& P, st
         }
     }
 
+    if (dim == 2) P.conn.assign(conn, conn + (size_t)E * K);
     return WSPM_OK;
+}
+
+bool build_fan(const HostPattern& P, int W, std::vector<int32_t>& col, std::vector<int32_t>& inc,
+               std::vector<int32_t>& pred, std::vector<uint32_t>& info, int& iw) {
+    const int n = (int)P.row_ptr.size() - 1;
+    if (P.conn.empty() || W > 16) return false;
+    // planes of the triangle table: triangle indices run to nq - 1 (closed)
+    // or nq (open), and are checked against W - 2 per row below
+    iw = 0;
+    for (int i = 0; i < n; ++i) iw = std::max(iw, P.inc_ptr[i + 1] - P.inc_ptr[i] + 1);
+    iw = std::min(iw, W - 1);
+    col.assign((size_t)W * n, 0);
+    inc.assign((size_t)std::max(iw, 1) * n, 0);
+    pred.assign(n, 0);
+    info.assign(n, 0);
+    int la_[16], a_[16], b_[16], ord[16], nb[17];
+    for (int i = 0; i < n; ++i)
+        if (P.inc_ptr[i + 1] - P.inc_ptr[i] > 15) return false;
+    for (int i = 0; i < n; ++i) {
+        const int q0 = P.inc_ptr[i], nq = P.inc_ptr[i + 1] - q0;
+        // the link of incidence u: the other two vertices in the element's own
+        // cyclic order (node = vertex la, then la + 1, la + 2)
+        for (int u = 0; u < nq; ++u) {
+            const int code = P.inc_code[q0 + u], e = code >> 2, la = code & 3;
+            la_[u] = code;
+            a_[u] = P.conn[(size_t)e * 3 + (la + 1) % 3];
+            b_[u] = P.conn[(size_t)e * 3 + (la + 2) % 3];
+        }
+        // every neighbour starts at most one link and ends at most one
+        int start = -1, nstart = 0;
+        for (int u = 0; u < nq; ++u) {
+            int outs = 0, ins = 0;
+            for (int v = 0; v < nq; ++v) {
+                outs += a_[v] == a_[u];
+                ins += b_[v] == a_[u];
+                if (v != u && b_[v] == b_[u]) return false;
+            }
+            if (outs != 1 || ins > 1) return false;
+            if (ins == 0) {
+                start = u;
+                ++nstart;
+            }
+        }
+        if (nstart > 1) return false;
+        const bool closed = nq > 0 && nstart == 0;
+        if (closed) start = 0;
+        // walk the links from the start
+        int F = 0;
+        if (nq > 0) {
+            ord[0] = start;
+            for (int k = 1; k < nq; ++k) {
+                int nx = -1;
+                for (int v = 0; v < nq; ++v)
+                    if (a_[v] == b_[ord[k - 1]]) nx = v;
+                if (nx < 0 || nx == start) return false;   // more than one fan
+                ord[k] = nx;
+            }
+            if (closed && b_[ord[nq - 1]] != a_[start]) return false;
+            for (int k = 0; k < nq; ++k) nb[1 + k] = a_[ord[k]];
+            F = closed ? nq : nq + 1;
+            if (!closed) nb[F] = b_[ord[nq - 1]];
+        }
+        if (F + 1 > W || F + 1 != P.row_ptr[i + 1] - P.row_ptr[i]) return false;
+        if ((closed ? nq - 1 : nq) >= iw) return false;    // a triangle index beyond the table
+        // the fan's nodes are exactly the row's columns
+        int srt[17];
+        srt[0] = i;
+        for (int k = 1; k <= F; ++k) srt[k] = nb[k];
+        std::sort(srt, srt + F + 1);
+        for (int k = 0; k <= F; ++k)
+            if (srt[k] != P.cols[P.row_ptr[i] + k]) return false;
+        col[i] = i;
+        for (int k = 1; k < W; ++k) col[(size_t)k * n + i] = k <= F ? nb[k] : i;
+        // fan triangle u joins slots u and u + 1: closed, triangle 0 is the
+        // last link (last neighbour -> first) and triangle u the u-th; open,
+        // triangle u (1 <= u < F) is the u-th link
+        for (int u = 0; u < nq; ++u) {
+            const int slot = closed ? (u + 1) % nq : u + 1;   // triangle index of link ord[u]
+            inc[(size_t)slot * n + i] = la_[ord[u]];
+        }
+        pred[i] = closed ? nb[F] : i;
+        info[i] = (uint32_t)F | (closed ? 16u : 0u);
+    }
+    return true;
 }
 
 void block_maxima(const HostPattern& P, int R, int& max_nnz, int& max_inc) {

@@ -279,6 +279,14 @@ class Context:
 
     # -- timing hooks ---------------------------------------------------------------
     @property
+    def operator_order(self) -> str:
+        """Order of each row's columns in the time-step operator: "fan" (the
+        node, then its neighbours around it) or "sorted"."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        return "fan" if info[7] == 2 else "sorted"
+
+    @property
     def spmv_format(self) -> str:
         """Operator layout of the time-step solves since the last Richards
         init: "ell" or "csr"."""

@@ -316,3 +316,69 @@ def test_ell_and_csr_time_steps_agree(monkeypatch):
     assert list(a["krylov_iterations"]) == list(b["krylov_iterations"])
     for k in range(len(a["times"])):
         assert np.linalg.norm(a["pressure_head"][k] - b["pressure_head"][k]) <= 1e-13 * np.linalg.norm(b["pressure_head"][k])
+
+
+def _jittered_problem(cells, solver, seed=3, nsteps=5):
+    from dataclasses import replace
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    pb = from_config(baseline_vg_config(cells, 2, solver=solver, precond="jacobi", nsteps=nsteps, dt=1e-4,
+                                        ks_field={"sigma": 0.5, "length": 0.1, "seed": seed}))
+    pts = pb.points.copy()
+    inner = (pts[:, 0] > 0) & (pts[:, 0] < 1) & (pts[:, 1] > 0) & (pts[:, 1] < 1)
+    pts[inner] += 0.3 / cells * np.random.default_rng(seed).uniform(-1, 1, size=(inner.sum(), 2))
+    return replace(pb, points=pts)
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cg"])
+def test_fan_order_time_steps(monkeypatch, solver):
+    """The time-step operator in fan order (each row: the node, then its
+    neighbours around it, so the assembly adds every triangle into slots
+    fixed at compile time) against the sorted ELL order of the neighbourhood
+    kernel (WSPM_ASM_FAN=0) and against the oracle, on a jittered 33x33 mesh
+    with per-element Ks: the same Picard and Krylov counts, trajectories
+    within 1e-12 of each other (summation order only) and 1e-10 of the
+    oracle."""
+    from jax_wspm_amd.richards import RichardsSolver
+    from oracle import picard as opicard
+    from tests._helpers import to_oracle
+    pb = _jittered_problem(33, solver)
+    out = {}
+    for fan in ("1", "0"):
+        print(f"[case] fan order {solver} WSPM_ASM_FAN={fan}", flush=True)
+        monkeypatch.setenv("WSPM_ASM_FAN", fan)
+        s = RichardsSolver(pb)
+        out[fan] = s.solve()
+        assert s.ctx.operator_order == ("fan" if fan == "1" else "sorted")
+        s.ctx.close()
+    a, b = out["1"], out["0"]
+    assert list(a["iterations"]) == list(b["iterations"])
+    assert list(a["krylov_iterations"]) == list(b["krylov_iterations"])
+    o = opicard.run(to_oracle(pb))
+    assert list(a["iterations"]) == list(o["iterations"])
+    for k in range(len(a["times"])):
+        assert rel_l2(a["pressure_head"][k], b["pressure_head"][k]) <= 1e-12, k
+        assert rel_l2(a["pressure_head"][k], o["pressure_head"][k]) <= 1e-10, k
+        assert rel_l2(a["theta"][k], o["theta"][k]) <= 1e-10, k
+
+
+def test_fan_order_needs_consistent_orientation():
+    """One triangle listed in the opposite orientation: its vertices' rows are
+    no longer fans of consistently oriented triangles, so the operator keeps
+    the sorted ELL order (and the results their parity)."""
+    from dataclasses import replace
+    from jax_wspm_amd.richards import RichardsSolver
+    from oracle import picard as opicard
+    from tests._helpers import to_oracle
+    pb = _jittered_problem(17, "bicgstab", nsteps=3)
+    el = pb.elements.copy()
+    el[40] = el[40][::-1]
+    pb = replace(pb, elements=el)
+    print("[case] fan order, one flipped triangle", flush=True)
+    s = RichardsSolver(pb)
+    g = s.solve()
+    assert s.ctx.spmv_format == "ell" and s.ctx.operator_order == "sorted"
+    s.ctx.close()
+    o = opicard.run(to_oracle(pb))
+    assert list(g["iterations"]) == list(o["iterations"])
+    for k in range(len(g["times"])):
+        assert rel_l2(g["pressure_head"][k], o["pressure_head"][k]) <= 1e-10, k
