@@ -144,3 +144,74 @@ def test_tracy_general_preconditioners(solver, precond, golden):
     assert int(np.sum(g["iterations"])) == 53
     e = tracy.final_errors(pb.points, g["pressure_head"][-1], g["theta"][-1], g["times"][-1], 1)
     assert f"{e['l2_pressure']:.4e}" == "1.7613e-01" and f"{e['l2_saturation']:.4e}" == "5.0311e-03"
+
+
+@pytest.mark.parametrize("cells,steps,tol", [(16, 5, 1e-10), (16, 5, 1e-13), (24, 3, 1e-13)])
+def test_mixed_3d_matches_oracle(cells, steps, tol):
+    """The 3D configuration with FP32 inner CG (Jacobi) inside FP64 iterative
+    refinement, FP64 Picard residual and update (unit cube, Kuhn tets, h0
+    -500, top -10, van Genuchten alpha 0.1, n 2), against the FP64 oracle at
+    the same Krylov tolerance: the same Picard counts, h and theta within
+    1e-8 relative L2 at every step."""
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print(f"[case] mixed 3D {cells}^3 CG+jacobi {steps} steps tol {tol:g}", flush=True)
+    pb = from_config(baseline_vg_config(cells, 3, solver="cg", precond="jacobi", nsteps=steps, dt=1e-4,
+                                        precision="mixed"))
+    pb.tol = tol
+    g = _run(pb)
+    o = opicard.run(to_oracle(pb))
+    assert list(g["iterations"]) == list(o["iterations"])
+    for k in range(steps):
+        eh = rel_l2(g["pressure_head"][k], o["pressure_head"][k])
+        et = rel_l2(g["theta"][k], o["theta"][k])
+        print(f"  step {k}: rel L2 h {eh:.2e} theta {et:.2e}")
+        assert eh <= 1e-8 and et <= 1e-8, k
+
+
+def _dry_instances():
+    """Corners of the ensemble's parameter box: alpha 0.01 / 0.2, n 1.3 / 3.0,
+    Ks 1e-4 / 1e-2, sides 32 and 48 (alpha 0.2, n 3 at h = -1000 gives
+    C ~ 2e-8: interior rows some 1e-8 the scale of the Dirichlet rows)."""
+    from jax_wspm_amd import mesh as M
+    from jax_wspm_amd.ensemble import Instance
+    out = []
+    for k, (alpha, n, ks, N) in enumerate([(0.01, 1.3, 1e-4, 32), (0.2, 3.0, 1e-4, 48), (0.2, 1.3, 1e-2, 32),
+                                           (0.01, 3.0, 1e-2, 48), (0.2, 3.0, 1e-2, 32), (0.01, 1.3, 1e-2, 48)]):
+        pts, el = M.unit_square(N, 1.0)
+        top = np.nonzero(pts[:, 1] == 1.0)[0]
+        ts, tr = (0.35, 0.1) if k % 2 else (0.5, 0.02)
+        out.append(Instance(points=pts, elements=el, phi=(alpha, ts, tr, n, 1.0 - 1.0 / n, ks),
+                            bc_lists=[(top, np.full(top.shape, -75.0))], h0=np.full(len(pts), -1000.0), cells=N))
+    return out
+
+
+def test_mixed_ensemble_dry_instances_within_1e8():
+    """Batched mixed precision on the driest, worst-conditioned corners of the
+    ensemble's parameter box: refined until the FP64 residual is 1e-14 of
+    |b| (where the FP64 solution itself is converged), every instance stays
+    within 1e-8 relative L2 of the FP64 ensemble at the same tolerance and of
+    the oracle, with the same Picard counts, over 10 steps."""
+    from jax_wspm_amd.ensemble import Ensemble
+    insts = _dry_instances()
+    steps, tol = 10, 1e-14
+    print("[case] mixed ensemble, dry corners, tol 1e-14", flush=True)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        ens = Ensemble(insts)
+        ens.richards_init("vg", "bicgstab", "jacobi", precision=prec, tol=tol, inner_tol=1e-6)
+        its = [ens.step(1e-4, per_instance=True)["picard_iters_i"].copy() for _ in range(steps)]
+        h, th, _ = ens.state()
+        res[prec] = (np.array(its), ens.split(h), ens.split(th))
+        ens.close()
+    (im, hm, tm), (i6, h6, t6) = res["mixed"], res["fp64"]
+    assert np.array_equal(im, i6)
+    for i, inst in enumerate(insts):
+        eh, et = rel_l2(hm[i], h6[i]), rel_l2(tm[i], t6[i])
+        print(f"  instance {i} alpha {inst.phi[0]} n {inst.phi[3]}: rel L2 h {eh:.2e} theta {et:.2e}")
+        assert eh <= 1e-8 and et <= 1e-8, i
+        o = opicard.run(opicard.Problem(points=inst.points, elems=inst.elements, soil_model="vg",
+                                        phi=tuple(inst.phi), bc_lists=inst.bc_lists, h0=inst.h0, dt0=1e-4,
+                                        nsteps=steps, adaptive=False, solver="bicgstab", precond="jacobi", tol=tol,
+                                        maxiter=100))
+        assert list(o["iterations"]) == [int(x) for x in im[:, i]], i
+        assert rel_l2(hm[i], o["pressure_head"][-1]) <= 1e-8, i
