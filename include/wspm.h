@@ -174,6 +174,23 @@ int32_t wspm_assemble_richards(wspm_ctx* ctx, const double* Ks_elem, const doubl
 /* y = A x on the context's pattern (dev pointers). */
 int32_t wspm_spmv(wspm_ctx* ctx, const double* A_vals, const double* x, double* y);
 
+/* Krylov solve that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The CG or
+ * BiCGSTAB of wspm_krylov (Jacobi with dinv, or none), on a CSR matrix the
+ * caller passes (dev: row_ptr [n+1], cols [nnz], A_vals [nnz]; nnz host),
+ * enqueued on `stream` (hipStream_t or NULL) without a context.  Its
+ * workspace is a file-scope device array of the library (40 MiB per GPU,
+ * held until the process ends): n <= 524288 rows and maxiter <= 10000, else
+ * WSPM_EINVAL.  No hipMalloc, hipFree or allocating call is made; the host
+ * reads the solver state back by a copy to its stack and a stream
+ * synchronisation.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (other threads, other streams) race.  Errors:
+ * wspm_last_error(NULL). */
+int32_t wspm_krylov_noalloc(int32_t method, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* cols,
+                            const double* A_vals, const double* dinv, const double* b, const double* x0, double* x,
+                            double tol, double atol, int32_t maxiter, void* stream, wspm_krylov_stats* stats);
+
 /* Preconditioned Krylov solve A x = b from x0 (dev pointers; x0 may equal x).
  * dinv NULL = no preconditioner, else Jacobi with that inverse diagonal. */
 int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const double* dinv,

@@ -83,6 +83,8 @@ PROTOTYPES = {
     "wspm_assemble_richards": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
     "wspm_spmv": (_I, [_P, _P, _P, _P]),
     "wspm_krylov": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, C.POINTER(KrylovStats)]),
+    "wspm_krylov_noalloc": (_I, [_I, _I, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _D, _D, _I, _P,
+                                 C.POINTER(KrylovStats)]),
     "wspm_krylov_pc": (_I, [_P, _I, _I, _P, _P, _P, _P, _D, _D, _I, _I, C.POINTER(KrylovStats)]),
     "wspm_krylov_mixed": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, _D, _I,
                                C.POINTER(KrylovStats)]),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
@@ -1424,6 +1425,45 @@ void krylov_solve(wspm_ctx* ctx, int method, const T* A, const T* dinv, const T*
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Krylov solve that allocates nothing (wspm_krylov_noalloc)
+// ---------------------------------------------------------------------------
+// Its workspace -- the Krylov vectors, two regions of partial rows, the
+// solver states and the iteration history -- lives in this file-scope
+// array, one copy per GPU, held until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current device
+// (the array's name is a host-side stand-in); every call on one GPU shares
+// it, so calls that overlap on one GPU race.
+static constexpr int NOALLOC_MAX_N = 1 << 19;          // rows
+static constexpr int NOALLOC_MAX_ITER = 10000;         // maxiter
+static constexpr size_t NOALLOC_VEC = 9;               // r, r_hat, p, p_hat, q, s_hat, t, z, w
+static constexpr size_t NOALLOC_PROW = MAX_BLOCKS;     // partial rows (vec_grid <= 2048 at NOALLOC_MAX_N)
+static constexpr size_t NOALLOC_DOUBLES =
+    NOALLOC_VEC * NOALLOC_MAX_N + 2 * NOALLOC_PROW * MAX_DOTS +
+    (3 * sizeof(KState) + (NOALLOC_MAX_ITER + 2) * sizeof(KHist) + 255) / sizeof(double) + 64;
+static_assert((NOALLOC_MAX_N + BLOCK - 1) / BLOCK <= (int)NOALLOC_PROW, "partial rows");
+static_assert(sizeof(KState) % 8 == 0 && sizeof(KHist) % 8 == 0, "scratch layout");
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+
+// Per-device address of scratch (set once per device, with that device
+// current) and the iterations of its last solve per method (the first chunk
+// of the next one).
+struct NoallocDev {
+    double* ptr = nullptr;
+    int last_iters[2] = {1, 1};
+};
+std::mutex g_noalloc_mu;
+NoallocDev g_noalloc[64];
+
+__global__ void k_fill_zero(double* p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C API
 // ---------------------------------------------------------------------------
 #define API_BEGIN try {
@@ -1952,6 +1992,101 @@ int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const d
     if (stats) *stats = st;
     return WSPM_OK;
     API_END
+}
+
+int32_t wspm_krylov_noalloc(int32_t method, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* cols,
+                            const double* A_vals, const double* dinv, const double* b, const double* x0, double* x,
+                            double tol, double atol, int32_t maxiter, void* stream, wspm_krylov_stats* stats) {
+    if (method != WSPM_CG && method != WSPM_BICGSTAB) {
+        g_create_error = "wspm_krylov_noalloc: method must be WSPM_CG or WSPM_BICGSTAB";
+        return WSPM_EINVAL;
+    }
+    if (n < 1 || n > NOALLOC_MAX_N) {
+        g_create_error = "wspm_krylov_noalloc: n = " + std::to_string(n) + " outside [1, " +
+                         std::to_string(NOALLOC_MAX_N) + "] (the static workspace's rows)";
+        return WSPM_EINVAL;
+    }
+    if (maxiter < 0 || maxiter > NOALLOC_MAX_ITER) {
+        g_create_error = "wspm_krylov_noalloc: maxiter outside [0, " + std::to_string(NOALLOC_MAX_ITER) + "]";
+        return WSPM_EINVAL;
+    }
+    if (nnz < n || !row_ptr || !cols || !A_vals || !b || !x0 || !x) {
+        g_create_error = "wspm_krylov_noalloc: null array or nnz < n";
+        return WSPM_EINVAL;
+    }
+    int dev = 0;
+    hipError_t he = hipGetDevice(&dev);
+    if (he != hipSuccess || dev < 0 || dev >= 64) {
+        g_create_error = std::string("wspm_krylov_noalloc: hipGetDevice: ") + hipGetErrorString(he);
+        return WSPM_EHIP;
+    }
+    NoallocDev* nd = &g_noalloc[dev];
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!nd->ptr) {
+            void* p = nullptr;
+            he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (he != hipSuccess || !p) {
+                g_create_error = std::string("wspm_krylov_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(he);
+                return WSPM_EHIP;
+            }
+            nd->ptr = static_cast<double*>(p);
+        }
+    }
+    // a context on the host stack over the caller's matrix and the scratch
+    // array: no device or pinned allocation anywhere below
+    wspm_ctx c;
+    wspm_ctx* ctx = &c;
+    KState hst[3];
+    std::memset(hst, 0, sizeof(hst));
+    c.stream = static_cast<hipStream_t>(stream);
+    c.dim = 2;
+    c.n = n;
+    c.nnz = nnz;
+    const double avg = (double)nnz / n;
+    int lanes = 2;
+    while (lanes < 64 && lanes < avg) lanes *= 2;
+    c.lanes = lanes;
+    c.spmv_grid = (int)std::min<long>((n + BLOCK / lanes - 1) / (BLOCK / lanes), MAX_BLOCKS);
+    c.vec_grid = std::min((n + BLOCK - 1) / BLOCK, 4096);
+    c.prow = (int)NOALLOC_PROW;
+    c.row_ptr = const_cast<int32_t*>(row_ptr);
+    c.cols = const_cast<int32_t*>(cols);
+    double* q = nd->ptr;
+    KWork<double>& w = c.w64;
+    w.n = n;
+    double** vecs[NOALLOC_VEC] = {&w.r, &w.rh, &w.p, &w.ph, &w.q, &w.sh, &w.t, &w.z, &w.w};
+    for (size_t k = 0; k < NOALLOC_VEC; ++k) *vecs[k] = q + k * (size_t)NOALLOC_MAX_N;
+    q += NOALLOC_VEC * (size_t)NOALLOC_MAX_N;
+    c.partials = q;
+    c.partials2 = q + NOALLOC_PROW * MAX_DOTS;
+    q += 2 * NOALLOC_PROW * MAX_DOTS;
+    double* zero_from = q;
+    c.states = reinterpret_cast<KState*>(q);
+    q += 3 * sizeof(KState) / sizeof(double);
+    c.khist[0] = reinterpret_cast<KHist*>(q);
+    c.khist_cap[0] = maxiter + 2;
+    q += (size_t)(maxiter + 2) * sizeof(KHist) / sizeof(double);
+    c.hstate = hst;
+    c.fast_poll = false;
+    c.last_iters[method] = nd->last_iters[method];
+    try {
+        // the states and the iteration history start at zero on every call
+        // (a fill kernel, not a memset node, so callers may capture the call)
+        hipLaunchKernelGGL(k_fill_zero, dim3(64), dim3(BLOCK), 0, c.stream, zero_from, (size_t)(q - zero_from));
+        HIPCHK(hipGetLastError());
+        wspm_krylov_stats st{};
+        krylov_solve<double>(ctx, method, A_vals, dinv, b, x0, x, tol, atol, maxiter, 30, 0, &st);
+        nd->last_iters[method] = c.last_iters[method];
+        if (stats) *stats = st;
+    } catch (const Err&) {
+        g_create_error = c.err;
+        return WSPM_EHIP;
+    } catch (const ApiFail& f) {
+        g_create_error = f.msg;
+        return f.code;
+    }
+    return WSPM_OK;
 }
 
 int32_t wspm_krylov_pc(wspm_ctx* ctx, int32_t method, int32_t precond, const double* A_vals, const double* b,
