@@ -658,16 +658,22 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
     if (a.tab == nullptr) {
         if (a.gate != nullptr && *a.gate == 0) return;   // uniform: a Picard iteration enqueued ahead, not needed
         if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
-        const int i = blockIdx.x * BLOCK + threadIdx.x;
+        // rows of logical block L (each XCD assembles one contiguous range of
+        // rows, whose neighbours' records then come through one L2); its
+        // partial sums in column L, as without the map
+        const int L = xcd_block(blockIdx.x, gridDim.x);
+        const int i = L * BLOCK + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
         if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
-        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow);
+        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, L);
         return;
     }
     // ensemble: fixed grid over the listed table entries (instances still
-    // iterating Picard), partial sums in the entry's column
-    const int cnt = a.act ? a.act[0] : a.n_tab;
-    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+    // iterating Picard), each XCD on one block of the list, partial sums in
+    // the entry's column
+    int v0, v1, vstep;
+    xcd_walk(a.act ? a.act[0] : a.n_tab, v0, v1, vstep);
+    for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
         const int t = a.act ? a.act[1 + v] : v;
         const int4 tb = a.tab[t];
         if (a.ens_st[tb.z].pdone) continue;

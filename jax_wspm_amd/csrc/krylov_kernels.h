@@ -119,8 +119,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
-    const int cnt = ew.count();
-    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+    int v0, v1, vstep;
+    xcd_walk(ew.count(), v0, v1, vstep);
+    for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;
@@ -184,8 +185,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __rest
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
-    const int cnt = ew.count();
-    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+    int v0, v1, vstep;
+    xcd_walk(ew.count(), v0, v1, vstep);
+    for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;
@@ -221,8 +223,9 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
     }
-    const int cnt = ew.count();
-    for (int v = blockIdx.x; v < cnt; v += gridDim.x) {   // uniform over the workgroup
+    int v0, v1, vstep;
+    xcd_walk(ew.count(), v0, v1, vstep);
+    for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;

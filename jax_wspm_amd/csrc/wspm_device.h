@@ -54,10 +54,9 @@ struct KState {
 // logical index such that every XCD receives one contiguous range of logical
 // blocks: neighbouring rows (which gather the same node and element records)
 // then share one L2 instead of being fetched by several.  A bijection of
-// [0, nb) for any nb.  Used by the one-system SpMV and vector kernels (it
-// brings the SpMV's fetched bytes down to its algorithmic bytes); measured
-// slower for the ensemble walks and neutral for the assembly, which keep
-// the plain order.
+// [0, nb) for any nb.  Used by the one-system SpMV, vector and assembly
+// kernels (it brings the SpMV's fetched bytes down to its algorithmic
+// bytes); the ensemble walks use xcd_walk below.
 constexpr int NXCD = 8;
 __device__ inline int xcd_block(int b, int nb) {
     const int q = nb / NXCD, r = nb % NXCD;
@@ -125,6 +124,29 @@ struct EnsWalk {
     __device__ int count() const { return act ? act[0] : n_tab; }
     __device__ int entry(int v) const { return act ? act[1 + v] : v; }
 };
+
+// The listed positions [v0, v1) this workgroup walks, with stride step, in
+// an ensemble launch of cnt listed entries.  Every XCD takes one contiguous
+// block of ceil(cnt / 8) positions, walked by its own workgroups (index b
+// runs on XCD b % 8), so the neighbouring entries of an instance -- whose
+// rows gather each other's node records and x values -- share one L2, and
+// the XCDs get equal shares of the list whatever its length.  Each entry
+// still writes its own partial-row column, so results do not depend on
+// the walk.  Grids below 8 workgroups walk the list plainly.
+__device__ inline void xcd_walk(int cnt, int& v0, int& v1, int& step) {
+    const int nb = gridDim.x;
+    if (nb < NXCD) {
+        v0 = blockIdx.x;
+        v1 = cnt;
+        step = nb;
+        return;
+    }
+    const int x = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
+    const int chunk = (cnt + NXCD - 1) / NXCD;
+    v0 = x * chunk + k;
+    v1 = min(cnt, (x + 1) * chunk);
+    step = nb / NXCD + (x < nb % NXCD ? 1 : 0);
+}
 
 // Store this workgroup's ND block sums in column bid of the partials
 // (default: the workgroup's index; NW = wavefronts per workgroup).
