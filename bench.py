@@ -460,7 +460,7 @@ def run_single(args):
 
     desc, cells, dim, solver, precision, ksf, dt = WORKLOADS[args.workload]
     mixed = precision == "mixed"
-    tol = TIGHT_TOL if (mixed and dim == 3) else 1e-10
+    tol = TIGHT_TOL if mixed else 1e-10
 
     def make(prec, ktol):
         cfg = baseline_vg_config(cells, dim, solver=solver, precond="jacobi", precision=prec, ks_field=ksf,

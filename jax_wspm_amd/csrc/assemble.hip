@@ -183,7 +183,11 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     __shared__ int rp_s[R + 1], ip_s[R + 1], dg_s[R];
     __shared__ uint8_t bc_s[R];
     __shared__ double bsh[R], dsh[R], cs_s[R], rs_s[R];
-    const int r0 = blockIdx.x * R;
+    // rows of logical block L: each XCD assembles one contiguous range of
+    // rows (their neighbours' records come through one L2); partials in
+    // column L
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = L * R;
     const int r1 = min(a.n, r0 + R);
     const int nrows = r1 - r0;
     if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
         if (g < nrows && l == 0) krylov_start<MODE>(a, ki, r0 + g, bsh[g], s, dsh[g], acc);
-        store_partials<3, ASM_THREADS / 64>(acc, partials, prow);
+        store_partials<3, ASM_THREADS / 64>(acc, partials, prow, L);
     }
 }
 
@@ -549,9 +553,6 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
     int col[W];
 #pragma unroll
     for (int s = 0; s < W; ++s) col[s] = s < a.ell_w ? a.ell_col[(size_t)s * a.n + i] : i;
-    int code[T];
-#pragma unroll
-    for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
     const int pr = a.fan_pred[i];
     // every gather unconditional: padding columns, pred and padding triangle
     // codes all name valid nodes / elements (the row's own node, element 0),
@@ -566,8 +567,13 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
 #pragma unroll
         for (int s = 0; s < W; ++s) xv[s] = ki.x0[col[s]];
     }
+    // per-element Ks through the fan triangles' elements (the table is read
+    // only then: the node's local vertex of each triangle is in the info word)
     double ks[T];
     if (a.Ks_elem != nullptr) {   // uniform
+        int code[T];
+#pragma unroll
+        for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
 #pragma unroll
         for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[code[u] >> 2];
     } else {
@@ -592,7 +598,7 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
             if (ex) {
                 const double4 P = (u == 0) ? np : nr[u];
                 const double4 Q = nr[u + 1];
-                const int la = code[u] & 3;
+                const int la = (int)((inf >> (5 + 2 * u)) & 3u);
                 const double J00 = P.x - sx, J01 = P.y - sy;
                 const double J10 = Q.x - sx, J11 = Q.y - sy;
                 const double det = J00 * J11 - J01 * J10;

@@ -10,6 +10,8 @@
 // enqueue several iterations ahead without reading anything back.
 #pragma once
 
+#include <type_traits>
+
 #include "wspm_device.h"
 
 namespace wspm {
@@ -146,15 +148,18 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
 // dependent memory round trips per row, no row pointers and no cross-lane
 // reduction.  The row sum runs over k in order.
 // ---------------------------------------------------------------------------
-template <int W, class T, class Epi>
-__device__ inline void ell_rows(int rb, int re, int n, const int32_t* __restrict__ ci, const T* __restrict__ av,
+// CI int32_t: the column table holds columns; int16_t: it holds each
+// column's offset from the row (2 bytes an entry instead of 4).
+template <int W, class T, class CI, class Epi>
+__device__ inline void ell_rows(int rb, int re, int n, const CI* __restrict__ ci, const T* __restrict__ av,
                                 const Epi& epi, double* acc) {
     for (int i = rb + (int)threadIdx.x; i < re; i += BLOCK) {
         int c[W];
         double v[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            c[k] = ci[(size_t)k * n + i];
+            if constexpr (std::is_same<CI, int16_t>::value) c[k] = i + (int)ci[(size_t)k * n + i];
+            else c[k] = ci[(size_t)k * n + i];
             v[k] = (double)av[(size_t)k * n + i];
         }
         double s = 0.0;
@@ -164,8 +169,8 @@ __device__ inline void ell_rows(int rb, int re, int n, const int32_t* __restrict
     }
 }
 
-template <int W, class T, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __restrict__ ci, const T* __restrict__ av,
+template <int W, class T, class Epi, class CI = int32_t>
+__global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const CI* __restrict__ ci, const T* __restrict__ av,
                                                      Epi epi, Red red, int* tflag, EnsWalk ew) {
     constexpr int ND = Epi::ND;
     constexpr int NA = ND > 0 ? ND : 1;
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __rest
             // contiguous chunk per workgroup, a multiple of BLOCK rows
             const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
             const int rb = xcd_block(blockIdx.x, gridDim.x) * chunk;
-            ell_rows<W, T>(rb, min(n, rb + chunk), n, ci, av, epi, acc);
+            ell_rows<W, T, CI>(rb, min(n, rb + chunk), n, ci, av, epi, acc);
         }
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow);
         return;
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const int32_t* __rest
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
-        ell_rows<W, T>(tb.x, tb.y, n, ci, av, e, acc);
+        ell_rows<W, T, CI>(tb.x, tb.y, n, ci, av, e, acc);
         if constexpr (ND > 0) store_partials<ND>(acc, red.partials, red.prow, t);
     }
 }

@@ -174,12 +174,14 @@ bool build_fan(const HostPattern& P, int W, std::vector<int32_t>& col, std::vect
         // fan triangle u joins slots u and u + 1: closed, triangle 0 is the
         // last link (last neighbour -> first) and triangle u the u-th; open,
         // triangle u (1 <= u < F) is the u-th link
+        uint32_t las = 0;
         for (int u = 0; u < nq; ++u) {
             const int slot = closed ? (u + 1) % nq : u + 1;   // triangle index of link ord[u]
             inc[(size_t)slot * n + i] = la_[ord[u]];
+            las |= (uint32_t)(la_[ord[u]] & 3) << (5 + 2 * slot);
         }
         pred[i] = closed ? nb[F] : i;
-        info[i] = (uint32_t)F | (closed ? 16u : 0u);
+        info[i] = (uint32_t)F | (closed ? 16u : 0u) | las;
     }
     return true;
 }

@@ -29,7 +29,8 @@ struct HostPattern {
 // boundary node, F - 1 triangles) has no triangle 0.  Tables in ELL layout
 // (entry k of row i at k * n + i): col [W][n] (padding: the node itself),
 // inc [iw][n] = element << 2 | the node's local vertex of fan triangle u
-// (padding 0), pred [n], info [n] = F | closed << 4.  Returns false (no fan
+// (padding 0), pred [n], info [n] = F | closed << 4 | (local vertex of the
+// node in fan triangle u) << (5 + 2u).  Returns false (no fan
 // order) unless every row is one fan of consistently oriented triangles
 // whose nodes are exactly the row's columns, with at most W - 1 neighbours.
 bool build_fan(const HostPattern& P, int W, std::vector<int32_t>& col, std::vector<int32_t>& inc,
