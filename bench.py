@@ -378,13 +378,13 @@ def run_headline(args):
         la, ma, ra = mixed["asm"]
         out["roofline"] = roofline("k_assemble_reg (fused P1 assembly + Dirichlet rows + Jacobi + refinement "
                                    "residual, one thread per row)", assembly_bytes(n, ne, nnz, 2, False), n,
-                                   la, ma, ra, "assembly_c3")
+                                   la, ma, ra, "assembly_c3_mixed")
         ls, ms, rs = mixed["spmv"]
         out["roofline_spmv"] = roofline(f"k_spmv_{mixed['format']} (FP32 SpMV of the inner solves, fused "
                                         "BiCGSTAB epilogues)", spmv_bytes(n, nnz, 4), n, ls, ms, rs, "spmv_c3_fp32")
         ls, ms, rs = f64r["spmv"]
         out["roofline_spmv_fp64"] = roofline(f"k_spmv_{f64r['format']} (FP64 SpMV, batched FP64 form at tol 1e-10)",
-                                             spmv_bytes(n, nnz, 8), n, ls, ms, rs)
+                                             spmv_bytes(n, nnz, 8), n, ls, ms, rs, "spmv_c3_fp64")
         cfg["timing_region"] = (f"{args.steps} further steps of the mixed form with HIP events around every "
                                 f"assembly and FP32 SpMV launch ({mixed['el_ev'] / args.steps * 1e3:.3f} ms/step "
                                 "there, groups stepped one after another)")
@@ -415,9 +415,9 @@ def run_ensemble_form(args):
                       "launch": launch_note()}}
     if "asm" in r:
         out["roofline"] = roofline("k_assemble_reg (fused assembly)", assembly_bytes(n, ne, nnz, 2, False), n,
-                                   *r["asm"], "assembly_c3")
+                                   *r["asm"], "assembly_c3_mixed" if precision == "mixed" else "assembly_c3")
         out["roofline_spmv"] = roofline(f"k_spmv_{r['format']}", spmv_bytes(n, nnz, 4 if precision == "mixed" else 8),
-                                        n, *r["spmv"])
+                                        n, *r["spmv"], None if precision == "mixed" else "spmv_c3_fp64")
     print(json.dumps(out))
 
 
@@ -521,10 +521,10 @@ def run_single(args):
            "config": cfgd,
            "roofline": roofline(f"k_assemble{'_reg' if dim == 2 else ''} (fused P1 assembly + Dirichlet rows + "
                                 "Jacobi + solver start)", assembly_bytes(n, ne, nnz, dim, ksf is not None), n,
-                                *asm, f"assembly_{args.workload.split('-')[0]}"),
+                                *asm, "assembly_c1" if args.workload == "c1" else None),
            "roofline_spmv": roofline(f"k_spmv_{fmt} ({'FP32' if mixed else 'FP64'} SpMV, fused Krylov epilogues)",
                                      spmv_bytes(n, nnz, 4 if mixed else 8), n, *sp,
-                                     f"spmv_{args.workload.split('-')[0]}" if not mixed else None)}
+                                     "spmv_c1" if args.workload == "c1" else None)}
     if args.cpu_steps > 0:
         pb_cpu = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", ks_field=ksf,
                                                 nsteps=args.cpu_steps, dt=dt))
