@@ -2585,10 +2585,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             if (std::isnan(err)) break;
             continue;
         }
-        if (mixed) {
-            ensure_mixed(ctx);
-            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        }
+        if (mixed) ensure_mixed(ctx);   // the assembly's residual start copies h_m into hB
         // the FP64 Krylov solve (constructed here so that a finish-free
         // BiCGSTAB can take over the assembly start's finish)
         std::unique_ptr<KrylovRun<double>> krun;
@@ -3384,7 +3381,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     while (true) {
         if (mixed) {
             ensure_mixed(ctx);
-            HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            // the assembly's residual start also copies h_m into the refinement's iterate hB
             do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
                         jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr, mode, ctx->hB, P.tol,
                         P.atol, P.maxiter, P.restart, ctx->ell);

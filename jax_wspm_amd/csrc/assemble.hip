@@ -167,6 +167,7 @@ __device__ inline void krylov_start(const AsmArgs& a, const KrylovInit& ki, int 
         acc[1] += mb * mb;
         acc[2] += w * w;
     } else if constexpr (MODE == ASM_INIT_RESID) {
+        if (ki.x != nullptr) ki.x[i] = ki.x0[i];   // the refinement's iterate starts at x0
         ki.r[i] = r;
         acc[0] += r * r;
         acc[1] += bi * bi;
