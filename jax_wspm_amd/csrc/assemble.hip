@@ -217,12 +217,46 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     if constexpr (MODE != ASM_INIT_NONE)
         for (int k = threadIdx.x; k < wlen; k += ASM_THREADS) xw[k] = ki.x0[a.cols[base + k]];
 
-    // 1. element row contributions of every incidence of the workgroup
-    for (int t = threadIdx.x; t < nq; t += ASM_THREADS) {
-        double out[D + 3];
-        slw[t] = element_row<D>(a, q0 + t, out);
+    // 1. element row contributions of every incidence of the workgroup, two
+    //    per thread and pass: the incidence, element and record gathers of
+    //    both are issued together (three dependent round trips per pair)
+    for (int t0 = threadIdx.x; t0 < nq; t0 += 2 * ASM_THREADS) {
+        const int t1 = t0 + ASM_THREADS;
+        const bool two = t1 < nq;
+        const int2 ic0 = a.inc[q0 + t0];
+        const int2 ic1 = two ? a.inc[q0 + t1] : ic0;
+        const int4 cn0 = a.conn4[ic0.x >> 2];
+        const int4 cn1 = a.conn4[ic1.x >> 2];
+        const int nd0[4] = {cn0.x, cn0.y, cn0.z, cn0.w};
+        const int nd1[4] = {cn1.x, cn1.y, cn1.z, cn1.w};
+        double4 r0[K], r1[K];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
+        for (int c = 0; c < K; ++c) {
+            r0[c] = a.nrec[nd0[c]];
+            r1[c] = a.nrec[nd1[c]];
+        }
+        const double ks0 = a.Ks_elem != nullptr ? a.Ks_elem[ic0.x >> 2] : 1.0;
+        const double ks1 = a.Ks_elem != nullptr ? a.Ks_elem[ic1.x >> 2] : 1.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !two) break;
+            const double4* r = h == 0 ? r0 : r1;
+            double px[K], py[K], pz[K], kn[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                px[c] = r[c].x;
+                py[c] = r[c].y;
+                pz[c] = (D == 3) ? r[c].z : 0.0;
+                kn[c] = (D == 3) ? r[c].w : r[c].z;
+            }
+            double out[D + 3];
+            const int2 ic = h == 0 ? ic0 : ic1;
+            element_compute<D>(a, px, py, pz, kn, h == 0 ? ks0 : ks1, ic.x & 3, out);
+            const int t = h == 0 ? t0 : t1;
+            slw[t] = (uint32_t)ic.y;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
+        }
     }
     __syncthreads();
 
@@ -245,11 +279,15 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
             bi = a.bc_val[r0 + j];
         } else {
             double msum = 0.0;
+            // an incidence's vertices are distinct, so at most one of its slot
+            // bytes equals sl: one zero-byte test finds it (the lowest flagged
+            // byte is the zero one; bytes beyond K are set to 0xff)
+            const uint32_t pat = sl * 0x01010101u;
+            const uint32_t pad = K == 4 ? 0u : 0xff000000u;
             for (int t = ip_s[j]; t < ip_s[j + 1]; ++t) {
-                const uint32_t w = slw[t];
-#pragma unroll
-                for (int c = 0; c < K; ++c)
-                    if (((w >> (8 * c)) & 0xffu) == sl) v += con[t * NC + c];
+                const uint32_t m = (slw[t] | pad) ^ pat;
+                const uint32_t hz = (m - 0x01010101u) & ~m & 0x80808080u;
+                if (hz != 0u) v += con[t * NC + ((__builtin_ctz(hz) - 7) >> 3)];
                 if (diag) {
                     bi += con[t * NC + K];
                     msum += con[t * NC + K + 1];
