@@ -40,6 +40,18 @@ __device__ inline D3 dpow(const D3& a, const D3& b) {
     for (int k = 0; k < 3; ++k) r.d[k] = p * (b.d[k] * la + b.v * a.d[k] / a.v);
     return r;
 }
+// log(a), d = a' / a; exp(a), d = e^a a'
+__device__ inline D3 dlog(const D3& a) {
+    D3 r{log(a.v), {}};
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] / a.v;
+    return r;
+}
+__device__ inline D3 dexp(const D3& a) {
+    const double e = exp(a.v);
+    D3 r{e, {}};
+    for (int k = 0; k < 3; ++k) r.d[k] = e * a.d[k];
+    return r;
+}
 // log1p(a), d = a' / (1 + a); expm1(a), d = e^a a'
 __device__ inline D3 dlog1p(const D3& a) {
     D3 r{log1p(a.v), {}};
@@ -72,13 +84,18 @@ __device__ inline void soil_dual(const SoilParams& sp, double hv, D3& th, D3& kr
     const D3 n{sp.p[3], {0.0, 0.0, 1.0}};
     const D3 one = dconst(1.0);
     const D3 m = one - one / n;
+    // the forward's steps: s = (a|h|)^n, L = log(1+s) and Q = log(1+1/s)
+    // from one log1p, Se = exp(-m L), 1 - (1 - Se^(1/m))^m = -expm1(-m Q)
     const D3 ah = alpha * (-h);
-    const D3 ahn = dpow(ah, n);
-    const D3 base = one + ahn;
-    const D3 bm = dpow(base, -m);
-    th = dconst(ts - tr) * bm + dconst(tr);
-    const D3 se = (th - dconst(tr)) / dconst(ts - tr);
-    const D3 inner = -dexpm1(m * dlog1p(-(one / base)));
+    const D3 nl = n * dlog(ah);
+    const D3 ahn = dexp(nl);
+    const bool big = ahn.v >= 1.0;
+    const D3 lp = dlog1p(big ? one / ahn : ahn);
+    const D3 L = big ? nl + lp : lp;
+    const D3 Q = big ? lp : lp - nl;
+    const D3 se = dexp(-(m * L));
+    th = dconst(ts - tr) * se + dconst(tr);
+    const D3 inner = -dexpm1(-(m * Q));
     kr = dsqrt(se) * (inner * inner);
 }
 

@@ -237,29 +237,38 @@ __device__ inline void soil_eval(const SoilParams& sp, double h, double& C, doub
     if (sp.model == 1) {
         const double alpha = sp.p[0], ts = sp.p[1], tr = sp.p[2], n = sp.p[3], m = sp.p[4], ks = sp.p[5];
         if (h < 0.0) {
-            // no pow: (a|h|)^n = exp(n log(a|h|)), (1+(a|h|)^n)^(-m) =
-            // exp(-m log1p((a|h|)^n)) (pow's extended-precision logarithm
-            // made this kernel compute-bound: 9.2 against 7.5 us per 263k
-            // nodes on MI355X, the two forms within 5e-15 relative of each
-            // other), (a|h|)^(n-1) = (a|h|)^n / (a|h|),
-            // (1+(a|h|)^n)^(-m-1) = (1+(a|h|)^n)^(-m) / (1+(a|h|)^n),
-            // Se^0.5 = sqrt(Se).  The Mualem term
-            // 1 - (1 - Se^(1/m))^m, with Se^(1/m) = 1 / (1+(a|h|)^n), is taken as
-            // -expm1(m log1p(-1/(1+(a|h|)^n))): the reference's form cancels
-            // when Se is small (dry soil) and carries the rounding of
-            // theta - theta_r; this one does neither.  K agrees with the
-            // reference's expression to its own rounding error (about 1e-12
-            // relative where K is above 1e-16 Ks, measured).
+            // With s = (a|h|)^n = exp(n log(a|h|)) (no pow: its
+            // extended-precision logarithm made this kernel compute-bound),
+            // L = log(1+s) and Q = log(1+1/s) = L - log s, both from ONE
+            // log1p without cancellation: s >= 1: Q = log1p(1/s), L = n
+            // log(a|h|) + Q (two non-negative terms); s < 1: L = log1p(s),
+            // Q = L - n log(a|h|) (likewise).  Then
+            //   Se = (1+s)^(-m) = exp(-m L),  theta = (ts-tr) Se + tr,
+            //   1 - (1 - Se^(1/m))^m = 1 - (s/(1+s))^m = -expm1(-m Q),
+            //   K = Ks sqrt(Se) (1 - (1 - Se^(1/m))^m)^2,
+            //   C = (ts-tr) a m n s/(a|h|) Se/(1+s) (one division).
+            // The reference's Mualem form cancels when Se is small (dry soil)
+            // and takes Se back from theta - theta_r; this one does neither:
+            // C, K and theta are within 1e-14 relative of 40-digit values over
+            // a in [0.01, 0.2], n in [1.1, 3], |h| in [1e-3, 1e4] (measured;
+            // the reference's K form is within 1e-11 there), and it costs six
+            // transcendental calls and two divisions per node instead of
+            // seven and four (a Picard update writing contiguous arrays over
+            // 7.2M nodes: 136 -> 102 us on MI355X).
             const double ah = alpha * fabs(h);
-            const double ahn = exp(n * log(ah));
-            const double base = 1.0 + ahn;
-            const double bm = exp(-m * log1p(ahn));
-            th = (ts - tr) * bm + tr;
-            const double se = (th - tr) / (ts - tr);
-            const double inner = -expm1(m * log1p(-1.0 / base));
+            const double nl = n * log(ah);
+            const double ahn = exp(nl);
+            const bool big = ahn >= 1.0;
+            const double lp = log1p(big ? 1.0 / ahn : ahn);
+            const double L = big ? nl + lp : lp;
+            const double Q = big ? lp : lp - nl;
+            const double se = exp(-m * L);
+            th = (ts - tr) * se + tr;
+            const double inner = -expm1(-m * Q);
             K = ks * (sqrt(se) * (inner * inner));
-            const double ahn1 = ah > 0.0 ? ahn / ah : pow(ah, n - 1.0);
-            C = (ts - tr) * alpha * m * n * ahn1 * (bm / base);
+            // a|h| == 0 (underflow): s = 0, Se = 1, s/(a|h|) -> (a|h|)^(n-1)
+            const double g = ah > 0.0 ? (ahn * se) / (ah * (1.0 + ahn)) : pow(ah, n - 1.0);
+            C = (ts - tr) * alpha * m * n * g;
         } else {
             th = ts;
             K = ks;
