@@ -33,7 +33,7 @@ struct AdjArgs {
     const int32_t* inc_ptr;
     const int2* inc;
     const int4* conn4;
-    const double4* nrec;
+    NodeRecs nrec;
     const uint8_t* bc_flag;     // nullable
     const double* ks_e;         // [E]
     const double* h;            // h_k
@@ -83,7 +83,7 @@ struct TrAdjArgs {
     const int32_t* inc_ptr;
     const int2* inc;
     const int4* conn4;
-    const double4* nrec;
+    NodeRecs nrec;
     TrQuad tq;
     const double* ks_e;         // [E]
     const double* h;            // h_k

@@ -141,7 +141,7 @@ __device__ inline void load_element(const Args& a, int e, int (&nd)[3], double (
     nd[2] = cn.z;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double4 r = a.nrec[nd[c]];
+        const double2 r = a.nrec.xy(nd[c]);
         px[c] = r.x;
         py[c] = r.y;
     }

@@ -61,7 +61,7 @@ bool env_is(const char* name, const char* value) {
 // ---------------------------------------------------------------------------
 // Soil / Picard kernels
 // ---------------------------------------------------------------------------
-// K goes to K[i] and/or to the node record (krec[4 i], stride of double4).
+// K goes to K[i] and/or to the assembly's nodal K (krec[i]).
 // ens: ensemble mode, the instance's soil parameters come from its state.
 struct SoilOp {
     static constexpr int ND = 0;
@@ -79,7 +79,7 @@ struct SoilOp {
         soil_eval(sp, h[i], c, k, t);
         C[i] = c; th[i] = t;
         if (K) K[i] = k;
-        if (krec) krec[4 * (size_t)i] = k;
+        if (krec) krec[i] = k;
     }
     __device__ void fin(const double*, KState*) const {}
 };
@@ -118,7 +118,7 @@ struct PicardOp {
         soil_eval(sp, hi, c, k, t);
         C[i] = c; th[i] = t;
         if (K) K[i] = k;
-        if (krec) krec[4 * (size_t)i] = k;
+        if (krec) krec[i] = k;
     }
     __device__ void fin(const double* t, KState* s) const {
         s->picard_d2 = t[0];
@@ -472,13 +472,13 @@ struct RefCorrect {  // x += |r| d32
     __device__ void fin(const double*, KState*) const {}
 };
 
-struct SetK {  // node records' K from a caller array
+struct SetK {  // the assembly's nodal K from a caller array
     static constexpr int ND = 0;
     const double* K;
     double* krec;
     __device__ bool skip(const KState*) const { return false; }
     __device__ void load(const KState*) {}
-    __device__ void row(int i, double*) const { krec[4 * (size_t)i] = K[i]; }
+    __device__ void row(int i, double*) const { krec[i] = K[i]; }
     __device__ void fin(const double*, KState*) const {}
 };
 
@@ -673,8 +673,9 @@ struct wspm_ctx {
     int32_t *row_ptr = nullptr, *cols = nullptr, *diag_pos = nullptr, *inc_ptr = nullptr;
     int2* inc = nullptr;            // (element << 2 | vertex, slot word) per incidence
     int4* conn4 = nullptr;          // padded connectivity
-    double4* nrec = nullptr;        // (x, y, K, -) / (x, y, z, K) node records
-    bool nrec_dirty = false;        // K in nrec changed by a direct assembly call
+    double* ncoord = nullptr;       // node coordinates: (x, y) / (x, y, z, -) per node (NodeRecs)
+    double* Kn = nullptr;           // nodal conductivity read by the assembly (NodeRecs)
+    bool nrec_dirty = false;        // Kn changed by a direct assembly call
     uint8_t* bc_flag = nullptr;
     double* bc_val = nullptr;
     int n_bc = 0;
@@ -806,8 +807,9 @@ P* dupload(wspm_ctx* ctx, const P* src, size_t count) {
     return d;
 }
 
-// K slot of the node records: component 2 (2D) or 3 (3D) of each double4.
-double* krec_of(wspm_ctx* ctx) { return reinterpret_cast<double*>(ctx->nrec) + (ctx->dim == 2 ? 2 : 3); }
+// The nodal K the assembly reads, and the node data of the element kernels.
+double* krec_of(wspm_ctx* ctx) { return ctx->Kn; }
+NodeRecs nrec_of(const wspm_ctx* ctx) { return NodeRecs{ctx->ncoord, ctx->Kn, ctx->dim}; }
 
 // Address of a field of a device-resident KState (no host dereference).
 template <class F>
@@ -1623,12 +1625,13 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
                 c4[e] = make_int4(c[0], c[1], c[2], dim == 3 ? c[3] : c[0]);
             }
             ctx->conn4 = dupload(ctx, c4.data(), c4.size());
-            std::vector<double4> nr(n_nodes);
-            for (int i = 0; i < n_nodes; ++i) {
-                const double* x = coords + (size_t)i * dim;
-                nr[i] = make_double4(x[0], x[1], dim == 3 ? x[2] : 0.0, 0.0);
-            }
-            ctx->nrec = dupload(ctx, nr.data(), nr.size());
+            const int cw = dim == 2 ? 2 : 4;
+            std::vector<double> nc((size_t)n_nodes * cw, 0.0);
+            for (int i = 0; i < n_nodes; ++i)
+                for (int d = 0; d < dim; ++d) nc[(size_t)i * cw + d] = coords[(size_t)i * dim + d];
+            ctx->ncoord = dupload(ctx, nc.data(), nc.size());
+            ctx->Kn = dalloc<double>(ctx, n_nodes);
+            HIPCHK(hipMemset(ctx->Kn, 0, (size_t)n_nodes * sizeof(double)));
         }
         ctx->bc_flag = dalloc<uint8_t>(ctx, n_nodes);
         ctx->bc_val = dalloc<double>(ctx, n_nodes);
@@ -1754,7 +1757,7 @@ static void launch_edge_list(wspm_ctx* ctx, int list, int kind, double flux, dou
     e.slot_prev = L.slot_prev;
     e.slot_next = L.slot_next;
     e.diag_pos = ctx->diag_pos;
-    e.nrec = ctx->nrec;
+    e.nrec = nrec_of(ctx);
     e.kind = kind;
     e.flux = flux;
     e.c_in = c_in;
@@ -1782,8 +1785,8 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.win_cap = ctx->asm_win;
     a.inc_cap = ctx->asm_inc;
     a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos; a.inc_ptr = ctx->inc_ptr;
-    a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
-    if (K) launch_vec(ctx, SetK{K, krec_of(ctx)}, 0);   // caller's K into the node records
+    a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = nrec_of(ctx);
+    if (K) launch_vec(ctx, SetK{K, krec_of(ctx)}, 0);   // caller's K into the assembly's nodal K
     a.Ks_elem = Ks_elem; a.C = C; a.theta_m = theta_m; a.theta_n = theta_n; a.h_m = h_m;
     a.inv_dt = 1.0 / dt;
     a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
@@ -2443,7 +2446,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const bool mixed = P.precision == WSPM_MIXED;
     const bool jac = P.precond == WSPM_PRECOND_JACOBI;
     const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
-    if (ctx->nrec_dirty) {  // a direct assembly call replaced K in the node records
+    if (ctx->nrec_dirty) {  // a direct assembly call replaced the nodal K
         launch_vec(ctx, SoilOp{ctx->sp, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx), 0}, 0);
         ctx->nrec_dirty = false;
     }
@@ -2605,8 +2608,8 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
             launch_edge_list(ctx, 0, EDGE_NEUMANN, ctx->rich_flux, 0.0, nullptr, 0, nullptr, nullptr, ctx->bv,
                              nullptr);
-        if (ctx->rich_freedrain)   // nitrogen_solver.py:121-125, K(h_m) from the node records
-            launch_edge_list(ctx, 1, EDGE_FREEDRAIN_WATER, 0.0, 0.0, krec_of(ctx), 4, nullptr, nullptr, ctx->bv,
+        if (ctx->rich_freedrain)   // nitrogen_solver.py:121-125, K(h_m) from the nodal K
+            launch_edge_list(ctx, 1, EDGE_FREEDRAIN_WATER, 0.0, 0.0, krec_of(ctx), 1, nullptr, nullptr, ctx->bv,
                              nullptr);
         wspm_krylov_stats st{};
         const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
@@ -2813,7 +2816,7 @@ int32_t wspm_transport_fields(wspm_ctx* ctx, const double* phi, const double* h,
     a.inc_ptr = ctx->inc_ptr;
     a.inc = ctx->inc;
     a.conn4 = ctx->conn4;
-    a.nrec = ctx->nrec;
+    a.nrec = nrec_of(ctx);
     a.h = h;
     a.ks_e = Ks_elem;
     a.sp = make_soil(WSPM_SOIL_VAN_GENUCHTEN, phi);
@@ -2840,7 +2843,7 @@ int32_t wspm_assemble_transport(wspm_ctx* ctx, const wspm_transport_params* tp, 
     TrArgs a{};
     a.n = ctx->n;
     a.row_ptr = ctx->row_ptr; a.cols = ctx->cols; a.diag_pos = ctx->diag_pos;
-    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
+    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = nrec_of(ctx);
     a.th = theta; a.qx = qx; a.qz = qz; a.dxx = dxx; a.dxz = dxz; a.dzz = dzz;
     a.c_n = c_n;
     a.c_src = c_src;
@@ -2925,7 +2928,7 @@ int32_t wspm_adj_step(wspm_ctx* ctx, const wspm_adj_params* prm, const double* K
     a.inc_ptr = ctx->inc_ptr;
     a.inc = ctx->inc;
     a.conn4 = ctx->conn4;
-    a.nrec = ctx->nrec;
+    a.nrec = nrec_of(ctx);
     a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
     a.ks_e = Ks_elem;
     a.h = h;
@@ -3006,7 +3009,7 @@ int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, co
     auto fields = [&](const double* hh, double* out) {
         FieldArgs f{};
         f.n = n;
-        f.inc_ptr = ctx->inc_ptr; f.inc = ctx->inc; f.conn4 = ctx->conn4; f.nrec = ctx->nrec;
+        f.inc_ptr = ctx->inc_ptr; f.inc = ctx->inc; f.conn4 = ctx->conn4; f.nrec = nrec_of(ctx);
         f.h = hh; f.ks_e = Ks_elem; f.sp = sp;
         f.DL = prm->DL; f.DT = prm->DT; f.Dm = prm->Dm;
         f.qx = out; f.qz = out + N; f.aq = out + 2 * N; f.dxx = out + 3 * N; f.dxz = out + 4 * N;
@@ -3017,7 +3020,7 @@ int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, co
     const double* F = W.f;
     TrAdjArgs a{};
     a.n = n; a.E = E;
-    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = ctx->nrec;
+    a.inc_ptr = ctx->inc_ptr; a.inc = ctx->inc; a.conn4 = ctx->conn4; a.nrec = nrec_of(ctx);
     a.tq = ctx->tq;
     a.ks_e = Ks_elem;
     a.h = h;
@@ -3042,7 +3045,7 @@ int32_t wspm_adj_transport_step(wspm_ctx* ctx, const wspm_adj_tr_params* prm, co
         TrArgs t{};
         t.n = n;
         t.row_ptr = ctx->row_ptr; t.cols = ctx->cols; t.diag_pos = ctx->diag_pos;
-        t.inc_ptr = ctx->inc_ptr; t.inc = ctx->inc; t.conn4 = ctx->conn4; t.nrec = ctx->nrec;
+        t.inc_ptr = ctx->inc_ptr; t.inc = ctx->inc; t.conn4 = ctx->conn4; t.nrec = nrec_of(ctx);
         t.th = F + 6 * N; t.qx = F; t.qz = F + N; t.dxx = F + 3 * N; t.dxz = F + 4 * N; t.dzz = F + 5 * N;
         t.c_n = c_prev + s * N;
         t.c_src = s > 0 ? c + (s - 1) * N : nullptr;
@@ -3095,10 +3098,15 @@ static constexpr int ENS_GRID = 2048;
 // number of workgroups that walk the entry lists.  Every listed entry (256
 // rows of one instance) is worked by one workgroup and writes its own
 // partial-row column, so the grid decides who works an entry, not what is
-// summed.  Candidates 256 ... 8192 workgroups (at most one per entry),
+// summed.  Candidates 8192 ... 256 workgroups (at most one per entry),
 // timed on the whole operator (zero values) with the time-step format: 2
-// warm-up and 10 timed launches each, the fastest kept (ties: the smaller
-// grid).  WSPM_ENS_GRID fixes the grid instead.
+// warm-up and 10 timed launches each; a smaller grid is kept only when it is
+// more than 3 % faster.  (The solves' launches walk short lists -- 8 % of the
+// entries on average over a 256-instance time step -- where a large grid
+// keeps every entry on a workgroup of its own; on the whole operator the
+// grids are within a few per cent of each other, and a smaller grid that won
+// that probe by noise cost 2-5 % per time step.)  WSPM_ENS_GRID fixes the
+// grid instead.
 static void ens_autotune(wspm_ctx* ctx) {
     const char* env = std::getenv("WSPM_LANES");
     if (env) {
@@ -3132,9 +3140,11 @@ static void ens_autotune(wspm_ctx* ctx) {
         const EpiPlain<double> epi{xy, xy + ctx->n};
         int best = std::min(ENS_GRID, ctx->n_tab), cands = 0;
         float best_ms = 1e30f;
-        for (int grid = 256; grid <= 8192; grid *= 2) {
+        int last = -1;
+        for (int grid = 8192; grid >= 256; grid /= 2) {
             const int gg = std::min(grid, ctx->n_tab);
-            if (grid > 256 && gg != grid) break;   // the table has fewer entries
+            if (gg == last) continue;               // the table has fewer entries
+            last = gg;
             ctx->ens_grid = gg;
             for (int r = 0; r < 2; ++r) launch_spmv<double>(ctx, av, epi, 0);
             HIPCHK(hipEventRecord(e0, ctx->stream));
@@ -3144,7 +3154,7 @@ static void ens_autotune(wspm_ctx* ctx) {
             float ms = 0.0f;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ++cands;
-            if (ms < best_ms * 0.98f) {
+            if (ms < best_ms * 0.97f) {
                 best_ms = ms;
                 best = gg;
             }
@@ -3529,7 +3539,7 @@ static DiagArgs diag_args(wspm_ctx* ctx) {
     DiagArgs a{};
     a.E = ctx->E;
     a.conn4 = ctx->conn4;
-    a.nrec = ctx->nrec;
+    a.nrec = nrec_of(ctx);
     a.partials = ctx->partials;
     a.prow = ctx->prow;
     a.grid = std::max(1, std::min((ctx->E + BLOCK - 1) / BLOCK, ctx->prow));

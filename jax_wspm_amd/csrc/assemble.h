@@ -6,6 +6,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "wspm_device.h"
+
 namespace wspm {
 
 struct KState;
@@ -47,9 +49,8 @@ struct QuadConst {
 //   inc[q]   = (element << 2 | local vertex, in-row offsets of the element's
 //              vertices, one byte each) for the q-th incidence;
 //   conn4[e] = element vertices, padded to 4;
-//   nrec[i]  = node record (x, y, K, -) in 2D, (x, y, z, K) in 3D, where K is
-//              the nodal conductivity (or Kr with a per-element Ks) written by
-//              the soil-model kernels.
+//   nrec     = node coordinates and the nodal conductivity K (or Kr with a
+//              per-element Ks) written by the soil-model kernels (NodeRecs).
 // One incidence then costs 1 + 1 + 2 (dim+1) (+1 for Ks_e) vector loads.
 struct AsmArgs {
     int n;
@@ -61,7 +62,7 @@ struct AsmArgs {
     const int32_t* inc_ptr;
     const int2* inc;
     const int4* conn4;
-    const double4* nrec;
+    NodeRecs nrec;
     const double* Ks_elem;   // nullable
     const double* C;         // row-node values (read once per row)
     const double* theta_m;

@@ -126,7 +126,7 @@ __device__ inline uint32_t element_row(const AsmArgs& a, int q, double (&out)[D 
     double px[K], py[K], pz[K], kn[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        const double4 r = a.nrec[nd[c]];
+        const double4 r = a.nrec.template rec<D>(nd[c]);
         px[c] = r.x;
         py[c] = r.y;
         pz[c] = (D == 3) ? r.z : 0.0;
@@ -232,8 +232,8 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
         double4 r0[K], r1[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            r0[c] = a.nrec[nd0[c]];
-            r1[c] = a.nrec[nd1[c]];
+            r0[c] = a.nrec.template rec<D>(nd0[c]);
+            r1[c] = a.nrec.template rec<D>(nd1[c]);
         }
         const double ks0 = a.Ks_elem != nullptr ? a.Ks_elem[ic0.x >> 2] : 1.0;
         const double ks1 = a.Ks_elem != nullptr ? a.Ks_elem[ic1.x >> 2] : 1.0;
@@ -368,7 +368,7 @@ __device__ inline void assemble_row(const AsmArgs& a, const KrylovInit& ki, int 
                 const int nd[4] = {cn[u].x, cn[u].y, cn[u].z, cn[u].w};
 #pragma unroll
                 for (int c = 0; c < K; ++c)
-                    nr[u][c] = (qb + u < qe) ? a.nrec[nd[c]] : make_double4(0.0, 0.0, 0.0, 0.0);
+                    nr[u][c] = (qb + u < qe) ? a.nrec.template rec<D>(nd[c]) : make_double4(0.0, 0.0, 0.0, 0.0);
                 ks[u] = (a.Ks_elem != nullptr && qb + u < qe) ? a.Ks_elem[ic[u].x >> 2] : 1.0;
             }
 #pragma unroll
@@ -471,7 +471,7 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     double xv[MAXR];
 #pragma unroll
     for (int s = 0; s < MAXR; ++s) {
-        nr[s] = (!dir && s < rl) ? a.nrec[col[s]] : make_double4(0.0, 0.0, 0.0, 0.0);
+        nr[s] = (!dir && s < rl) ? a.nrec.rec<2>(col[s]) : make_double4(0.0, 0.0, 0.0, 0.0);
         if constexpr (MODE != ASM_INIT_NONE) xv[s] = s < rl ? ki.x0[col[s]] : 0.0;
     }
     double ks[MAXI];
@@ -599,8 +599,8 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
     // and a zero record compiles into a flat load through a selected pointer.)
     double4 nr[W];
 #pragma unroll
-    for (int s = 0; s < W; ++s) nr[s] = a.nrec[col[s]];
-    const double4 np = a.nrec[pr];
+    for (int s = 0; s < W; ++s) nr[s] = a.nrec.rec<2>(col[s]);
+    const double4 np = a.nrec.rec<2>(pr);
     double xv[W];
     if constexpr (MODE != ASM_INIT_NONE) {
 #pragma unroll

@@ -13,7 +13,7 @@ namespace wspm {
 struct DiagArgs {
     int E;
     const int4* conn4;        // padded connectivity
-    const double4* nrec;      // (x, y, ...) node records
+    NodeRecs nrec;      // node coordinates (xy)
     double* partials;         // column-major [2][prow]
     int prow;
     int grid;                 // workgroups of the element pass (<= prow)

@@ -28,7 +28,7 @@ __device__ inline void gather_xy(const DiagArgs& a, int e, int (&nd)[3], double 
     nd[2] = c.z;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double4 r = a.nrec[nd[k]];
+        const double2 r = a.nrec.xy(nd[k]);
         px[k] = r.x;
         py[k] = r.y;
     }

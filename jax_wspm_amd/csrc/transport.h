@@ -30,7 +30,7 @@ struct FieldArgs {
     const int32_t* inc_ptr;
     const int2* inc;
     const int4* conn4;
-    const double4* nrec;
+    NodeRecs nrec;
     const double* h;
     const double* ks_e;       // nullable: per-element Ks; K = soil K(h) * mean of the incident Ks_e
     SoilParams sp;
@@ -48,7 +48,7 @@ struct TrArgs {
     const int32_t* inc_ptr;
     const int2* inc;
     const int4* conn4;
-    const double4* nrec;
+    NodeRecs nrec;
     const double *th, *qx, *qz, *dxx, *dxz, *dzz;
     const double* c_n;        // concentration at the start of the step
     const double* c_src;      // coupling species (nullable)
@@ -68,7 +68,7 @@ struct EdgeArgs {
     const int32_t* slot_prev;   // CSR position of (nodes[k], nodes[k-1]), -1 for k = 0
     const int32_t* slot_next;   // CSR position of (nodes[k], nodes[k+1]), -1 for the last
     const int32_t* diag_pos;
-    const double4* nrec;        // x coordinate in .x
+    NodeRecs nrec;        // node coordinates (xy)
     int kind;
     double flux, c_in;
     const double* K;            // K[i * kstride]

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_transport_fields(FieldArgs a) {
         double x[3], y[3], hv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double4 r = a.nrec[nd[c]];
+            const double2 r = a.nrec.xy(nd[c]);
             x[c] = r.x;
             y[c] = r.y;
             hv[c] = a.h[nd[c]];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_transport(TrArgs a) {
         double px[3], py[3], th[3], qx[3], qz[3], dxx[3], dxz[3], dzz[3], cs[3], cnv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double4 r = a.nrec[nd[c]];
+            const double2 r = a.nrec.xy(nd[c]);
             px[c] = r.x;
             py[c] = r.y;
             th[c] = a.th[nd[c]];
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_transport(TrArgs a) {
 // row) and m[1] to (row, other); returns the b contribution.
 __device__ inline double edge_row(const EdgeArgs& a, int I, int J, int l, double (&m)[2]) {
     const double xi[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
-    const double jac = (a.nrec[J].x - a.nrec[I].x) / 2;
+    const double jac = (a.nrec.xy(J).x - a.nrec.xy(I).x) / 2;
     m[0] = m[1] = 0.0;
     double bl = 0.0;
     for (int j = 0; j < 2; ++j) {

@@ -101,6 +101,33 @@ __device__ inline void block_sum(double (&v)[ND], double* sh) {
     }
 }
 
+// Node data of the element kernels: static coordinates (2D: (x, y) as
+// double2, 16 bytes a node; 3D: (x, y, z, -) as double4) and the nodal
+// conductivity K in an array of its own, rewritten by the soil-model kernels
+// every Picard iteration.  (K inside a 32-byte coordinate record made those
+// stores partial-sector writes: a Picard update over 7.2M nodes took 145
+// against 102 us with K contiguous, on MI355X.)  rec<D>(j) is (x, y, K, -)
+// in 2D and (x, y, z, K) in 3D; xy(j) the first two coordinates.
+struct NodeRecs {
+    const double* coord;
+    const double* K;
+    int dim;
+    template <int D>
+    __device__ double4 rec(int j) const {
+        if constexpr (D == 2) {
+            const double2 p = reinterpret_cast<const double2*>(coord)[j];
+            return make_double4(p.x, p.y, K[j], 0.0);
+        } else {
+            double4 r = reinterpret_cast<const double4*>(coord)[j];
+            r.w = K[j];
+            return r;
+        }
+    }
+    __device__ double2 xy(int j) const {
+        return *reinterpret_cast<const double2*>(coord + (size_t)j * (dim == 2 ? 2 : 4));
+    }
+};
+
 struct Red {
     double* partials;     // column-major [MAX_DOTS][prow]: dot j of workgroup b at j * prow + b
     int prow;
