@@ -282,15 +282,36 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
             // an incidence's vertices are distinct, so at most one of its slot
             // bytes equals sl: one zero-byte test finds it (the lowest flagged
             // byte is the zero one; bytes beyond K are set to 0xff)
+            // Four incidences per pass: their slot words, then the matching
+            // contributions (+0.0 where none matches, which leaves the sum
+            // bitwise as it is: the entries start at +0.0 and so never become
+            // -0.0), each issued together, so a pass costs two LDS round
+            // trips instead of two per incidence; the sum still runs in
+            // incidence order.
             const uint32_t pat = sl * 0x01010101u;
             const uint32_t pad = K == 4 ? 0u : 0xff000000u;
-            for (int t = ip_s[j]; t < ip_s[j + 1]; ++t) {
-                const uint32_t m = (slw[t] | pad) ^ pat;
-                const uint32_t hz = (m - 0x01010101u) & ~m & 0x80808080u;
-                if (hz != 0u) v += con[t * NC + ((__builtin_ctz(hz) - 7) >> 3)];
+            const int te = ip_s[j + 1];
+            for (int t = ip_s[j]; t < te; t += 4) {
+                uint32_t hz[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t m = ((t + u < te ? slw[t + u] : 0xffffffffu) | pad) ^ pat;
+                    hz[u] = (m - 0x01010101u) & ~m & 0x80808080u;
+                }
+                double c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    c[u] = hz[u] != 0u ? con[(t + u) * NC + ((__builtin_ctz(hz[u]) - 7) >> 3)] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v += c[u];
                 if (diag) {
-                    bi += con[t * NC + K];
-                    msum += con[t * NC + K + 1];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (t + u < te) {
+                            bi += con[(t + u) * NC + K];
+                            msum += con[(t + u) * NC + K + 1];
+                        }
+                    }
                 }
             }
             if (diag) {
