@@ -78,6 +78,8 @@ WORKLOADS = {
            {"sigma": 0.5, "length": 0.05, "seed": 0}, 1e-3),
 }
 DEFAULT_STEPS = {"c2": 100, "c2-fp64": 100, "c3": 100, "c3-fp64": 100, "c3-mixed": 100, "c4": 200}
+# oracle steps of the CPU baseline: about 10-20 s of one host core each
+DEFAULT_CPU_STEPS = {"c1": 10, "c1-mixed": 10, "c2": 3, "c2-fp64": 3, "c3": 15}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
@@ -87,7 +89,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 100; c1: 200; c4: 200)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle steps for the CPU baseline (0 = skip; default: about 10-20 s of CPU work)")
     ap.add_argument("--no-timing", action="store_true", help="skip the event-timed roofline region")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (a kernel of known traffic for rocprofv3 --pmc "
@@ -105,6 +108,8 @@ def parse():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = DEFAULT_STEPS.get(args.workload, 200)
+    if args.cpu_steps is None:
+        args.cpu_steps = DEFAULT_CPU_STEPS.get(args.workload, 0)
     return args
 
 
