@@ -98,6 +98,10 @@ def parse():
     ap.add_argument("--slow-group", type=lambda v: [int(k) for k in v.split(",")], default=[8],
                     help="ensembles: instances (the slowest Picard loops of a probe) stepped in a second context "
                          "and stream; a,b: the a slowest and the next b in two more; 0 = one context")
+    ap.add_argument("--inner", choices=["krylov", "jacobi"], default="jacobi",
+                    help="mixed forms: FP32 inner solves of each refinement sweep (jacobi: Jacobi-preconditioned "
+                         "Richardson steps; krylov: the method's BiCGSTAB / CG to the inner tolerance)")
+    ap.add_argument("--inner-iters", type=int, default=3, help="--inner jacobi: steps per refinement sweep")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: timed steps per instance of the plain single-instance path (0 = skip)")
     ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
@@ -210,14 +214,16 @@ def _ens_groups(insts, args):
     return EnsembleGroups.split_slowest(counts, args.slow_group)
 
 
-def _ens_form(insts, groups, precision, tol, args, timing=True):
+def _ens_form(insts, groups, precision, tol, args, timing=True, inner=None):
     """One batched form: warm-up, timed steps, final state, then the
     event-timed roofline region.  Returns a dict of its figures."""
     import torch
     from jax_wspm_amd.ensemble import Ensemble, EnsembleGroups
     os.environ["WSPM_ENS_INNER"] = str(MIXED_INNER_ITERS)
+    inner = inner or args.inner
     ens = EnsembleGroups(insts, groups) if groups else Ensemble(insts)
-    ens.richards_init("vg", "bicgstab", "jacobi", precision=precision, tol=tol, inner_tol=MIXED_INNER_TOL)
+    ens.richards_init("vg", "bicgstab", "jacobi", precision=precision, tol=tol, inner_tol=MIXED_INNER_TOL,
+                      inner=inner, inner_iters=args.inner_iters if inner == "jacobi" else 0)
     for _ in range(args.warmup):
         ens.step(DT)
     h_warm = ens.state()[0].cpu().numpy()
@@ -238,7 +244,8 @@ def _ens_form(insts, groups, precision, tol, args, timing=True):
            "h_warm": h_warm, "n": ens.n, "n_elems": ens.n_elems, "nnz": ens.nnz, "format": ens.spmv_format,
            "shape": ens.shape}
     assert all(bool(torch.isfinite(x).all()) for x in (h, th)), "non-finite state"
-    log(f"{precision} tol {tol:g}: {out['rate']:.0f} instance-steps/s, {out['krylov']:.1f} summed Krylov its/step")
+    log(f"{precision}{' (' + inner + ' inner)' if precision == 'mixed' else ''} tol {tol:g}: {out['rate']:.0f} "
+        f"instance-steps/s, {out['krylov']:.1f} summed Krylov its/step")
     if timing and not args.no_timing:
         seq = (lambda d: ens.step(d, concurrent=False)) if groups else ens.step
         sp, asm, el_ev = timed_region(ens, seq, DT, args.steps, precision == "mixed")
@@ -320,6 +327,16 @@ def _cpu_ensemble_baseline(insts, h_warm, node_off, args):
             "krylov_iters_per_instance_step": round(kry / ns, 3)}
 
 
+def _inner_desc(args, inner=None):
+    inner = inner or args.inner
+    if inner == "jacobi":
+        return {"method": "jacobi", "steps_per_sweep": args.inner_iters,
+                "what": "Jacobi-preconditioned Richardson steps d <- d + M (r - A d) in FP32 storage from d = 0, "
+                        "no dot products; sweeps until the FP64 residual meets the Krylov tolerance"}
+    return {"method": "krylov", "inner_tol": MIXED_INNER_TOL, "max_iters_per_sweep": MIXED_INNER_ITERS,
+            "what": "the method's FP32 Krylov solve per refinement sweep"}
+
+
 def run_headline(args):
     """c3: batched mixed vs batched FP64 vs plain FP64 on the 256-instance ensemble."""
     import numpy as np
@@ -329,6 +346,9 @@ def run_headline(args):
     groups = _ens_groups(insts, args)
     log(f"c3: 256 instances, groups {[len(g) for g in groups] if groups else [256]}")
     mixed = _ens_form(insts, groups, "mixed", TIGHT_TOL, args)
+    # the other inner method, for comparison (no timing region)
+    other = "krylov" if args.inner == "jacobi" else "jacobi"
+    mixed_o = _ens_form(insts, groups, "mixed", TIGHT_TOL, args, timing=False, inner=other)
     f64t = _ens_form(insts, groups, "fp64", TIGHT_TOL, args, timing=False)
     f64r = _ens_form(insts, groups, "fp64", 1e-10, args)
     plain_r = plain_t = None
@@ -344,11 +364,17 @@ def run_headline(args):
         "workload": WORKLOADS["c3"][0], "instances": len(insts), "nodes": n, "elements": ne, "nnz": nnz, "dt": DT,
         "value_counts": "instance time steps of the batched mixed-precision form (each timed step advances every "
                         "instance once)",
-        "mixed": {"krylov_tol": TIGHT_TOL, "inner_tol": MIXED_INNER_TOL, "inner_iters_per_sweep": MIXED_INNER_ITERS,
+        "mixed": {"krylov_tol": TIGHT_TOL, "inner": _inner_desc(args),
                   "instance_steps_per_s": round(mixed["rate"], 1),
                   "max_picard_iters_per_step": round(mixed["picard"], 3),
-                  "fp32_krylov_iters_per_step_summed": round(mixed["krylov"], 1),
+                  "fp32_inner_iters_per_step_summed": round(mixed["krylov"], 1),
                   "all_converged": mixed["converged"]},
+        f"mixed_{other}_inner": {"inner": _inner_desc(args, other),
+                                 "instance_steps_per_s": round(mixed_o["rate"], 1),
+                                 "fp32_inner_iters_per_step_summed": round(mixed_o["krylov"], 1),
+                                 "all_converged": mixed_o["converged"],
+                                 "vs_fp64_same_tol": _err_summary(mixed_o["h"], f64t["h"], mixed_o["theta"],
+                                                                  f64t["theta"])},
         "batched_fp64": {"krylov_tol_1e-10_instance_steps_per_s": round(f64r["rate"], 1),
                          f"krylov_tol_{TIGHT_TOL:g}_instance_steps_per_s": round(f64t["rate"], 1),
                          "krylov_iters_per_step_summed_tol_1e-10": round(f64r["krylov"], 1),
@@ -385,8 +411,9 @@ def run_headline(args):
                                    "residual, one thread per row)", assembly_bytes(n, ne, nnz, 2, False), n,
                                    la, ma, ra, "assembly_c3_mixed")
         ls, ms, rs = mixed["spmv"]
-        out["roofline_spmv"] = roofline(f"k_spmv_{mixed['format']} (FP32 SpMV of the inner solves, fused "
-                                        "BiCGSTAB epilogues)", spmv_bytes(n, nnz, 4), n, ls, ms, rs, "spmv_c3_fp32")
+        epi = "Jacobi-step epilogues" if args.inner == "jacobi" else "BiCGSTAB epilogues"
+        out["roofline_spmv"] = roofline(f"k_spmv_{mixed['format']} (FP32 SpMV of the inner solves, fused {epi})",
+                                        spmv_bytes(n, nnz, 4), n, ls, ms, rs, "spmv_c3_fp32")
         ls, ms, rs = f64r["spmv"]
         out["roofline_spmv_fp64"] = roofline(f"k_spmv_{f64r['format']} (FP64 SpMV, batched FP64 form at tol 1e-10)",
                                              spmv_bytes(n, nnz, 8), n, ls, ms, rs, "spmv_c3_fp64")
@@ -474,6 +501,8 @@ def run_single(args):
         pb.tol = ktol
         if prec == "mixed":
             pb.inner_tol = MIXED_INNER_TOL
+            pb.inner = args.inner
+            pb.inner_iters = args.inner_iters if args.inner == "jacobi" else 0
         rs = RichardsSolver(pb, history="last")
         rs.init_state()
         return pb, rs.ctx
@@ -513,7 +542,7 @@ def run_single(args):
             c64.richards_step(dt)
         h64, th64, _ = (x.cpu().numpy() for x in c64.richards_state())
         c64.close()
-        cfgd["inner_tol"] = MIXED_INNER_TOL
+        cfgd["inner"] = _inner_desc(args)
         cfgd["rel_l2_vs_fp64_h"] = float(np.linalg.norm(h - h64) / np.linalg.norm(h64))
         cfgd["rel_l2_vs_fp64_theta"] = float(np.linalg.norm(th - th64) / np.linalg.norm(th64))
         cfgd["error_sample"] = (f"after {args.warmup + args.steps} steps, against the FP64 form at the same Krylov "

@@ -83,6 +83,12 @@ enum { WSPM_DIRECT = 3 };
 enum { WSPM_PRECOND_NONE = 0, WSPM_PRECOND_JACOBI = 1, WSPM_PRECOND_BLOCK_JACOBI = 2,
        WSPM_PRECOND_SSOR = 3, WSPM_PRECOND_ILU = 4 };
 enum { WSPM_FP64 = 0, WSPM_MIXED = 1 };
+/* Mixed precision: the FP32 inner solve of each refinement sweep.  KRYLOV:
+ * the method's own CG / BiCGSTAB / GMRES to inner_tol.  JACOBI: inner_iters
+ * Jacobi-preconditioned Richardson steps d <- d + M (r - A d) from d = 0 (no
+ * dot products; needs the Jacobi preconditioner), switched to KRYLOV for the
+ * rest of the context's life once a sweep fails to halve the FP64 residual. */
+enum { WSPM_INNER_KRYLOV = 0, WSPM_INNER_JACOBI = 1 };
 enum { WSPM_PICARD_REL = 0, WSPM_PICARD_ABS = 1 };
 
 typedef struct {
@@ -118,6 +124,8 @@ typedef struct {
     int32_t picard_maxit;
     int32_t picard_mode;  /* WSPM_PICARD_REL | WSPM_PICARD_ABS               */
     double inner_tol;     /* mixed precision: FP32 inner relative tolerance   */
+    int32_t inner_method; /* mixed precision: WSPM_INNER_KRYLOV | _JACOBI     */
+    int32_t inner_iters;  /* WSPM_INNER_JACOBI: steps per sweep (0: 3; <= 16) */
 } wspm_richards_params;
 
 /* Library ------------------------------------------------------------------ */
@@ -368,7 +376,9 @@ int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylo
  * time-step operator is ELL (0: CSR), [1] its width (ELL) or lanes per row
  * (CSR), [2] workgroups walking the entry lists, [3] rows per entry, [4]
  * entries, [5] grids the autotuner timed (0: fixed by WSPM_ENS_GRID), [6]
- * the chosen grid's SpMV time over the whole operator in nanoseconds, [7] 0.
+ * the chosen grid's SpMV time over the whole operator in nanoseconds, [7] 1
+ * if mixed-precision Jacobi inner steps have stalled and Krylov inner solves
+ * took over (WSPM_INNER_JACOBI), else 0.
  * The format follows from the meshes alone, so results are repeatable; the
  * tuned grid changes no bit of them. */
 int32_t wspm_ens_shape(const wspm_ctx* ctx, int32_t* out);

@@ -18,6 +18,7 @@ METHOD = {"cg": 0, "bicgstab": 1, "gmres": 2}
 STEP_METHOD = dict(METHOD, direct=3)       # Picard steps also take the banded-LU direct solve
 PRECOND = {"none": 0, None: 0, "jacobi": 1, "block_jacobi": 2, "ssor": 3, "ilu": 4}
 PRECISION = {"fp64": 0, "mixed": 1}
+INNER = {"krylov": 0, "jacobi": 1}
 PICARD = {"rel": 0, "abs": 1}
 
 
@@ -42,7 +43,7 @@ class RichardsParams(C.Structure):
                 ("precision", C.c_int32), ("phi", C.c_double * 6), ("tol", C.c_double),
                 ("atol", C.c_double), ("maxiter", C.c_int32), ("restart", C.c_int32),
                 ("picard_tol", C.c_double), ("picard_maxit", C.c_int32), ("picard_mode", C.c_int32),
-                ("inner_tol", C.c_double)]
+                ("inner_tol", C.c_double), ("inner_method", C.c_int32), ("inner_iters", C.c_int32)]
 
 
 class TransportParams(C.Structure):

@@ -56,6 +56,10 @@ def parse_args(argv=None):
     p.add_argument("--maxiter", type=int, default=100, help="Krylov max iterations (GMRES: cycles)")
     p.add_argument("--restart", type=int, default=30, help="GMRES restart")
     p.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    p.add_argument("--inner", choices=["krylov", "jacobi"], default="krylov",
+                   help="mixed precision: FP32 inner solves of each refinement sweep (jacobi: Richardson steps, "
+                        "needs --preconditioner jacobi)")
+    p.add_argument("--inner-iters", type=int, default=0, help="--inner jacobi: steps per refinement sweep (0: 3)")
     p.add_argument("--nsteps", type=int, default=None, help="fixed-dt mode: exactly this many steps")
     p.add_argument("--cells", type=int, default=None, help="VGSquare/VGCube cells per side / Test3D cells")
     p.add_argument("--ks-sigma", type=float, default=None, help="VG cases: per-element log-normal Ks std")
@@ -88,7 +92,8 @@ def build_config(args) -> SimulationConfig:
     cfg = SimulationConfig(exponential=ExponentialParameters(), van_genuchten=vg, time=time, solver=solver,
                            test_case=args.test_case, save_frequency=args.save_frequency,
                            picard=PicardParameters(rule=args.picard_rule),
-                           device=DeviceParameters(precision=args.precision), problem=problem)
+                           device=DeviceParameters(precision=args.precision, inner=args.inner,
+                                                   inner_iters=args.inner_iters), problem=problem)
     cfg.mesh.mesh_size = args.mesh_size
     if args.mesh_dir:
         cfg.mesh.mesh_dir = args.mesh_dir
@@ -105,6 +110,8 @@ def run(args):
         raise ValueError("the direct solver is FP64 (--precision fp64)")
     if args.precision == "mixed" and args.preconditioner not in ("none", "jacobi"):
         raise NotImplementedError("mixed precision takes --preconditioner none or jacobi")
+    if args.precision == "mixed" and args.inner == "jacobi" and args.preconditioner != "jacobi":
+        raise ValueError("--inner jacobi needs --preconditioner jacobi")
     from .richards import RichardsSolver, RichardsSolver3D
 
     out_dir = Path(args.output_dir)

@@ -146,6 +146,8 @@ class PicardParameters:
 class DeviceParameters:
     precision: str = "fp64"          # "fp64" | "mixed"
     inner_tol: float = 1e-6          # FP32 inner relative tolerance in mixed mode
+    inner: str = "krylov"            # mixed mode inner solves: "krylov" | "jacobi"
+    inner_iters: int = 0             # "jacobi": Richardson steps per refinement sweep (0: 3)
 
     def to_dict(self) -> Dict:
         return asdict(self)

@@ -102,6 +102,10 @@ struct PicardOp {
     int rel;
     int ens;
     const int* gate = nullptr;   // one system: *gate == 0 skips this update (an iteration enqueued ahead)
+    // one system, mixed precision: the refinement's state, whose convergence
+    // flag, stall flag and Jacobi inner steps this update's state takes over
+    // (rdone, rstall, kry), so that one read-back tells the host all three
+    const KState* ref = nullptr;
     __device__ bool skip(const KState* s) const {
         return (gate != nullptr && *gate == 0) || (ens && (s->pdone || !s->done || s->pev));
     }
@@ -124,6 +128,11 @@ struct PicardOp {
         s->picard_d2 = t[0];
         s->picard_h2 = t[1];
         s->picard_err = rel ? sqrt(t[0]) / sqrt(t[1]) : sqrt(t[0]);
+        if (ref != nullptr) {
+            s->rdone = ref->rdone;
+            s->rstall = ref->rstall;
+            s->kry = ref->steps;
+        }
         if (ens) {
             s->pev = 1;
             s->pit += 1;
@@ -209,30 +218,36 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __res
 
 // Counts of running instances (one workgroup): out[0] instances whose Krylov
 // solve is still running, out[1] instances still iterating Picard, out[2] the
-// most Krylov steps of a solve.  hseq non-null: out is pinned host memory and
+// most Krylov steps of a solve, out[3] instances still iterating Picard whose
+// Jacobi refinement sweep has stalled.  hseq non-null: out is pinned host memory and
 // the counts are published as by k_publish.
 __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__ st, int n_inst, int* out,
                                                       int* dseq, int* hseq) {
     __shared__ int cnt[3];
     if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
     __syncthreads();
-    int kact = 0, pact = 0, ms = 0;
+    __shared__ int stalled;
+    if (threadIdx.x == 0) stalled = 0;
+    int kact = 0, pact = 0, ms = 0, sta = 0;
     for (int i = threadIdx.x; i < n_inst; i += BLOCK) {
         if (!st[i].pdone) {
             ++pact;
             if (!st[i].done) ++kact;
+            if (st[i].rstall) ++sta;
         }
         ms = max(ms, st[i].steps);
     }
+    __syncthreads();
     atomicAdd(&cnt[0], kact);
     atomicAdd(&cnt[1], pact);
     atomicMax(&cnt[2], ms);
+    atomicAdd(&stalled, sta);
     __syncthreads();
     if (threadIdx.x == 0) {
         out[0] = cnt[0];
         out[1] = cnt[1];
         out[2] = cnt[2];
-        out[3] = 0;
+        out[3] = stalled;
     }
     if (hseq != nullptr) publish_seq(dseq, hseq);
 }
@@ -316,6 +331,11 @@ __global__ void k_ens_collect(const KState* __restrict__ st, int n_inst, int* ip
 
 // Iterative-refinement sweeps per solve (single system and ensembles).
 constexpr int MAX_REFINE = 50;
+// Jacobi inner steps per refinement sweep: at most, and by default.
+constexpr int MAX_JAC_STEPS = 16;
+constexpr int JAC_STEPS = 3;
+// A Jacobi sweep stalls unless it brings |r| below this fraction of |r| before it.
+constexpr double JAC_STALL = 0.5;
 
 // Ensemble mixed precision, per instance: FP64 residual r = b - A x of the
 // instances still refining; the finish records |r| and |b| and marks the
@@ -329,6 +349,12 @@ struct RefResidE {
     double* r;
     double tol, atol;
     int first;
+    // stall > 0: the residual after a sweep of Jacobi inner steps (steps of
+    // them); rstall is raised when |r| has not fallen below stall times the
+    // previous |r| and the refinement goes on.  The steps add to the
+    // iteration count.
+    double stall = 0.0;
+    int steps = 0;
     __device__ bool skip(const KState* s) const { return !first && s->rdone; }
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return x[j]; }
@@ -340,12 +366,96 @@ struct RefResidE {
         acc[1] += bi * bi;
     }
     __device__ void fin(const double* t, KState* s) const {
+        const double prev = s->rnorm;
         s->rnorm = sqrt(t[0]);
         s->bnorm = sqrt(t[1]);
         s->refine = first ? 0 : s->refine + 1;
         s->rdone = !(s->rnorm > fmax(tol * s->bnorm, atol)) || s->refine >= MAX_REFINE;
         s->done = s->rdone;
+        if (first) {
+            s->rstall = 0;
+            s->steps = 0;
+        } else if (stall > 0.0) {
+            s->steps += steps;
+            if (!s->rdone && !(s->rnorm <= stall * prev)) s->rstall = 1;
+        }
     }
+};
+
+// Mixed precision with Jacobi inner steps.  A refinement sweep solves the
+// correction equation A d = r~ (r~ = r / |r| in FP32, A and M = diag(A)^-1 in
+// FP32) by k steps of d <- d + M (r~ - A d) from d = 0, with no dot products,
+// then adds |r| d to the FP64 iterate.  The first step's d = M r~ is formed at
+// the gathered columns of the second step, and the last step adds its result
+// to x, so a sweep is k - 1 SpMV launches.  On the time-step operators of
+// small dt (diagonally dominant: |I - M A| about 1e-3 to 1e-2) three steps
+// reach the FP32 floor of the correction; the FP64 residual after the sweep
+// decides whether another sweep follows.  gated: the state says whether the
+// refinement has converged (rdone), and then the launch does nothing.
+struct JacFirst {     // d = M r~ + M (r~ - A M r~); x += |r| d (x set) or d, r~ stored
+    static constexpr int ND = 0;
+    const double* r;
+    const float* dinv;
+    float* r32;
+    float* dout;
+    double* x;
+    int gated;
+    double inv, scale;
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ void load(const KState* s) {
+        scale = s->rnorm;
+        inv = scale > 0.0 ? 1.0 / scale : 0.0;
+    }
+    __device__ float rt(int j) const { return (float)(r[j] * inv); }
+    __device__ double xv(int j) const { return (double)(dinv[j] * rt(j)); }
+    __device__ void row(int i, double ax, double*) const {
+        const float ri = rt(i);
+        const float mi = dinv[i];
+        const double d = (double)(mi * ri) + (double)mi * ((double)ri - ax);
+        if (x != nullptr) {
+            x[i] += scale * d;
+        } else {
+            r32[i] = ri;
+            dout[i] = (float)d;
+        }
+    }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct JacOnly {      // one step: x += |r| M r~
+    static constexpr int ND = 0;
+    const double* r;
+    const float* dinv;
+    double* x;
+    int gated;
+    double inv, scale;
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ void load(const KState* s) {
+        scale = s->rnorm;
+        inv = scale > 0.0 ? 1.0 / scale : 0.0;
+    }
+    __device__ void row(int i, double*) const { x[i] += scale * (double)(dinv[i] * (float)(r[i] * inv)); }
+    __device__ void fin(const double*, KState*) const {}
+};
+
+struct JacStep {      // d' = d + M (r~ - A d); x += |r| d' (x set) or d' stored
+    static constexpr int ND = 0;
+    const float* din;
+    const float* r32;
+    const float* dinv;
+    float* dout;
+    double* x;
+    int gated;
+    double scale;
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ void load(const KState* s) { scale = s->rnorm; }
+    __device__ double xv(int j) const { return (double)din[j]; }
+    __device__ void row(int i, double ax, double*) const {
+        const double d = (double)din[i] + (double)dinv[i] * ((double)r32[i] - ax);
+        if (x != nullptr) x[i] += scale * d;
+        else dout[i] = (float)d;
+    }
+    __device__ void fin(const double*, KState*) const {}
 };
 
 struct RefScaleE {  // r32 = r / |r|, d32 = 0 on the instances whose inner solve starts
@@ -401,28 +511,6 @@ struct RefCorrectE {  // x += |r| d32 on the instances refined in this sweep
 template <class Op>
 struct ActiveOnly : Op {
     __device__ bool skip(const KState* s) const { return s->done; }
-};
-
-// Mixed precision: FP64 residual r = b - A x, |r|, |b|
-struct RefResid {
-    static constexpr int ND = 2;
-    const double* b;
-    const double* x;
-    double* r;
-    __device__ bool skip(const KState*) const { return false; }
-    __device__ void load(const KState*) {}
-    __device__ double xv(int j) const { return x[j]; }
-    __device__ void row(int i, double ax, double* acc) const {
-        const double bi = b[i];
-        const double ri = bi - ax;
-        r[i] = ri;
-        acc[0] += ri * ri;
-        acc[1] += bi * bi;
-    }
-    __device__ void fin(const double* t, KState* s) const {
-        s->rnorm = sqrt(t[0]);
-        s->bnorm = sqrt(t[1]);
-    }
 };
 
 struct RefScale {  // r32 = r / |r|; d32 = 0
@@ -691,6 +779,9 @@ struct wspm_ctx {
     // mixed precision
     double* ref_r = nullptr;
     float *ref_r32 = nullptr, *ref_d32 = nullptr, *A32 = nullptr, *dinv32 = nullptr;
+    float* ref_d32b = nullptr;      // second iterate of the Jacobi inner steps (more than 3)
+    bool jac_inner_off = false;     // a Jacobi sweep stalled: Krylov inner solves from then on
+    double jac_stall = 0.5;         // a Jacobi sweep stalls unless |r| falls below this times |r| before it
     // Richards state
     bool rich_init = false;
     bool has_ks = false;
@@ -1868,7 +1959,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
             if (ctx->tab)
                 launch_finish<RefResidE, true>(ctx, RefResidE{b, x, ctx->ref_r, tol, atol, 1}, grid, 0);
             else
-                launch_finish<RefResid, true>(ctx, RefResid{b, x, ctx->ref_r}, grid, 2);
+                launch_finish<RefResidE, true>(ctx, RefResidE{b, x, ctx->ref_r, tol, atol, 1}, grid, 2);
             break;
     }
     if (ctx->tab) ens_compact(ctx, LIST_K);
@@ -2136,24 +2227,79 @@ static void ensure_mixed(wspm_ctx* ctx) {
         ctx->ref_r = dalloc<double>(ctx, ctx->n);
         ctx->ref_r32 = dalloc<float>(ctx, ctx->n);
         ctx->ref_d32 = dalloc<float>(ctx, ctx->n);
+        ctx->ref_d32b = dalloc<float>(ctx, ctx->n);
     }
 }
 
-// started: the first residual r = b - A x and its norms are already in
-// ctx->ref_r / state 2 (assembly epilogue).
+static int env_int(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::max(1, std::atoi(e)) : dflt;
+}
+
+// Jacobi inner steps for the mixed-precision solves of a time step: asked for,
+// Jacobi-preconditioned, and no sweep of this context has stalled yet.
+static bool jacobi_on(const wspm_ctx* ctx, const float* dinv32) {
+    return ctx->prm.inner_method == WSPM_INNER_JACOBI && dinv32 != nullptr && !ctx->jac_inner_off;
+}
+
+// Refinement sweeps enqueued per read-back with Jacobi inner steps (each is a
+// few launches; the ones after convergence return at once).  On the
+// 256-instance ensemble 2, 3, 4, 6, 8 and 12 sweeps of 3 steps per read-back
+// measured 60.9k-65.7k instance-steps/s, within one box's spread from 3 on;
+// WSPM_JAC_SWEEPS overrides it.
+static int jac_sweeps() { return env_int("WSPM_JAC_SWEEPS", 3); }
+
+// One refinement sweep with Jacobi inner steps: ctx->prm.inner_iters steps on
+// the correction, x += |r| d, then the FP64 residual r = b - A x with its
+// finish into state `which` (ensembles: per instance, compact_k rebuilds the
+// list of instances still refining).  Every launch skips a refinement that
+// has converged.
+static void jacobi_sweep(wspm_ctx* ctx, const double* A, const float* A32, const float* dinv32, const double* b,
+                         double* x, double tol, double atol, int which, bool compact_k) {
+    const int k = ctx->prm.inner_iters > 0 ? ctx->prm.inner_iters : JAC_STEPS;
+    const int list = ctx->tab ? LIST_K : LIST_ALL;
+    if (k == 1) {
+        launch_vec(ctx, JacOnly{ctx->ref_r, dinv32, x, 1, 0.0, 0.0}, which, list);
+    } else {
+        float* da = ctx->ref_d32;
+        float* db = ctx->ref_d32b;
+        launch_spmv<float>(ctx, A32, JacFirst{ctx->ref_r, dinv32, ctx->ref_r32, da, k == 2 ? x : nullptr, 1, 0.0, 0.0},
+                           which, true, list);
+        for (int j = 3; j <= k; ++j) {
+            launch_spmv<float>(ctx, A32, JacStep{da, ctx->ref_r32, dinv32, db, j == k ? x : nullptr, 1, 0.0}, which,
+                               true, list);
+            std::swap(da, db);
+        }
+    }
+    launch_spmv<double>(ctx, A, RefResidE{b, x, ctx->ref_r, tol, atol, 0, ctx->jac_stall, k}, which, true, list,
+                        nullptr, true, compact_k);
+}
+
+// Iterative refinement of one system: FP32 corrections of the FP64 residual
+// until |r| <= max(tol |b|, atol) or max_refine sweeps.  started: the first
+// residual r = b - A x and its norms are already in ctx->ref_r / state 2
+// (assembly epilogue, or sweeps enqueued by the caller behind it).  The inner
+// solves are the method's Krylov solve to inner_tol, or Jacobi steps
+// (jacobi_on, time steps only: jac), several sweeps per read-back.
 static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double* dinv, const float* A32,
                         const float* dinv32, const double* b, double* x, double tol, double atol,
                         int maxiter, int restart, double inner_tol, int max_refine, wspm_krylov_stats* st,
-                        bool started = false) {
+                        bool started = false, bool jac = false) {
     ensure_mixed(ctx);
     int inner_total = 0, arn = 0, sweeps = 0;
     KState& hs = ctx->hstate[2];
+    if (!started) launch_spmv<double>(ctx, A, RefResidE{b, x, ctx->ref_r, tol, atol, 1}, 2, true);
     while (true) {
-        if (!started) launch_spmv<double>(ctx, A, RefResid{b, x, ctx->ref_r}, 2, true);
-        started = false;
         poll(ctx, 2);
         const double thresh = std::max(tol * hs.bnorm, atol);
         if (!(hs.rnorm > thresh) || sweeps >= max_refine) break;
+        if (hs.rstall) ctx->jac_inner_off = true;
+        if (jac && jacobi_on(ctx, dinv32)) {
+            const int ns = std::min(jac_sweeps(), max_refine - sweeps);
+            for (int k = 0; k < ns; ++k) jacobi_sweep(ctx, A, A32, dinv32, b, x, tol, atol, 2, false);
+            sweeps += ns;
+            continue;
+        }
         wspm_krylov_stats ist{};
         if (method == WSPM_GMRES) {
             launch_vec(ctx, RefScale{ctx->ref_r, ctx->ref_r32, ctx->ref_d32, 0.0}, 2);
@@ -2181,10 +2327,11 @@ static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double
         inner_total += ist.iters;
         arn += ist.arnoldi;
         launch_vec(ctx, RefCorrect{x, ctx->ref_d32, 0.0}, 2);
+        launch_spmv<double>(ctx, A, RefResidE{b, x, ctx->ref_r, tol, atol, 0}, 2, true);
         ++sweeps;
     }
     if (st) {
-        st->iters = inner_total;
+        st->iters = inner_total + hs.steps;   // state 2 counts the Jacobi steps since the start
         st->arnoldi = arn;
         st->code = 0;
         st->refinements = sweeps;
@@ -2240,6 +2387,13 @@ static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm)
         return fail(ctx, WSPM_EINVAL, "unknown precision");
     if (prm->precision == WSPM_MIXED && prm->precond > WSPM_PRECOND_JACOBI)
         return fail(ctx, WSPM_EINVAL, "mixed precision takes no preconditioner or Jacobi");
+    if (prm->inner_method != WSPM_INNER_KRYLOV && prm->inner_method != WSPM_INNER_JACOBI)
+        return fail(ctx, WSPM_EINVAL, "unknown inner method");
+    if (prm->precision == WSPM_MIXED && prm->inner_method == WSPM_INNER_JACOBI &&
+        prm->precond != WSPM_PRECOND_JACOBI)
+        return fail(ctx, WSPM_EINVAL, "Jacobi inner iterations need the Jacobi preconditioner");
+    if (prm->inner_iters < 0 || prm->inner_iters > MAX_JAC_STEPS)
+        return fail(ctx, WSPM_EINVAL, "inner_iters must be in [0, 16]");
     if (prm->picard_maxit < 1) return fail(ctx, WSPM_EINVAL, "picard_maxit must be >= 1");
     return WSPM_OK;
 }
@@ -2373,6 +2527,13 @@ static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const
         ctx->dinv32 = dalloc<float>(ctx, n);
     }
     ctx->prm = *prm;
+    ctx->jac_inner_off = false;
+    {
+        // WSPM_JAC_STALL overrides the stall ratio (a tiny one forces the
+        // switch to Krylov inner solves at the first sweep that does not converge)
+        const char* js = std::getenv("WSPM_JAC_STALL");
+        ctx->jac_stall = js ? std::atof(js) : JAC_STALL;
+    }
     // the time-step solves read the operator in ELL order when the rows allow
     ctx->ell = ell_ok(ctx, prm);
     if (ctx->ell) {
@@ -2602,7 +2763,8 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         }
         const bool defer = krun && krun->fold && P.method == WSPM_BICGSTAB;
         do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
-                    (jac && !direct) ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr, (mixed && jac) ? ctx->dinv32 : nullptr,
+                    (jac && !direct && !mixed) ? ctx->dinv : nullptr, mixed ? ctx->A32 : nullptr,
+                    (mixed && jac) ? ctx->dinv32 : nullptr,
                     mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, ctx->ell, defer ? &start_grid : nullptr);
         if (defer) krun->start_in = FoldIn{ctx->partials, ctx->prow, start_grid};
         if (ctx->rich_neumann)   // src/solvers/nitrogen_solver.py:115-119
@@ -2619,9 +2781,33 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             picard_poll(ctx, pop);
             if (direct_info(ctx)) throw ApiFail{WSPM_EINVAL, "direct solve: singular matrix"};
         } else if (mixed) {
-            mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr,
-                        ctx->bv, ctx->hB, P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, !own_start);
-            picard_poll(ctx, pop);
+            const float* d32 = jac ? ctx->dinv32 : nullptr;
+            if (own_start)   // the refinement starts from h_m
+                HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            bool started = !own_start, refined = false;
+            if (jacobi_on(ctx, d32)) {
+                // Jacobi sweeps and the Picard update enqueued behind the
+                // residual start and read back once; if the refinement needs
+                // more sweeps, the update is redone after them
+                if (own_start)
+                    launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 1}, 2,
+                                        true);
+                started = true;
+                for (int k = 0; k < jac_sweeps(); ++k)
+                    jacobi_sweep(ctx, ctx->Av, ctx->A32, d32, ctx->bv, ctx->hB, P.tol, P.atol, 2, false);
+                PicardOp popr = pop;
+                popr.ref = states_of(ctx, 2);
+                picard_poll(ctx, popr);
+                refined = ctx->hstate[0].rdone != 0;
+                if (ctx->hstate[0].rstall) ctx->jac_inner_off = true;   // a sweep stalled
+                if (refined) st.iters = ctx->hstate[0].kry;
+            }
+            if (!refined) {
+                mixed_solve(ctx, P.method, ctx->Av, jac ? ctx->dinv : nullptr, ctx->A32, d32, ctx->bv, ctx->hB,
+                            P.tol, P.atol, P.maxiter, P.restart, P.inner_tol, 50, &st, started, true);
+                picard_poll(ctx, pop);
+            }
         } else {
             // Enqueue the expected number of Krylov iterations and the Picard
             // error behind them, then read both back with one synchronisation;
@@ -3335,10 +3521,6 @@ int32_t wspm_ens_richards_init(wspm_ctx* ctx, const wspm_richards_params* prm, c
 static constexpr int ENS_SWEEPS = 1;
 static constexpr int ENS_INNER = 2;
 
-static int env_int(const char* name, int dflt) {
-    const char* e = std::getenv(name);
-    return e ? std::max(1, std::atoi(e)) : dflt;
-}
 
 static void ens_mixed_solve(wspm_ctx* ctx, bool jac) {
     const wspm_richards_params& P = ctx->prm;
@@ -3392,14 +3574,39 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         if (mixed) {
             ensure_mixed(ctx);
             // the assembly's residual start also copies h_m into the refinement's iterate hB
-            do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv,
-                        jac ? ctx->dinv : nullptr, ctx->A32, jac ? ctx->dinv32 : nullptr, mode, ctx->hB, P.tol,
-                        P.atol, P.maxiter, P.restart, ctx->ell);
-            ens_mixed_solve(ctx, jac);
+            // (mixed precision reads the FP32 inverse diagonal only)
+            do_assemble(ctx, Ks, ctx->hA, ctx->Cn, nullptr, ctx->thm, ctx->thn, dt, ctx->Av, ctx->bv, nullptr,
+                        ctx->A32, jac ? ctx->dinv32 : nullptr, mode, ctx->hB, P.tol, P.atol, P.maxiter, P.restart,
+                        ctx->ell);
             const PicardOp pop{ctx->sp, ctx->hB, ctx->hA, ctx->Cn, nullptr, ctx->thm, krec_of(ctx),
                                P.picard_mode == WSPM_PICARD_REL, 1};
-            launch_vec(ctx, pop, 0, LIST_P);
-            poll_ens(ctx, 0);
+            if (jacobi_on(ctx, jac ? ctx->dinv32 : nullptr)) {
+                // Jacobi sweeps on the instances still refining, then the
+                // Picard update of those refined (the others skip it), read
+                // back once; more sweeps and updates while any instance refines
+                const bool fc = fused_compact(ctx);
+                const int ns = jac_sweeps();
+                while (true) {
+                    for (int k = 0; k < ns; ++k) {
+                        jacobi_sweep(ctx, ctx->Av, ctx->A32, ctx->dinv32, ctx->bv, ctx->hB, P.tol, P.atol, 0, fc);
+                        if (!fc) ens_compact(ctx, LIST_K);
+                    }
+                    launch_vec(ctx, pop, 0, LIST_P);
+                    poll_ens(ctx, 0);
+                    if (ctx->hstatus[3] > 0) ctx->jac_inner_off = true;   // a sweep stalled
+                    if (ctx->hstatus[0] == 0) break;
+                    if (!jacobi_on(ctx, ctx->dinv32)) {
+                        ens_mixed_solve(ctx, jac);   // Krylov sweeps for the rest
+                        launch_vec(ctx, pop, 0, LIST_P);
+                        poll_ens(ctx, 0);
+                        break;
+                    }
+                }
+            } else {
+                ens_mixed_solve(ctx, jac);
+                launch_vec(ctx, pop, 0, LIST_P);
+                poll_ens(ctx, 0);
+            }
             ens_compact(ctx, LIST_P);
             ++it;
             if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
@@ -3521,7 +3728,7 @@ int32_t wspm_ens_shape(const wspm_ctx* ctx, int32_t* out) {
     out[4] = ctx->n_tab;
     out[5] = ctx->ens_grid_cands;
     out[6] = (int32_t)std::lround(ctx->ens_tune_us * 1000.0f);
-    out[7] = 0;
+    out[7] = ctx->jac_inner_off ? 1 : 0;
     return WSPM_OK;
 }
 

@@ -36,7 +36,7 @@ struct KState {
     int32_t cycles, arnoldi, restart, nan_flag;
     int32_t jdone, refine;
     int32_t rdone;                           // ensemble mixed precision: refinement converged
-    int32_t pad1;
+    int32_t rstall;                          // mixed precision: a Jacobi sweep failed to halve |r|
     // Ensemble mode (one KState per instance): Picard control of the instance
     // within the current time step, and its soil parameters.
     int32_t pdone;                           // Picard converged or capped this step

@@ -166,7 +166,11 @@ class Context:
     # -- device-resident time stepping --------------------------------------------
     def richards_init(self, soil_model: str, phi, method: str, precond: Optional[str], h0, *,
                       Ks_elem=None, tol=1e-10, atol=0.0, maxiter=100, restart=30, picard_tol=1e-6,
-                      picard_maxit=100, picard_mode="rel", precision="fp64", inner_tol=1e-6):
+                      picard_maxit=100, picard_mode="rel", precision="fp64", inner_tol=1e-6,
+                      inner="krylov", inner_iters=0):
+        """inner (mixed precision): "krylov", the method's FP32 solve to inner_tol per
+        refinement sweep, or "jacobi", inner_iters Jacobi-preconditioned Richardson
+        steps per sweep (0: 3; needs precond "jacobi")."""
         if method not in _lib.STEP_METHOD:
             raise ValueError(f"Unsupported solver type: {method}")
         if precond not in _lib.PRECOND:
@@ -184,6 +188,8 @@ class Context:
         prm.picard_tol, prm.picard_maxit = float(picard_tol), int(picard_maxit)
         prm.picard_mode = _lib.PICARD[picard_mode]
         prm.inner_tol = float(inner_tol)
+        prm.inner_method = _lib.INNER[inner]
+        prm.inner_iters = int(inner_iters)
         h0 = _f64(h0)
         ks = _f64(Ks_elem) if Ks_elem is not None else None
         self._ck(self.lib.wspm_richards_init(self.h, C.byref(prm), ptr(ks), ptr(h0)))

@@ -117,9 +117,12 @@ class Ensemble:
 
     def richards_init(self, soil_model: str = "vg", method: str = "bicgstab", precond: Optional[str] = "jacobi", *,
                       tol=1e-10, atol=0.0, maxiter=100, picard_tol=1e-6, picard_maxit=100, picard_mode="rel",
-                      precision: str = "fp64", inner_tol: float = 1e-6):
-        """precision "mixed": FP32 Krylov corrections of the FP64 residual,
-        refined per instance until that residual meets ``tol``."""
+                      precision: str = "fp64", inner_tol: float = 1e-6, inner: str = "krylov",
+                      inner_iters: int = 0):
+        """precision "mixed": FP32 corrections of the FP64 residual, refined per
+        instance until that residual meets ``tol``; inner "krylov": the
+        method's FP32 solve to inner_tol per sweep, "jacobi": inner_iters
+        Jacobi-preconditioned Richardson steps per sweep (0: 3)."""
         if method not in ("cg", "bicgstab"):
             raise ValueError(f"ensembles solve with cg or bicgstab, not {method!r}")
         if precond not in _lib.PRECOND:
@@ -130,6 +133,8 @@ class Ensemble:
         prm.precond = _lib.PRECOND[precond]
         prm.precision = _lib.PRECISION[precision]
         prm.inner_tol = float(inner_tol)
+        prm.inner_method = _lib.INNER[inner]
+        prm.inner_iters = int(inner_iters)
         prm.tol, prm.atol, prm.maxiter, prm.restart = float(tol), float(atol), int(maxiter), 30
         prm.picard_tol, prm.picard_maxit = float(picard_tol), int(picard_maxit)
         prm.picard_mode = _lib.PICARD[picard_mode]
@@ -188,12 +193,13 @@ class Ensemble:
         """Launch shape (wspm_ens_shape): operator format and width, the
         tuned number of workgroups walking the entry lists, rows per entry,
         entries, grids timed by the autotuner and the chosen grid's SpMV time
-        over the whole operator (us)."""
+        over the whole operator (us), and whether stalled Jacobi inner steps
+        have handed over to Krylov inner solves."""
         out = (C.c_int32 * 8)()
         check(self.lib.wspm_ens_shape(self.h, out), self.h)
         return {"format": "ell" if out[0] else "csr", "width": int(out[1]), "grid": int(out[2]),
                 "rows_per_entry": int(out[3]), "entries": int(out[4]), "grids_timed": int(out[5]),
-                "tuned_spmv_us": out[6] / 1000.0}
+                "tuned_spmv_us": out[6] / 1000.0, "jacobi_switched_off": bool(out[7])}
 
     def timing(self, on: bool, fp32: bool = False, assembly: bool = False):
         """Time the FP64 SpMV launches (or, fp32=True, the FP32 ones of the

@@ -51,6 +51,8 @@ class Problem:
     restart: int = 30
     precision: str = "fp64"
     inner_tol: float = 1e-6
+    inner: str = "krylov"
+    inner_iters: int = 0
     Ks_elem: Optional[np.ndarray] = None
     m_it: int = 5
     M_it: int = 7
@@ -105,6 +107,7 @@ def from_config(cfg: SimulationConfig, mesh_dir: Optional[str] = None) -> Proble
     sv = cfg.solver
     common = dict(solver=sv.solver_type, precond=sv.precond_type, tol=sv.tol, maxiter=sv.maxiter,
                   restart=sv.restart, precision=cfg.device.precision, inner_tol=cfg.device.inner_tol,
+                  inner=cfg.device.inner, inner_iters=cfg.device.inner_iters,
                   m_it=cfg.time.m_it, M_it=cfg.time.M_it, lambda_amp=cfg.time.lambda_amp,
                   lambda_red=cfg.time.lambda_red, dt0=cfg.time.dt_init, tmax=cfg.time.Tmax,
                   nsteps=cfg.time.nsteps)

@@ -94,7 +94,8 @@ class RichardsSolver:
         self.ctx.richards_init(pb.soil_model, pb.phi, pb.solver, pb.precond, pb.h0, Ks_elem=pb.Ks_elem,
                                tol=pb.tol, atol=pb.atol, maxiter=pb.maxiter, restart=pb.restart,
                                picard_tol=pb.picard_tol, picard_maxit=pb.picard_maxit,
-                               picard_mode=pb.picard_mode, precision=pb.precision, inner_tol=pb.inner_tol)
+                               picard_mode=pb.picard_mode, precision=pb.precision, inner_tol=pb.inner_tol,
+                               inner=pb.inner, inner_iters=pb.inner_iters)
 
     def _keep(self, k: int) -> bool:
         if self.history == "all":

@@ -38,12 +38,37 @@ def test_library_exports_every_declared_symbol():
     assert b"null" in loaded.wspm_last_error(None)
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """Every parameter struct of the binding has the size and field offsets
+    the C compiler gives the header's struct (a C program built with gcc
+    against include/wspm.h prints them)."""
+    import shutil
+    import subprocess
     from jax_wspm_amd import _lib
-    assert ctypes.sizeof(_lib.KrylovStats) == 4 * 4 + 2 * 8
-    assert ctypes.sizeof(_lib.StepStats) == 4 * 4 + 8
-    # 4 int32, phi[6], tol, atol, 2 int32, picard_tol, 2 int32, inner_tol
-    assert ctypes.sizeof(_lib.RichardsParams) == 16 + 48 + 16 + 8 + 8 + 8 + 8
+    pairs = {"wspm_krylov_stats": _lib.KrylovStats, "wspm_step_stats": _lib.StepStats,
+             "wspm_richards_params": _lib.RichardsParams, "wspm_transport_params": _lib.TransportParams,
+             "wspm_adj_params": _lib.AdjParams, "wspm_adj_tr_params": _lib.AdjTrParams}
+    assert ctypes.sizeof(_lib.RichardsParams) == 16 + 48 + 16 + 8 + 8 + 8 + 8 + 8
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "wspm.h"', "int main(void) {"]
+    for cname, cls in pairs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, val = ln.split()
+        got[(cname, field)] = int(val)
+    for cname, cls in pairs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
 
 
 @pytest.mark.parametrize("N,L", [(25, 15.24), (50, 1.0), (7, 2.0)])

@@ -114,8 +114,9 @@ def test_ensemble_rejects_single_instance_calls():
     ens.close()
 
 
-@pytest.mark.parametrize("solver", ["bicgstab", "cg"])
-def test_mixed_ensemble_within_1e8_of_fp64(solver):
+@pytest.mark.parametrize("solver,inner", [("bicgstab", "krylov"), ("cg", "krylov"), ("bicgstab", "jacobi"),
+                                          ("cg", "jacobi")])
+def test_mixed_ensemble_within_1e8_of_fp64(solver, inner):
     """FP32 inner / FP64 outer ensemble: FP32 Krylov corrections of the FP64
     residual, refined per instance until that residual meets the FP64
     tolerance.  Every instance's h and theta stay within 1e-8 relative L2 of
@@ -123,12 +124,12 @@ def test_mixed_ensemble_within_1e8_of_fp64(solver):
     Picard iteration counts."""
     from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
     insts = vg_ensemble(n_inst=6, sides=(8, 12, 16, 20, 24, 32), seed=11)
-    print(f"[case] mixed ensemble 6 {solver}+jacobi", flush=True)
+    print(f"[case] mixed ensemble 6 {solver}+jacobi inner {inner}", flush=True)
     steps, dt = 4, 1e-4
     runs = {}
     for prec in ("fp64", "mixed"):
         ens = Ensemble(insts)
-        ens.richards_init("vg", solver, "jacobi", precision=prec)
+        ens.richards_init("vg", solver, "jacobi", precision=prec, inner=inner)
         runs[prec] = _run(ens, steps, dt)
         ens.close()
     (h64, th64, it64), (h32, th32, it32) = runs["fp64"], runs["mixed"]
@@ -170,7 +171,8 @@ def test_grouped_ensemble_equals_single_context():
 def _ens_run(insts, steps, solver="bicgstab", precision="fp64"):
     from jax_wspm_amd.ensemble import Ensemble
     ens = Ensemble(insts)
-    ens.richards_init("vg", solver, "jacobi", precision=precision)
+    prec, _, inner = precision.partition("-")
+    ens.richards_init("vg", solver, "jacobi", precision=prec, inner=inner or "krylov")
     its = [ens.step(1e-4, per_instance=True)["picard_iters_i"].copy() for _ in range(steps)]
     h = ens.state()[0].cpu().numpy().copy()
     shape = ens.shape
@@ -178,7 +180,7 @@ def _ens_run(insts, steps, solver="bicgstab", precision="fp64"):
     return h, np.array(its), shape
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-jacobi"])
 def test_ensemble_repeatable_across_runs_and_grids(monkeypatch, precision):
     """Fresh contexts of the same instances, each autotuned on its own, and
     contexts with the launch grid forced to 1, 3 and 64 workgroups: bitwise
@@ -198,3 +200,54 @@ def test_ensemble_repeatable_across_runs_and_grids(monkeypatch, precision):
         assert sh2["grid"] == min(int(g), sh2["entries"]) and sh2["grids_timed"] == 0
         assert np.array_equal(it0, it2), g
         assert np.array_equal(h0, h2), g
+
+
+def test_mixed_jacobi_stall_hands_over_to_krylov(monkeypatch):
+    """A Jacobi refinement sweep that does not bring the FP64 residual below
+    the stall ratio times the residual before it hands the context over to
+    Krylov inner solves.  Forced here with a stall ratio of 1e-30, so the
+    first sweep that leaves an instance unconverged stalls: the context
+    reports the hand-over and every instance still matches the FP64 ensemble
+    within 1e-8 with the same Picard counts.  With the default ratio the
+    same instances never stall."""
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    insts = vg_ensemble(n_inst=5, sides=(8, 16, 24, 32), seed=5)
+    print("[case] mixed ensemble, Jacobi inner steps, forced stall", flush=True)
+    steps, dt = 3, 1e-4
+    runs, shapes = {}, {}
+    for key, prec, stall in (("fp64", "fp64", None), ("jac", "mixed", None), ("stall", "mixed", "1e-30")):
+        if stall:
+            monkeypatch.setenv("WSPM_JAC_STALL", stall)
+        else:
+            monkeypatch.delenv("WSPM_JAC_STALL", raising=False)
+        ens = Ensemble(insts)
+        ens.richards_init("vg", "bicgstab", "jacobi", precision=prec, tol=1e-13, inner="jacobi")
+        runs[key] = _run(ens, steps, dt)
+        shapes[key] = ens.shape
+        ens.close()
+    assert not shapes["jac"]["jacobi_switched_off"]
+    assert shapes["stall"]["jacobi_switched_off"]
+    h64, th64, it64 = runs["fp64"]
+    for key in ("jac", "stall"):
+        h, th, it = runs[key]
+        for k in range(steps):
+            assert np.array_equal(it[k], it64[k]), (key, k)
+            for i in range(len(insts)):
+                assert rel_l2(h[k][i], h64[k][i]) <= 1e-8, (key, i, k)
+                assert rel_l2(th[k][i], th64[k][i]) <= 1e-8, (key, i, k)
+
+
+def test_mixed_jacobi_needs_jacobi_preconditioner():
+    """Jacobi inner steps without the Jacobi preconditioner, and out-of-range
+    step counts, are refused at initialisation."""
+    from jax_wspm_amd._lib import WspmError
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    insts = vg_ensemble(n_inst=2, sides=(8, 12), seed=1)
+    ens = Ensemble(insts)
+    with pytest.raises(WspmError, match="Jacobi preconditioner"):
+        ens.richards_init("vg", "bicgstab", None, precision="mixed", inner="jacobi")
+    with pytest.raises(WspmError, match="inner_iters"):
+        ens.richards_init("vg", "bicgstab", "jacobi", precision="mixed", inner="jacobi", inner_iters=17)
+    ens.richards_init("vg", "bicgstab", "jacobi", precision="mixed", inner="jacobi", inner_iters=16)
+    ens.step(1e-4)
+    ens.close()

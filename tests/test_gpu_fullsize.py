@@ -69,15 +69,17 @@ def test_fullsize_ensemble_sample_matches_oracle():
         assert rel_l2(h[i], o["pressure_head"][0]) <= 1e-10, i
 
 
-def test_fullsize_mixed_3d_matches_oracle():
-    """The 64^3 mixed-precision configuration (FP32 inner CG, FP64 Picard
-    residual, update and refinement to 1e-13), one step, against the FP64
-    oracle at the same tolerance: h and theta within 1e-8, the same Picard
-    count."""
+@pytest.mark.parametrize("inner", ["krylov", "jacobi"])
+def test_fullsize_mixed_3d_matches_oracle(inner):
+    """The 64^3 mixed-precision configuration (FP32 inner CG or Jacobi steps,
+    FP64 Picard residual, update and refinement to 1e-13), one step, against
+    the FP64 oracle at the same tolerance: h and theta within 1e-8, the same
+    Picard count."""
     from jax_wspm_amd.problems import baseline_vg_config, from_config
-    print("[case] full size 3D 64 mixed, cg+jacobi, 1 step", flush=True)
+    print(f"[case] full size 3D 64 mixed, cg+jacobi, 1 step, inner {inner}", flush=True)
     pb = from_config(baseline_vg_config(64, 3, solver="cg", precond="jacobi", nsteps=1, dt=1e-4, precision="mixed"))
     pb.tol = 1e-13
+    pb.inner = inner
     g = _run_gpu(pb)
     o = opicard.run(to_oracle(pb))
     assert list(g["iterations"]) == list(o["iterations"])

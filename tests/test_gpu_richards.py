@@ -111,15 +111,18 @@ def test_vg_cube_fixed_steps():
     _compare(g, o)
 
 
-@pytest.mark.parametrize("dim,solver", [(2, "bicgstab"), (3, "cg"), (2, "gmres")])
-def test_mixed_precision_vs_fp64(dim, solver):
-    """FP32 inner Krylov + FP64 refinement/Picard stays within 1e-8 of FP64."""
+@pytest.mark.parametrize("dim,solver,inner", [(2, "bicgstab", "krylov"), (3, "cg", "krylov"), (2, "gmres", "krylov"),
+                                              (2, "bicgstab", "jacobi"), (3, "cg", "jacobi")])
+def test_mixed_precision_vs_fp64(dim, solver, inner):
+    """FP32 inner solves (the method's Krylov solve, or Jacobi steps) + FP64
+    refinement/Picard stays within 1e-8 of FP64."""
     from jax_wspm_amd.problems import baseline_vg_config, from_config
-    print(f"[case] mixed dim={dim} {solver}", flush=True)
+    print(f"[case] mixed dim={dim} {solver} inner {inner}", flush=True)
     cells = 40 if dim == 2 else 10
     pb64 = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", nsteps=5, dt=1e-4))
     pbmx = from_config(baseline_vg_config(cells, dim, solver=solver, precond="jacobi", nsteps=5, dt=1e-4,
                                           precision="mixed"))
+    pbmx.inner = inner
     g64 = _run(pb64)
     gmx = _run(pbmx)
     for k in range(5):
@@ -146,18 +149,21 @@ def test_tracy_general_preconditioners(solver, precond, golden):
     assert f"{e['l2_pressure']:.4e}" == "1.7613e-01" and f"{e['l2_saturation']:.4e}" == "5.0311e-03"
 
 
-@pytest.mark.parametrize("cells,steps,tol", [(16, 5, 1e-10), (16, 5, 1e-13), (24, 3, 1e-13)])
-def test_mixed_3d_matches_oracle(cells, steps, tol):
+@pytest.mark.parametrize("cells,steps,tol,inner", [(16, 5, 1e-10, "krylov"), (16, 5, 1e-13, "krylov"),
+                                                 (24, 3, 1e-13, "krylov"), (16, 5, 1e-13, "jacobi"),
+                                                 (24, 3, 1e-14, "jacobi")])
+def test_mixed_3d_matches_oracle(cells, steps, tol, inner):
     """The 3D configuration with FP32 inner CG (Jacobi) inside FP64 iterative
     refinement, FP64 Picard residual and update (unit cube, Kuhn tets, h0
     -500, top -10, van Genuchten alpha 0.1, n 2), against the FP64 oracle at
     the same Krylov tolerance: the same Picard counts, h and theta within
     1e-8 relative L2 at every step."""
     from jax_wspm_amd.problems import baseline_vg_config, from_config
-    print(f"[case] mixed 3D {cells}^3 CG+jacobi {steps} steps tol {tol:g}", flush=True)
+    print(f"[case] mixed 3D {cells}^3 CG+jacobi {steps} steps tol {tol:g} inner {inner}", flush=True)
     pb = from_config(baseline_vg_config(cells, 3, solver="cg", precond="jacobi", nsteps=steps, dt=1e-4,
                                         precision="mixed"))
     pb.tol = tol
+    pb.inner = inner
     g = _run(pb)
     o = opicard.run(to_oracle(pb))
     assert list(g["iterations"]) == list(o["iterations"])
@@ -185,7 +191,8 @@ def _dry_instances():
     return out
 
 
-def test_mixed_ensemble_dry_instances_within_1e8():
+@pytest.mark.parametrize("inner", ["krylov", "jacobi"])
+def test_mixed_ensemble_dry_instances_within_1e8(inner):
     """Batched mixed precision on the driest, worst-conditioned corners of the
     ensemble's parameter box: refined until the FP64 residual is 1e-14 of
     |b| (where the FP64 solution itself is converged), every instance stays
@@ -194,11 +201,11 @@ def test_mixed_ensemble_dry_instances_within_1e8():
     from jax_wspm_amd.ensemble import Ensemble
     insts = _dry_instances()
     steps, tol = 10, 1e-14
-    print("[case] mixed ensemble, dry corners, tol 1e-14", flush=True)
+    print(f"[case] mixed ensemble, dry corners, tol 1e-14, inner {inner}", flush=True)
     res = {}
     for prec in ("mixed", "fp64"):
         ens = Ensemble(insts)
-        ens.richards_init("vg", "bicgstab", "jacobi", precision=prec, tol=tol, inner_tol=1e-6)
+        ens.richards_init("vg", "bicgstab", "jacobi", precision=prec, tol=tol, inner_tol=1e-6, inner=inner)
         its = [ens.step(1e-4, per_instance=True)["picard_iters_i"].copy() for _ in range(steps)]
         h, th, _ = ens.state()
         res[prec] = (np.array(its), ens.split(h), ens.split(th))
