@@ -238,6 +238,12 @@ int32_t wspm_timing_enable(wspm_ctx* ctx, int32_t on);
  * summed event-to-event duration (ms) and the rows they worked on (rows may
  * be NULL; an ensemble launch skips the rows of converged instances). */
 int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
+/* The same figures over the SpMV launches that worked on at least min_rows
+ * rows, without resetting them (call before wspm_timing_read): what the
+ * kernel reaches when a launch has enough rows to stream, apart from the
+ * fixed cost of small launches. */
+int32_t wspm_timing_read_over(wspm_ctx* ctx, int64_t min_rows, int64_t* launches, double* total_ms,
+                              int64_t* rows);
 /* With on & 4 in wspm_timing_enable, the assembly launches (a Picard
  * iteration's fused assembly + solver start) are bracketed by HIP events
  * too; this reads them as wspm_timing_read does (launches that assembled no

@@ -95,6 +95,8 @@ PROTOTYPES = {
     "wspm_timing_enable": (_I, [_P, _I]),
     "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "wspm_timing_read_asm": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "wspm_timing_read_over": (_I, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int64)]),
     "wspm_edge_set": (_I, [_P, _I, _I, _P]),
     "wspm_richards_edges": (_I, [_P, _I, _D, _I]),
     "wspm_transport_fields": (_I, [_P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),

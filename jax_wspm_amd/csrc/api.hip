@@ -2902,9 +2902,8 @@ int32_t wspm_timing_read_asm(wspm_ctx* ctx, int64_t* launches, double* total_ms,
     API_END
 }
 
-int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows) {
-    if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
-    API_BEGIN
+// Timed SpMV launches with at least min_rows rows: count, summed ms, rows.
+static void timing_sum(wspm_ctx* ctx, int64_t min_rows, int64_t* launches, double* total_ms, int64_t* rows) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const size_t pairs = ctx->ev_used / 2;
     std::vector<int> flags(pairs, 0);
@@ -2912,7 +2911,7 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int
     double tot = 0.0;
     int64_t worked = 0, nrows = 0;
     for (size_t k = 0; k < pairs; ++k) {
-        if (!flags[k]) continue;
+        if (!flags[k] || flags[k] < min_rows) continue;
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[2 * k], ctx->ev_pool[2 * k + 1]));
         tot += ms;
@@ -2922,6 +2921,21 @@ int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int
     *launches = worked;
     *total_ms = tot;
     if (rows) *rows = nrows;
+}
+
+int32_t wspm_timing_read_over(wspm_ctx* ctx, int64_t min_rows, int64_t* launches, double* total_ms,
+                              int64_t* rows) {
+    if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
+    API_BEGIN
+    timing_sum(ctx, min_rows, launches, total_ms, rows);
+    return WSPM_OK;
+    API_END
+}
+
+int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows) {
+    if (!ctx || !launches || !total_ms) return WSPM_EINVAL;
+    API_BEGIN
+    timing_sum(ctx, 1, launches, total_ms, rows);
     ctx->ev_used = 0;
     if (ctx->tflags) HIPCHK(hipMemset(ctx->tflags, 0, TIMING_SLOTS * sizeof(int)));
     return WSPM_OK;

@@ -659,25 +659,30 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
                 const double4 P = (u == 0) ? np : nr[u];
                 const double4 Q = nr[u + 1];
                 const int la = (int)((inf >> (5 + 2 * u)) & 3u);
-                const double J00 = P.x - sx, J01 = P.y - sy;
-                const double J10 = Q.x - sx, J11 = Q.y - sy;
-                const double det = J00 * J11 - J01 * J10;
-                const double id = 1.0 / det;
-                const double G01 = J11 * id, G11 = -J10 * id;
-                const double G02 = -J01 * id, G12 = J00 * id;
-                const double G00 = -(G01 + G02), G10 = -(G11 + G12);
+                // edge vectors from the node: d1 = P - S, d2 = Q - S (the
+                // rows of J), e = Q - P (the edge opposite the node).  With
+                // G = J^-1 grad N, the node's stiffness row kbar det G0.Gc is
+                // c (e.e, -e.d2, e.d1) with c = kbar / det (the three sum to
+                // zero), and its gravity term kbar det dN0/dy = kbar (J10 - J00)
+                // = kbar e.x needs no division
+                const double d1x = P.x - sx, d1y = P.y - sy;
+                const double d2x = Q.x - sx, d2y = Q.y - sy;
+                const double det = d1x * d2y - d1y * d2x;
+                const double ex = d2x - d1x, ey = d2y - d1y;
                 const double ckS = la == 0 ? qc.ck[0] : (la == 1 ? qc.ck[1] : qc.ck[2]);
                 const double ckP = la == 0 ? qc.ck[1] : (la == 1 ? qc.ck[2] : qc.ck[0]);
                 const double ckQ = la == 0 ? qc.ck[2] : (la == 1 ? qc.ck[0] : qc.ck[1]);
                 const double mlS = la == 0 ? qc.ml[0] : (la == 1 ? qc.ml[1] : qc.ml[2]);
                 const double kbar = (ckS * sk + ckP * P.z + ckQ * Q.z) * ks[u];
-                const double kd = kbar * det;
-                v[0] += kd * (G00 * G00 + G10 * G10);
-                const double oP = kd * (G00 * G01 + G10 * G11);
-                if (u == 0) vp += oP;
-                else v[u] += oP;
-                v[u + 1] += kd * (G00 * G02 + G10 * G12);
-                bi += -kd * G10;
+                const double c = kbar / det;
+                const double kss = c * (ex * ex + ey * ey);
+                const double ksq = c * (ex * d1x + ey * d1y);
+                const double ksp = -(kss + ksq);
+                v[0] += kss;
+                if (u == 0) vp += ksp;
+                else v[u] += ksp;
+                v[u + 1] += ksq;
+                bi -= kbar * ex;
                 msum += det * mlS;
             }
         }

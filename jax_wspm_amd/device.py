@@ -314,6 +314,15 @@ class Context:
         self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
         return int(n.value), float(ms.value), int(rows.value)
 
+    def timing_read_over(self, min_rows: int):
+        """timing_read's figures over the launches of at least min_rows rows,
+        without resetting the timing slots (read these first)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read_over(self.h, int(min_rows), C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)
+
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""
         n = C.c_int64()

@@ -215,6 +215,15 @@ class Ensemble:
         self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
         return int(n.value), float(ms.value), int(rows.value)
 
+    def timing_read_over(self, min_rows: int):
+        """timing_read's figures over the launches of at least min_rows rows,
+        without resetting the timing slots (read these first)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        rows = C.c_int64()
+        self._ck(self.lib.wspm_timing_read_over(self.h, int(min_rows), C.byref(n), C.byref(ms), C.byref(rows)))
+        return int(n.value), float(ms.value), int(rows.value)
+
     def timing_read(self):
         """(working SpMV launches, their summed duration in ms, rows they worked on)."""
         n = C.c_int64()
@@ -332,6 +341,10 @@ class EnsembleGroups:
 
     def timing(self, on: bool, fp32: bool = False, assembly: bool = False):
         self._each(lambda m: m.timing(on, fp32, assembly))
+
+    def timing_read_over(self, min_rows: int):
+        res = self._each(lambda m: m.timing_read_over(min_rows))
+        return sum(r[0] for r in res), sum(r[1] for r in res), sum(r[2] for r in res)
 
     def timing_read(self):
         res = self._each(lambda m: m.timing_read())

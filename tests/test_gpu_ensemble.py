@@ -251,3 +251,28 @@ def test_mixed_jacobi_needs_jacobi_preconditioner():
     ens.richards_init("vg", "bicgstab", "jacobi", precision="mixed", inner="jacobi", inner_iters=16)
     ens.step(1e-4)
     ens.close()
+
+
+def test_timing_hooks_count_working_launches():
+    """The benchmark's timing hooks: every timed SpMV launch that worked is
+    counted once with the rows it worked on, the launches over a row
+    threshold are a subset of them (read without resetting), and a read
+    resets the slots."""
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    insts = vg_ensemble(n_inst=4, sides=(8, 16, 24, 32), seed=2)
+    ens = Ensemble(insts)
+    ens.richards_init("vg", "bicgstab", "jacobi")
+    ens.step(1e-4)
+    ens.timing(True, assembly=True)
+    ens.step(1e-4)
+    over_all = ens.timing_read_over(1)
+    over_big = ens.timing_read_over(ens.n // 2)
+    allr = ens.timing_read()
+    asm = ens.timing_read_asm()
+    assert over_all[0] == allr[0] and over_all[2] == allr[2] and abs(over_all[1] - allr[1]) < 1e-9
+    assert allr[0] >= 1 and 0 < allr[2] <= allr[0] * ens.n
+    assert over_big[0] <= allr[0] and over_big[2] >= over_big[0] * (ens.n // 2)
+    assert asm[0] >= 1 and asm[2] <= asm[0] * ens.n
+    assert ens.timing_read() == (0, 0.0, 0)
+    ens.timing(False)
+    ens.close()
