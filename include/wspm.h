@@ -157,7 +157,8 @@ int32_t wspm_set_stream(wspm_ctx* ctx, void* stream);
  * CSR SpMV's lanes per row, and 1 if the time-step solves of the last
  * wspm_richards_init / wspm_ens_richards_init read the operator in ELL order
  * with each row's columns sorted, 2 in ELL order with each row's columns in
- * fan order (the node, then its neighbours around it), 0 in CSR order. */
+ * fan order (the node, then its neighbours around it), 3 in fan order with
+ * the assembly staging each row block's node window in LDS, 0 in CSR order. */
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info);
 /* Copy the CSR pattern to host: row_ptr int32[n+1], cols int32[nnz]. */
 int32_t wspm_get_pattern(const wspm_ctx* ctx, int32_t* row_ptr, int32_t* cols);

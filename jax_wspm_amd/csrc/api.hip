@@ -737,6 +737,12 @@ struct wspm_ctx {
     bool fan = false;
     int32_t *fan_inc = nullptr, *fan_pred = nullptr;
     uint32_t* fan_info = nullptr;
+    // node windows of the windowed fan assembly (build_node_windows)
+    int32_t* nwin = nullptr;
+    uint16_t *nwin_col = nullptr, *nwin_pred = nullptr;
+    int nwin_cap = 0;
+    bool nwin_ok = false;
+    std::vector<RowTab> htab;       // ensembles: the row table on the host
     int fan_iw = 0;
     // finish-free CG / BiCGSTAB: second partial-row region and the iteration
     // histories of the main (0) and FP32 inner (1) solves
@@ -1775,7 +1781,7 @@ int32_t wspm_set_stream(wspm_ctx* ctx, void* stream) {
 int32_t wspm_ctx_info(const wspm_ctx* ctx, int64_t* info) {
     if (!ctx || !info) return WSPM_EINVAL;
     info[0] = ctx->n; info[1] = ctx->E; info[2] = ctx->nnz; info[3] = ctx->dim;
-    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->ell ? (ctx->fan ? 2 : 1) : 0;
+    info[4] = ctx->max_row; info[5] = ctx->n_bc; info[6] = ctx->lanes; info[7] = ctx->ell ? (ctx->fan ? (ctx->nwin_ok ? 3 : 2) : 1) : 0;
     return WSPM_OK;
 }
 
@@ -1893,6 +1899,13 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
         a.fan_info = ctx->fan_info;
         a.fan_iw = ctx->fan_iw;
         kernel = ASM_K_FAN;
+        if (ctx->nwin_ok) {
+            a.nwin = ctx->nwin;
+            a.nwin_col = ctx->nwin_col;
+            a.nwin_pred = ctx->nwin_pred;
+            a.nwin_cap = ctx->nwin_cap;
+            kernel = ASM_K_WIN;
+        }
     } else if (ell && ctx->ell_info && asm_is_nbr(ctx->asm_kernel)) {
         a.ell_col = ctx->ell_col;
         a.ell_inc = ctx->ell_inc;
@@ -2438,6 +2451,86 @@ static void ell_offsets(wspm_ctx* ctx, const std::vector<int32_t>& c) {
     ctx->ell_col16 = dupload(ctx, o.data(), o.size());
 }
 
+// Node windows of the windowed fan assembly.  Row blocks are BLOCK rows
+// from 0 on one mesh and the row-table entries on an ensemble.  A block's
+// window is the sorted set of nodes its rows touch (fan slots and
+// predecessors), cut at its WIN_SEG - 1 widest gaps into segments of
+// consecutive nodes; every position of a segment is staged, so a structured
+// mesh's block takes its own rows and the grid rows either side (one
+// segment when they touch, three on wide grids).  nwin_ok stays false if a
+// window would exceed WIN_CAP nodes (the fan kernel then gathers from
+// global memory); WSPM_ASM_WIN=0 keeps that kernel.
+static void build_node_windows(wspm_ctx* ctx, const std::vector<int32_t>& fc, const std::vector<int32_t>& fp) {
+    ctx->nwin_ok = false;
+    if (env_is("WSPM_ASM_WIN", "0")) return;
+    const int n = ctx->n, W = ctx->max_row;
+    std::vector<std::pair<int, int>> blocks;
+    if (ctx->tab) {
+        for (const RowTab& t : ctx->htab) blocks.emplace_back(t.x, t.y);
+    } else {
+        for (int r = 0; r < n; r += BLOCK) blocks.emplace_back(r, std::min(n, r + BLOCK));
+    }
+    std::vector<int32_t> win(WIN_WORDS * blocks.size(), 0);
+    std::vector<uint16_t> wcol((size_t)W * n), wpred(n);
+    std::vector<int> cols;
+    int cap = 0;
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        const int r0 = blocks[b].first, r1 = blocks[b].second;
+        cols.clear();
+        for (int i = r0; i < r1; ++i) {
+            for (int s = 0; s < W; ++s) cols.push_back(fc[(size_t)s * n + i]);
+            cols.push_back(fp[i]);
+        }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        // cut at the widest gaps (earliest first among equals)
+        std::vector<int> cut;   // index k: a segment ends after cols[k]
+        for (int s = 0; s < WIN_SEG - 1; ++s) {
+            int best = -1, gap = 1;
+            for (size_t k = 0; k + 1 < cols.size(); ++k) {
+                const int g = cols[k + 1] - cols[k];
+                if (g > gap && std::find(cut.begin(), cut.end(), (int)k) == cut.end()) {
+                    gap = g;
+                    best = (int)k;
+                }
+            }
+            if (best >= 0) cut.push_back(best);
+        }
+        std::sort(cut.begin(), cut.end());
+        int lo = cols.front(), off = 0, seg = 0;
+        int segs[WIN_SEG][3];   // lo, len, offset
+        for (size_t q = 0; q <= cut.size(); ++q) {
+            const int hi = q < cut.size() ? cols[cut[q]] : cols.back();
+            segs[seg][0] = lo;
+            segs[seg][1] = hi - lo + 1;
+            segs[seg][2] = off;
+            off += hi - lo + 1;
+            ++seg;
+            if (q < cut.size()) lo = cols[cut[q] + 1];
+        }
+        if (off > WIN_CAP) return;
+        cap = std::max(cap, off);
+        for (int k = 0; k < seg; ++k) {
+            win[WIN_WORDS * b + 2 * k] = segs[k][0];
+            win[WIN_WORDS * b + 2 * k + 1] = segs[k][1];
+        }
+        auto pos = [&](int c) {
+            for (int k = 0; k < seg; ++k)
+                if (c >= segs[k][0] && c < segs[k][0] + segs[k][1]) return segs[k][2] + (c - segs[k][0]);
+            return -1;
+        };
+        for (int i = r0; i < r1; ++i) {
+            for (int s = 0; s < W; ++s) wcol[(size_t)s * n + i] = (uint16_t)pos(fc[(size_t)s * n + i]);
+            wpred[i] = (uint16_t)pos(fp[i]);
+        }
+    }
+    ctx->nwin = dupload(ctx, win.data(), win.size());
+    ctx->nwin_col = dupload(ctx, wcol.data(), wcol.size());
+    ctx->nwin_pred = dupload(ctx, wpred.data(), wpred.size());
+    ctx->nwin_cap = cap;
+    ctx->nwin_ok = true;
+}
+
 // Column table of the ELL layout: entry k of row i at k * n + i; padding
 // points at the row itself (its value stays 0).  With the neighbourhood
 // assembly, the rows are put in fan order when every row of the mesh is one
@@ -2461,6 +2554,7 @@ static void ensure_ell(wspm_ctx* ctx) {
             ctx->fan_info = dupload(ctx, finfo.data(), finfo.size());
             ctx->fan_iw = iw;
             ctx->fan = true;
+            build_node_windows(ctx, fc, fp);
             return;
         }
     }
@@ -3438,6 +3532,7 @@ int32_t wspm_ens_create(int32_t dim, int32_t n_inst, const int32_t* node_counts,
         ctx->n_inst = n_inst;
         ctx->n_tab = (int)tab.size();
         ctx->tab = dupload(ctx, tab.data(), tab.size());
+        ctx->htab = tab;
         ctx->iblk = dupload(ctx, iblk.data(), iblk.size());
         if (ctx->n_tab > ctx->prow) {
             ctx->prow = ctx->n_tab;

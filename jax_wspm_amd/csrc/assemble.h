@@ -97,6 +97,16 @@ struct AsmArgs {
     const int32_t* fan_pred;
     const uint32_t* fan_info;
     int fan_iw;
+    // Windowed fan kernel (ASM_K_WIN): per row block (BLOCK rows from 0, or
+    // an ensemble entry) up to WIN_SEG segments of consecutive nodes, at
+    // nwin[WIN_WORDS * block] = (lo, len) pairs, that cover every column of
+    // its rows; staged in LDS (coordinates, K and x0) at the segments' running
+    // offsets, nwin_cap nodes at most.  nwin_col[s * n + i]: the LDS position
+    // of fan slot s of row i (nwin_pred: of its predecessor).
+    const int32_t* nwin;
+    const uint16_t* nwin_col;
+    const uint16_t* nwin_pred;
+    int nwin_cap;
     QuadConst qc;
     // Ensemble mode (register kernel only): a grid of ens_grid workgroups
     // walks the row-table entries listed in act (act[0] = count; null: all
@@ -132,7 +142,13 @@ int asm_reg_max_row(int dim);
 // registers (WSPM_ASM_KERNEL=blend).
 // ASM_K_FAN: the time-step assembly into an ELL operator in fan order
 // (every row a fan of consistently oriented triangles; chosen per launch).
-enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3, ASM_K_FAN = 4 };
+// ASM_K_WIN: the same with each row block's node records staged in LDS.
+enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3, ASM_K_FAN = 4, ASM_K_WIN = 5 };
+// Node windows of the windowed fan kernel: segments per block, ints per
+// block in AsmArgs::win, and the most nodes a window holds.
+constexpr int WIN_SEG = 3;
+constexpr int WIN_WORDS = 2 * WIN_SEG;
+constexpr int WIN_CAP = 1024;
 inline bool asm_is_nbr(int k) { return k == ASM_K_NBR || k == ASM_K_NBR_BLEND; }
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, int kernel);

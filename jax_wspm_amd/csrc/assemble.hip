@@ -590,6 +590,17 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
+// The fan kernels' quadrature weights by the node's local vertex la,
+// (ck[la], ck[la + 1], ck[la + 2], ml[la]) at qw[4 * la], into LDS; read
+// after a workgroup barrier.  (A lane-varying select of the kernel
+// argument's weights compiles into loads from the argument segment.)
+__device__ inline void fan_weights(const QuadConst& qc, double* qw) {
+    if (threadIdx.x < 12) {
+        const int la = threadIdx.x / 4, c = threadIdx.x % 4;
+        qw[threadIdx.x] = c == 3 ? qc.ml[la] : qc.ck[(la + c) % 3];
+    }
+}
+
 // Fan variant (2D, the time-step operator in fan order).  The row's slots
 // are the node itself (slot 0) and its neighbours in their cyclic order
 // around it, so fan triangle u is (node, slot u, slot u + 1), with the
@@ -603,7 +614,8 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
 // la + 2.  Sums run over the triangles in fan order; a closed fan's
 // triangle 0 contribution to the last neighbour is added after the others.
 template <int MODE>
-__device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
+__device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, int i, const double* qw,
+                                        double (&acc)[3]) {
     constexpr int W = 8;          // the node + up to 7 neighbours
     constexpr int T = W - 1;      // fan triangles
     const uint32_t inf = a.fan_info[i];
@@ -651,7 +663,6 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
         bi = 0.0;
         double msum = 0.0, vp = 0.0;
         const double sx = nr[0].x, sy = nr[0].y, sk = nr[0].z;
-        const QuadConst& qc = a.qc;
 #pragma unroll
         for (int u = 0; u < T; ++u) {
             const bool ex = closed ? (u < F) : (u >= 1 && u < F);
@@ -659,6 +670,7 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
                 const double4 P = (u == 0) ? np : nr[u];
                 const double4 Q = nr[u + 1];
                 const int la = (int)((inf >> (5 + 2 * u)) & 3u);
+                const double4 w = *reinterpret_cast<const double4*>(qw + 4 * la);
                 // edge vectors from the node: d1 = P - S, d2 = Q - S (the
                 // rows of J), e = Q - P (the edge opposite the node).  With
                 // G = J^-1 grad N, the node's stiffness row kbar det G0.Gc is
@@ -669,10 +681,7 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
                 const double d2x = Q.x - sx, d2y = Q.y - sy;
                 const double det = d1x * d2y - d1y * d2x;
                 const double ex = d2x - d1x, ey = d2y - d1y;
-                const double ckS = la == 0 ? qc.ck[0] : (la == 1 ? qc.ck[1] : qc.ck[2]);
-                const double ckP = la == 0 ? qc.ck[1] : (la == 1 ? qc.ck[2] : qc.ck[0]);
-                const double ckQ = la == 0 ? qc.ck[2] : (la == 1 ? qc.ck[0] : qc.ck[1]);
-                const double mlS = la == 0 ? qc.ml[0] : (la == 1 ? qc.ml[1] : qc.ml[2]);
+                const double ckS = w.x, ckP = w.y, ckQ = w.z, mlS = w.w;
                 const double kbar = (ckS * sk + ckP * P.z + ckQ * Q.z) * ks[u];
                 const double c = kbar / det;
                 const double kss = c * (ex * ex + ey * ey);
@@ -716,16 +725,208 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
+// Windowed fan variant.  The workgroup first stages its row block's node
+// window (coordinates, K and the Krylov start's x0 of every node its rows
+// touch, a few consecutive segments) in LDS with coalesced loads, then each
+// row reads the records of its fan slots from LDS at the positions
+// nwin_col gives.  One global round trip per row (its own data and the
+// window, issued together) instead of two, and the records are read per
+// triangle instead of held in registers for all slots at once.  The
+// arithmetic and its order are those of assemble_row_fan, so the values are
+// bitwise the same.
+// qw: the quadrature weights by local vertex (fan_weights).
+template <int MODE>
+__device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, int i, const double* lx,
+                                        const double* ly, const double* lk, const double* lx0, const double* qw,
+                                        double (&acc)[3]) {
+    constexpr int W = 8;          // the node + up to 7 neighbours
+    constexpr int T = W - 1;      // fan triangles
+    const uint32_t inf = a.fan_info[i];
+    const int F = (int)(inf & 15u);
+    const bool closed = (inf & 16u) != 0;
+    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
+    int lc[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) lc[s] = (int)a.nwin_col[(size_t)(s < a.ell_w ? s : 0) * a.n + i];
+    const int lp = (int)a.nwin_pred[i];
+    double ks[T];
+    if (a.Ks_elem != nullptr) {   // uniform
+        int code[T];
+#pragma unroll
+        for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
+#pragma unroll
+        for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[code[u] >> 2];
+    } else {
+#pragma unroll
+        for (int u = 0; u < T; ++u) ks[u] = 1.0;
+    }
+    double v[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) v[s] = 0.0;
+    double bi;
+    if (dir) {
+        v[0] = 1.0;
+        bi = a.bc_val[i];
+    } else {
+        bi = 0.0;
+        double msum = 0.0, vp = 0.0;
+        const double sx = lx[lc[0]], sy = ly[lc[0]], sk = lk[lc[0]];
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            const bool ex = closed ? (u < F) : (u >= 1 && u < F);
+            if (ex) {
+                const int pp = (u == 0) ? lp : lc[u];
+                const int qq = lc[u + 1];
+                const int la = (int)((inf >> (5 + 2 * u)) & 3u);
+                const double Px = lx[pp], Py = ly[pp], Pk = lk[pp];
+                const double Qx = lx[qq], Qy = ly[qq], Qk = lk[qq];
+                const double d1x = Px - sx, d1y = Py - sy;
+                const double d2x = Qx - sx, d2y = Qy - sy;
+                const double det = d1x * d2y - d1y * d2x;
+                const double ex_ = d2x - d1x, ey = d2y - d1y;
+                const double4 w = *reinterpret_cast<const double4*>(qw + 4 * la);
+                const double ckS = w.x, ckP = w.y, ckQ = w.z, mlS = w.w;
+                const double kbar = (ckS * sk + ckP * Pk + ckQ * Qk) * ks[u];
+                const double c = kbar / det;
+                const double kss = c * (ex_ * ex_ + ey * ey);
+                const double ksq = c * (ex_ * d1x + ey * d1y);
+                const double ksp = -(kss + ksq);
+                v[0] += kss;
+                if (u == 0) vp += ksp;
+                else v[u] += ksp;
+                v[u + 1] += ksq;
+                bi -= kbar * ex_;
+                msum += det * mlS;
+            }
+        }
+        // the closed fan's triangle 0 joins the last neighbour (slot F)
+#pragma unroll
+        for (int s = 1; s < W; ++s) v[s] = (closed && s == F) ? v[s] + vp : v[s];
+        const double ci = a.C[i];
+        const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+        v[0] += a.inv_dt * msum * ci;
+        bi -= a.inv_dt * msum * rhs;
+    }
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        if (s <= F && s < a.ell_w) {
+            const size_t o = (size_t)s * a.ell_n + i;
+            a.A[o] = v[s];
+            if (a.A32 != nullptr) a.A32[o] = (float)v[s];
+        }
+    }
+    a.b[i] = bi;
+    const double dgv = v[0];
+    const double di = 1.0 / (dgv != 0.0 ? dgv : 1.0);
+    if (a.dinv != nullptr) a.dinv[i] = di;
+    if (a.dinv32 != nullptr) a.dinv32[i] = 1.0f / (dgv != 0.0 ? (float)dgv : 1.0f);
+    if constexpr (MODE != ASM_INIT_NONE) {
+        double ax = 0.0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (s <= F) ax += v[s] * lx0[lc[s]];
+        krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
+    }
+}
+
+// The node window of row block blk into LDS (x, y, K, x0 planes of
+// nwin_cap doubles each), then a workgroup barrier.
+template <int MODE>
+__device__ inline void stage_window(const AsmArgs& a, const KrylovInit& ki, int blk, double* lx, double* ly,
+                                    double* lk, double* lx0) {
+    const double2* xy = reinterpret_cast<const double2*>(a.nrec.coord);
+    const int32_t* w = a.nwin + (size_t)WIN_WORDS * blk;
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < WIN_SEG; ++k) {
+        const int lo = w[2 * k], len = w[2 * k + 1];
+        for (int j = threadIdx.x; j < len; j += BLOCK) {
+            const double2 p = xy[lo + j];
+            lx[off + j] = p.x;
+            ly[off + j] = p.y;
+            lk[off + j] = a.nrec.K[lo + j];
+            if constexpr (MODE != ASM_INIT_NONE) lx0[off + j] = ki.x0[lo + j];
+        }
+        off += len;
+    }
+    __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+    extern __shared__ double wl[];
+    double* lx = wl;
+    double* ly = wl + a.nwin_cap;
+    double* lk = wl + 2 * a.nwin_cap;
+    double* lx0 = wl + 3 * a.nwin_cap;
+    // quadrature weights by local vertex (read after the window's barrier)
+    __shared__ __align__(16) double qw[12];
+    fan_weights(a.qc, qw);
+    if (a.tab == nullptr) {
+        if (a.gate != nullptr && *a.gate == 0) return;   // uniform: a Picard iteration enqueued ahead, not needed
+        if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
+        // rows of logical block L, as in k_assemble_reg; its window is block L's
+        const int L = xcd_block(blockIdx.x, gridDim.x);
+        stage_window<MODE>(a, ki, L, lx, ly, lk, lx0);
+        const int i = L * BLOCK + threadIdx.x;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (i < a.n) assemble_row_win<MODE>(a, ki, i, lx, ly, lk, lx0, qw, acc);
+        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, L);
+        return;
+    }
+    // ensemble: the walk of k_assemble_reg, the window of each entry
+    int v0, v1, vstep;
+    xcd_walk(a.act ? a.act[0] : a.n_tab, v0, v1, vstep);
+    for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
+        const int t = a.act ? a.act[1 + v] : v;
+        const int4 tb = a.tab[t];
+        if (a.ens_st[tb.z].pdone) continue;
+        if (a.tflag != nullptr && threadIdx.x == 0) atomicAdd(a.tflag, tb.y - tb.x);
+        stage_window<MODE>(a, ki, t, lx, ly, lk, lx0);
+        const int i = tb.x + threadIdx.x;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (i < tb.y) assemble_row_win<MODE>(a, ki, i, lx, ly, lk, lx0, qw, acc);
+        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
+        __syncthreads();   // the next entry's window replaces this one
+    }
+}
+
+template <int MODE>
+static void launch_win_mode(dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki, double* partials,
+                            int prow) {
+    const size_t lds = 4 * (size_t)a.nwin_cap * sizeof(double);
+    hipLaunchKernelGGL((k_assemble_win<MODE>), g, dim3(BLOCK), lds, s, a, ki, partials, prow);
+}
+
+static hipError_t launch_win(int mode, dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki,
+                             double* partials, int prow) {
+    switch (mode) {
+        case ASM_INIT_BICG: launch_win_mode<ASM_INIT_BICG>(g, s, a, ki, partials, prow); break;
+        case ASM_INIT_CG: launch_win_mode<ASM_INIT_CG>(g, s, a, ki, partials, prow); break;
+        case ASM_INIT_GMRES: launch_win_mode<ASM_INIT_GMRES>(g, s, a, ki, partials, prow); break;
+        case ASM_INIT_RESID: launch_win_mode<ASM_INIT_RESID>(g, s, a, ki, partials, prow); break;
+        default: launch_win_mode<ASM_INIT_NONE>(g, s, a, ki, partials, prow); break;
+    }
+    return hipGetLastError();
+}
+
 // NB: 0 incidence gathers; 1, 2 neighbourhood gathers with LDS = NB - 1; 3 fan order
+// (qw: the fan kernel's quadrature weights by local vertex, fan_weights)
 template <int D, int MAXR, int MODE, int NB>
-__device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, int i, double (&acc)[3]) {
-    if constexpr (NB == 3 && D == 2) assemble_row_fan<MODE>(a, ki, i, acc);
+__device__ inline void assemble_row_any(const AsmArgs& a, const KrylovInit& ki, int i, const double* qw,
+                                        double (&acc)[3]) {
+    if constexpr (NB == 3 && D == 2) assemble_row_fan<MODE>(a, ki, i, qw, acc);
     else if constexpr (NB > 0 && D == 2) assemble_row_nbr<MAXR, MODE, NB - 1>(a, ki, i, acc);
     else assemble_row<D, MAXR, MODE>(a, ki, i, acc);
 }
 
 template <int D, int MAXR, int MODE, int NBR>
 __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki, double* partials, int prow) {
+    __shared__ __align__(16) double qw[NBR == 3 ? 12 : 1];
+    if constexpr (NBR == 3) {
+        fan_weights(a.qc, qw);
+        __syncthreads();
+    }
     if (a.tab == nullptr) {
         if (a.gate != nullptr && *a.gate == 0) return;   // uniform: a Picard iteration enqueued ahead, not needed
         if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
@@ -735,7 +936,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
         const int L = xcd_block(blockIdx.x, gridDim.x);
         const int i = L * BLOCK + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
-        if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
+        if (i < a.n) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, qw, acc);
         if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, L);
         return;
     }
@@ -751,7 +952,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_reg(AsmArgs a, KrylovInit ki
         if (a.tflag != nullptr && threadIdx.x == 0) atomicAdd(a.tflag, tb.y - tb.x);
         const int i = tb.x + threadIdx.x;
         double acc[3] = {0.0, 0.0, 0.0};
-        if (i < tb.y) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, acc);
+        if (i < tb.y) assemble_row_any<D, MAXR, MODE, NBR>(a, ki, i, qw, acc);
         if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
     }
 }
@@ -836,6 +1037,7 @@ hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit
     if (kernel != ASM_K_LDS) {
         const int grid = a.tab ? a.ens_grid : (a.n + BLOCK - 1) / BLOCK;
         if (grid_out) *grid_out = grid;
+        if (kernel == ASM_K_WIN) return launch_win(mode, dim3(grid), s, a, ki, partials, prow);
         if (kernel == ASM_K_FAN) return launch_reg<2, ASM_REG_ROW_2D, 3>(mode, dim3(grid), s, a, ki, partials, prow);
         if (kernel == ASM_K_NBR) return launch_reg<2, ASM_REG_ROW_2D, 2>(mode, dim3(grid), s, a, ki, partials, prow);
         if (kernel == ASM_K_NBR_BLEND) return launch_reg<2, ASM_REG_ROW_2D, 1>(mode, dim3(grid), s, a, ki, partials, prow);

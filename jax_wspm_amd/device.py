@@ -290,7 +290,15 @@ class Context:
         node, then its neighbours around it) or "sorted"."""
         info = (C.c_int64 * 8)()
         check(self.lib.wspm_ctx_info(self.h, info), self.h)
-        return "fan" if info[7] == 2 else "sorted"
+        return "fan" if info[7] >= 2 else "sorted"
+
+    @property
+    def assembly_windows(self) -> bool:
+        """True if the time-step assembly stages each row block's node window
+        in LDS (fan order, every window within the kernel's capacity)."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        return info[7] == 3
 
     @property
     def spmv_format(self) -> str:

@@ -181,6 +181,13 @@ class Ensemble:
         return [x[self.node_off[i]:self.node_off[i + 1]] for i in range(self.n_inst)]
 
     @property
+    def assembly_windows(self) -> bool:
+        """True if the time-step assembly stages each entry's node window in LDS."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.wspm_ctx_info(self.h, info), self.h)
+        return info[7] == 3
+
+    @property
     def spmv_format(self) -> str:
         """Operator layout of the time-step solves since the last Richards
         init: "ell" or "csr"."""

@@ -222,3 +222,53 @@ def test_mixed_ensemble_dry_instances_within_1e8(inner):
                                         maxiter=100))
         assert list(o["iterations"]) == [int(x) for x in im[:, i]], i
         assert rel_l2(hm[i], o["pressure_head"][-1]) <= 1e-8, i
+
+
+def _trajectory(pb, steps):
+    from jax_wspm_amd.richards import RichardsSolver
+    s = RichardsSolver(pb, history="last")
+    s.init_state()
+    its = [s.ctx.richards_step(pb.dt0)["picard_iters"] for _ in range(steps)]
+    h, th, _ = s.ctx.richards_state()
+    windows = s.ctx.assembly_windows
+    s.ctx.close()
+    return its, h.cpu().numpy(), th.cpu().numpy(), windows
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_windowed_fan_assembly_bitwise(monkeypatch, precision):
+    """The fan assembly that stages each row block's node window in LDS,
+    against the fan assembly that gathers the records from global memory
+    (WSPM_ASM_WIN=0): bitwise the same trajectories and Picard counts, on a
+    300 x 300 grid with per-element Ks (a row block then touches three
+    separate stretches of nodes: its own rows and the grid rows either side)
+    and on an ensemble of uneven instances."""
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    from jax_wspm_amd.problems import baseline_vg_config, from_config
+    print(f"[case] windowed fan assembly bitwise, {precision}", flush=True)
+    steps = 3
+    res = {}
+    for win in ("1", "0"):
+        monkeypatch.setenv("WSPM_ASM_WIN", win)
+        pb = from_config(baseline_vg_config(300, 2, solver="bicgstab", precond="jacobi", nsteps=steps, dt=1e-4,
+                                            precision=precision,
+                                            ks_field={"sigma": 0.5, "length": 0.05, "seed": 3}))
+        if precision == "mixed":
+            pb.tol = 1e-13
+            pb.inner = "jacobi"
+        res[win] = _trajectory(pb, steps)
+    assert res["1"][3] and not res["0"][3]
+    assert res["1"][0] == res["0"][0]
+    assert np.array_equal(res["1"][1], res["0"][1]) and np.array_equal(res["1"][2], res["0"][2])
+    insts = vg_ensemble(n_inst=5, sides=(16, 40, 64, 100, 300), seed=6)
+    ens_res = {}
+    for win in ("1", "0"):
+        monkeypatch.setenv("WSPM_ASM_WIN", win)
+        ens = Ensemble(insts)
+        ens.richards_init("vg", "bicgstab", "jacobi", precision=precision, tol=1e-13, inner="jacobi")
+        its = [ens.step(1e-4, per_instance=True)["picard_iters_i"].copy() for _ in range(steps)]
+        ens_res[win] = (np.array(its), ens.state()[0].cpu().numpy(), ens.assembly_windows)
+        ens.close()
+    assert ens_res["1"][2] and not ens_res["0"][2]
+    assert np.array_equal(ens_res["1"][0], ens_res["0"][0])
+    assert np.array_equal(ens_res["1"][1], ens_res["0"][1])
