@@ -101,7 +101,7 @@ struct AsmArgs {
     // an ensemble entry) up to WIN_SEG segments of consecutive nodes, at
     // nwin[WIN_WORDS * block] = (lo, len) pairs, that cover every column of
     // its rows; staged in LDS (coordinates, K and x0) at the segments' running
-    // offsets, nwin_cap nodes at most.  nwin_col[s * n + i]: the LDS position
+    // offsets, nwin_cap <= WIN_CAP nodes at most.  nwin_col[s * n + i]: the LDS position
     // of fan slot s of row i (nwin_pred: of its predecessor).
     const int32_t* nwin;
     const uint16_t* nwin_col;
@@ -145,10 +145,12 @@ int asm_reg_max_row(int dim);
 // ASM_K_WIN: the same with each row block's node records staged in LDS.
 enum { ASM_K_LDS = 0, ASM_K_REG = 1, ASM_K_NBR = 2, ASM_K_NBR_BLEND = 3, ASM_K_FAN = 4, ASM_K_WIN = 5 };
 // Node windows of the windowed fan kernel: segments per block, ints per
-// block in AsmArgs::win, and the most nodes a window holds.
+// block in AsmArgs::nwin, and the most nodes a window holds (4 planes of
+// doubles: 30 KiB, so that 5 workgroups share a CU's 160 KiB of LDS, the
+// occupancy its registers allow).
 constexpr int WIN_SEG = 3;
 constexpr int WIN_WORDS = 2 * WIN_SEG;
-constexpr int WIN_CAP = 1024;
+constexpr int WIN_CAP = 960;
 inline bool asm_is_nbr(int k) { return k == ASM_K_NBR || k == ASM_K_NBR_BLEND; }
 hipError_t launch_assemble(int dim, const AsmArgs& a, int mode, const KrylovInit& ki, double* partials,
                            int prow, size_t lds, int* grid_out, hipStream_t s, int kernel);

@@ -590,6 +590,18 @@ __device__ inline void assemble_row_nbr(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
+// 1 / d for an element determinant (finite, normal, non-zero): the hardware
+// reciprocal refined by two Newton steps, within an ulp or so of the IEEE
+// quotient in 5 instructions instead of the 10 of the full division (the
+// fan kernels are bound by their instruction issue at ensemble scale).
+__device__ inline double recip_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // The fan kernels' quadrature weights by the node's local vertex la,
 // (ck[la], ck[la + 1], ck[la + 2], ml[la]) at qw[4 * la], into LDS; read
 // after a workgroup barrier.  (A lane-varying select of the kernel
@@ -683,7 +695,7 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
                 const double ex = d2x - d1x, ey = d2y - d1y;
                 const double ckS = w.x, ckP = w.y, ckQ = w.z, mlS = w.w;
                 const double kbar = (ckS * sk + ckP * P.z + ckQ * Q.z) * ks[u];
-                const double c = kbar / det;
+                const double c = kbar * recip_nr(det);
                 const double kss = c * (ex * ex + ey * ey);
                 const double ksq = c * (ex * d1x + ey * d1y);
                 const double ksp = -(kss + ksq);
@@ -787,7 +799,7 @@ __device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, 
                 const double4 w = *reinterpret_cast<const double4*>(qw + 4 * la);
                 const double ckS = w.x, ckP = w.y, ckQ = w.z, mlS = w.w;
                 const double kbar = (ckS * sk + ckP * Pk + ckQ * Qk) * ks[u];
-                const double c = kbar / det;
+                const double c = kbar * recip_nr(det);
                 const double kss = c * (ex_ * ex_ + ey * ey);
                 const double ksq = c * (ex_ * d1x + ey * d1y);
                 const double ksp = -(kss + ksq);
@@ -854,11 +866,13 @@ __device__ inline void stage_window(const AsmArgs& a, const KrylovInit& ki, int 
 
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki, double* partials, int prow) {
-    extern __shared__ double wl[];
+    // the window's planes at fixed offsets (one address per node, immediate
+    // offsets for the planes)
+    __shared__ double wl[4 * WIN_CAP];
     double* lx = wl;
-    double* ly = wl + a.nwin_cap;
-    double* lk = wl + 2 * a.nwin_cap;
-    double* lx0 = wl + 3 * a.nwin_cap;
+    double* ly = wl + WIN_CAP;
+    double* lk = wl + 2 * WIN_CAP;
+    double* lx0 = wl + 3 * WIN_CAP;
     // quadrature weights by local vertex (read after the window's barrier)
     __shared__ __align__(16) double qw[12];
     fan_weights(a.qc, qw);
@@ -894,8 +908,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki
 template <int MODE>
 static void launch_win_mode(dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki, double* partials,
                             int prow) {
-    const size_t lds = 4 * (size_t)a.nwin_cap * sizeof(double);
-    hipLaunchKernelGGL((k_assemble_win<MODE>), g, dim3(BLOCK), lds, s, a, ki, partials, prow);
+    hipLaunchKernelGGL((k_assemble_win<MODE>), g, dim3(BLOCK), 0, s, a, ki, partials, prow);
 }
 
 static hipError_t launch_win(int mode, dim3 g, hipStream_t s, const AsmArgs& a, const KrylovInit& ki,
