@@ -655,11 +655,13 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
     // only then: the node's local vertex of each triangle is in the info word)
     double ks[T];
     if (a.Ks_elem != nullptr) {   // uniform
+        // every code load issued (past fan_iw: plane 0's, unused), so that
+        // none waits behind a branch
         int code[T];
 #pragma unroll
-        for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
+        for (int u = 0; u < T; ++u) code[u] = a.fan_inc[(size_t)(u < a.fan_iw ? u : 0) * a.n + i];
 #pragma unroll
-        for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[code[u] >> 2];
+        for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[(u < a.fan_iw ? code[u] : 0) >> 2];
     } else {
 #pragma unroll
         for (int u = 0; u < T; ++u) ks[u] = 1.0;
@@ -737,58 +739,86 @@ __device__ inline void assemble_row_fan(const AsmArgs& a, const KrylovInit& ki, 
     }
 }
 
-// Windowed fan variant.  The workgroup first stages its row block's node
-// window (coordinates, K and the Krylov start's x0 of every node its rows
-// touch, a few consecutive segments) in LDS with coalesced loads, then each
-// row reads the records of its fan slots from LDS at the positions
-// nwin_col gives.  One global round trip per row (its own data and the
-// window, issued together) instead of two, and the records are read per
-// triangle instead of held in registers for all slots at once.  The
+// Windowed fan variant.  The workgroup stages its row block's node window
+// (coordinates, K and the Krylov start's x0 of every node its rows touch, a
+// few consecutive segments) in LDS with coalesced loads, then each row reads
+// the records of its fan slots from LDS at the positions nwin_col gives.
+// Every global load of a row block is issued before the window's barrier
+// and waited for once: the row's own inputs (WinRow), the window, and, with
+// per-element Ks, the fan triangles' element codes then their Ks -- no load
+// sits behind a branch or behind another load it does not depend on.  The
 // arithmetic and its order are those of assemble_row_fan, so the values are
 // bitwise the same.
-// qw: the quadrature weights by local vertex (fan_weights).
-template <int MODE>
-__device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, int i, const double* lx,
-                                        const double* ly, const double* lk, const double* lx0, const double* qw,
-                                        double (&acc)[3]) {
-    constexpr int W = 8;          // the node + up to 7 neighbours
-    constexpr int T = W - 1;      // fan triangles
-    const uint32_t inf = a.fan_info[i];
-    const int F = (int)(inf & 15u);
-    const bool closed = (inf & 16u) != 0;
-    const bool dir = a.bc_flag != nullptr && a.bc_flag[i];
-    int lc[W];
+struct WinRow {
+    uint32_t inf;      // F | closed << 4 | local vertices << 5
+    int dir;
+    int lc[8];         // window positions of the fan slots
+    int lp;            // of the closed fan's predecessor
+    double ks[7];      // per fan triangle (1 without per-element Ks)
+    double C, thm, thn, hm;
+};
+
+// The row's own inputs.  bc_flag may be null (no Dirichlet rows): the flag
+// byte is then read from fan_info and not used, so that no load sits behind
+// a branch.  code receives the fan triangles' element codes (with per-element
+// Ks; 0 past fan_iw), whose Ks load_win_ks reads once they have arrived.
+__device__ inline void load_win_row(const AsmArgs& a, int i, WinRow& r, int (&code)[7]) {
+    constexpr int W = 8, T = W - 1;
+    r.inf = a.fan_info[i];
+    const uint8_t* bf = a.bc_flag != nullptr ? a.bc_flag : reinterpret_cast<const uint8_t*>(a.fan_info);
+    const int fl = bf[i];
+    r.dir = a.bc_flag != nullptr ? fl : 0;
 #pragma unroll
-    for (int s = 0; s < W; ++s) lc[s] = (int)a.nwin_col[(size_t)(s < a.ell_w ? s : 0) * a.n + i];
-    const int lp = (int)a.nwin_pred[i];
-    double ks[T];
+    for (int s = 0; s < W; ++s) r.lc[s] = (int)a.nwin_col[(size_t)(s < a.ell_w ? s : 0) * a.n + i];
+    r.lp = (int)a.nwin_pred[i];
+    r.C = a.C[i];
+    r.thm = a.theta_m[i];
+    r.thn = a.theta_n[i];
+    r.hm = a.h_m[i];
     if (a.Ks_elem != nullptr) {   // uniform
-        int code[T];
 #pragma unroll
-        for (int u = 0; u < T; ++u) code[u] = u < a.fan_iw ? a.fan_inc[(size_t)u * a.n + i] : 0;
+        for (int u = 0; u < T; ++u) code[u] = a.fan_inc[(size_t)(u < a.fan_iw ? u : 0) * a.n + i];
+    }
+}
+
+__device__ inline void load_win_ks(const AsmArgs& a, const int (&code)[7], WinRow& r) {
+    constexpr int T = 7;
+    if (a.Ks_elem != nullptr) {   // uniform
 #pragma unroll
-        for (int u = 0; u < T; ++u) ks[u] = a.Ks_elem[code[u] >> 2];
+        for (int u = 0; u < T; ++u) r.ks[u] = a.Ks_elem[(u < a.fan_iw ? code[u] : 0) >> 2];
     } else {
 #pragma unroll
-        for (int u = 0; u < T; ++u) ks[u] = 1.0;
+        for (int u = 0; u < T; ++u) r.ks[u] = 1.0;
     }
+}
+
+// qw: the quadrature weights by local vertex (fan_weights).
+template <int MODE>
+__device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, int i, const WinRow& r,
+                                        const double* lx, const double* ly, const double* lk, const double* lx0,
+                                        const double* qw, double (&acc)[3]) {
+    constexpr int W = 8;          // the node + up to 7 neighbours
+    constexpr int T = W - 1;      // fan triangles
+    const uint32_t inf = r.inf;
+    const int F = (int)(inf & 15u);
+    const bool closed = (inf & 16u) != 0;
     double v[W];
 #pragma unroll
     for (int s = 0; s < W; ++s) v[s] = 0.0;
     double bi;
-    if (dir) {
+    if (r.dir) {
         v[0] = 1.0;
-        bi = a.bc_val[i];
+        bi = a.bc_val[i];   // the few Dirichlet rows read theirs here
     } else {
         bi = 0.0;
         double msum = 0.0, vp = 0.0;
-        const double sx = lx[lc[0]], sy = ly[lc[0]], sk = lk[lc[0]];
+        const double sx = lx[r.lc[0]], sy = ly[r.lc[0]], sk = lk[r.lc[0]];
 #pragma unroll
         for (int u = 0; u < T; ++u) {
             const bool ex = closed ? (u < F) : (u >= 1 && u < F);
             if (ex) {
-                const int pp = (u == 0) ? lp : lc[u];
-                const int qq = lc[u + 1];
+                const int pp = (u == 0) ? r.lp : r.lc[u];
+                const int qq = r.lc[u + 1];
                 const int la = (int)((inf >> (5 + 2 * u)) & 3u);
                 const double Px = lx[pp], Py = ly[pp], Pk = lk[pp];
                 const double Qx = lx[qq], Qy = ly[qq], Qk = lk[qq];
@@ -798,7 +828,7 @@ __device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, 
                 const double ex_ = d2x - d1x, ey = d2y - d1y;
                 const double4 w = *reinterpret_cast<const double4*>(qw + 4 * la);
                 const double ckS = w.x, ckP = w.y, ckQ = w.z, mlS = w.w;
-                const double kbar = (ckS * sk + ckP * Pk + ckQ * Qk) * ks[u];
+                const double kbar = (ckS * sk + ckP * Pk + ckQ * Qk) * r.ks[u];
                 const double c = kbar * recip_nr(det);
                 const double kss = c * (ex_ * ex_ + ey * ey);
                 const double ksq = c * (ex_ * d1x + ey * d1y);
@@ -814,8 +844,8 @@ __device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, 
         // the closed fan's triangle 0 joins the last neighbour (slot F)
 #pragma unroll
         for (int s = 1; s < W; ++s) v[s] = (closed && s == F) ? v[s] + vp : v[s];
-        const double ci = a.C[i];
-        const double rhs = a.theta_m[i] - a.theta_n[i] - ci * a.h_m[i];
+        const double ci = r.C;
+        const double rhs = r.thm - r.thn - ci * r.hm;
         v[0] += a.inv_dt * msum * ci;
         bi -= a.inv_dt * msum * rhs;
     }
@@ -836,31 +866,73 @@ __device__ inline void assemble_row_win(const AsmArgs& a, const KrylovInit& ki, 
         double ax = 0.0;
 #pragma unroll
         for (int s = 0; s < W; ++s)
-            if (s <= F) ax += v[s] * lx0[lc[s]];
+            if (s <= F) ax += v[s] * lx0[r.lc[s]];
         krylov_start<MODE>(a, ki, i, bi, ax, di, acc);
     }
 }
 
-// The node window of row block blk into LDS (x, y, K, x0 planes of
-// nwin_cap doubles each), then a workgroup barrier.
-template <int MODE>
-__device__ inline void stage_window(const AsmArgs& a, const KrylovInit& ki, int blk, double* lx, double* ly,
-                                    double* lk, double* lx0) {
-    const double2* xy = reinterpret_cast<const double2*>(a.nrec.coord);
-    const int32_t* w = a.nwin + (size_t)WIN_WORDS * blk;
-    int off = 0;
+// The node window of a row block: up to WIN_SEG segments (lo, len) of
+// consecutive nodes, staged at their running offsets in the x, y, K, x0
+// planes of wl, WIN_CAP doubles apart.  Each thread loads its (at most
+// WIN_CAP / BLOCK rounded up) window positions at once; positions past the
+// window load the first node again and are not stored.  The caller's barrier
+// publishes the planes.
+constexpr int WIN_PER = (WIN_CAP + BLOCK - 1) / BLOCK;
+struct WinStage {
+    double2 p[WIN_PER];
+    double kk[WIN_PER], xx[WIN_PER];
+    int tot;
+};
+
+__device__ inline void load_win_words(const AsmArgs& a, int blk, int2 (&w)[WIN_SEG]) {
+    const int2* src = reinterpret_cast<const int2*>(a.nwin + (size_t)WIN_WORDS * blk);
 #pragma unroll
-    for (int k = 0; k < WIN_SEG; ++k) {
-        const int lo = w[2 * k], len = w[2 * k + 1];
-        for (int j = threadIdx.x; j < len; j += BLOCK) {
-            const double2 p = xy[lo + j];
-            lx[off + j] = p.x;
-            ly[off + j] = p.y;
-            lk[off + j] = a.nrec.K[lo + j];
-            if constexpr (MODE != ASM_INIT_NONE) lx0[off + j] = ki.x0[lo + j];
-        }
-        off += len;
+    for (int k = 0; k < WIN_SEG; ++k) w[k] = src[k];
+}
+
+template <int MODE>
+__device__ inline void stage_load(const AsmArgs& a, const KrylovInit& ki, const int2 (&w)[WIN_SEG], WinStage& st) {
+    static_assert(WIN_SEG == 3, "three segments");
+    const double2* xy = reinterpret_cast<const double2*>(a.nrec.coord);
+    const int l0 = w[0].y, l1 = w[1].y;
+    st.tot = l0 + l1 + w[2].y;
+#pragma unroll
+    for (int m = 0; m < WIN_PER; ++m) {
+        const int q = (int)threadIdx.x + m * BLOCK;
+        int g = q < l0 ? w[0].x + q : (q < l0 + l1 ? w[1].x + (q - l0) : w[2].x + (q - l0 - l1));
+        g = q < st.tot ? g : w[0].x;
+        st.p[m] = xy[g];
+        st.kk[m] = a.nrec.K[g];
+        if constexpr (MODE != ASM_INIT_NONE) st.xx[m] = ki.x0[g];
     }
+}
+
+template <int MODE>
+__device__ inline void stage_store(const WinStage& st, double* wl) {
+#pragma unroll
+    for (int m = 0; m < WIN_PER; ++m) {
+        const int q = (int)threadIdx.x + m * BLOCK;
+        if (q < st.tot) {
+            wl[q] = st.p[m].x;
+            wl[WIN_CAP + q] = st.p[m].y;
+            wl[2 * WIN_CAP + q] = st.kk[m];
+            if constexpr (MODE != ASM_INIT_NONE) wl[3 * WIN_CAP + q] = st.xx[m];
+        }
+    }
+}
+
+// Everything a row block reads from global memory past its window words,
+// issued in dependency order (the row's inputs, then the window and the Ks),
+// the window stored to LDS, then the barrier.
+template <int MODE>
+__device__ inline void load_block(const AsmArgs& a, const KrylovInit& ki, const int2 (&w)[WIN_SEG], int i, WinRow& r,
+                                  double* wl) {
+    int code[7];
+    load_win_row(a, i, r, code);
+    WinStage st;
+    stage_load<MODE>(a, ki, w, st);
+    load_win_ks(a, code, r);
+    stage_store<MODE>(st, wl);
     __syncthreads();
 }
 
@@ -869,10 +941,10 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki
     // the window's planes at fixed offsets (one address per node, immediate
     // offsets for the planes)
     __shared__ double wl[4 * WIN_CAP];
-    double* lx = wl;
-    double* ly = wl + WIN_CAP;
-    double* lk = wl + 2 * WIN_CAP;
-    double* lx0 = wl + 3 * WIN_CAP;
+    const double* lx = wl;
+    const double* ly = wl + WIN_CAP;
+    const double* lk = wl + 2 * WIN_CAP;
+    const double* lx0 = wl + 3 * WIN_CAP;
     // quadrature weights by local vertex (read after the window's barrier)
     __shared__ __align__(16) double qw[12];
     fan_weights(a.qc, qw);
@@ -881,10 +953,13 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki
         if (a.tflag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.tflag = a.n;
         // rows of logical block L, as in k_assemble_reg; its window is block L's
         const int L = xcd_block(blockIdx.x, gridDim.x);
-        stage_window<MODE>(a, ki, L, lx, ly, lk, lx0);
         const int i = L * BLOCK + threadIdx.x;
+        int2 w[WIN_SEG];
+        load_win_words(a, L, w);
+        WinRow r;
+        load_block<MODE>(a, ki, w, i < a.n ? i : a.n - 1, r, wl);
         double acc[3] = {0.0, 0.0, 0.0};
-        if (i < a.n) assemble_row_win<MODE>(a, ki, i, lx, ly, lk, lx0, qw, acc);
+        if (i < a.n) assemble_row_win<MODE>(a, ki, i, r, lx, ly, lk, lx0, qw, acc);
         if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, L);
         return;
     }
@@ -892,15 +967,23 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_win(AsmArgs a, KrylovInit ki
     int v0, v1, vstep;
     xcd_walk(a.act ? a.act[0] : a.n_tab, v0, v1, vstep);
     for (int v = v0; v < v1; v += vstep) {   // uniform over the workgroup
-        const int t = a.act ? a.act[1 + v] : v;
+        const int t = __builtin_amdgcn_readfirstlane(a.act ? a.act[1 + v] : v);
+        int2 w[WIN_SEG];
+        load_win_words(a, t, w);
         const int4 tb = a.tab[t];
-        if (a.ens_st[tb.z].pdone) continue;
-        if (a.tflag != nullptr && threadIdx.x == 0) atomicAdd(a.tflag, tb.y - tb.x);
-        stage_window<MODE>(a, ki, t, lx, ly, lk, lx0);
-        const int i = tb.x + threadIdx.x;
-        double acc[3] = {0.0, 0.0, 0.0};
-        if (i < tb.y) assemble_row_win<MODE>(a, ki, i, lx, ly, lk, lx0, qw, acc);
-        if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
+        const int r0 = __builtin_amdgcn_readfirstlane(tb.x), r1 = __builtin_amdgcn_readfirstlane(tb.y);
+        // the instance's Picard flag is read with the entry's data (listed
+        // entries are not converged; the flag only guards)
+        const int pdv = a.ens_st[__builtin_amdgcn_readfirstlane(tb.z)].pdone;
+        const int i = r0 + threadIdx.x;
+        WinRow r;
+        load_block<MODE>(a, ki, w, i < r1 ? i : r1 - 1, r, wl);
+        if (!__builtin_amdgcn_readfirstlane(pdv)) {   // uniform
+            if (a.tflag != nullptr && threadIdx.x == 0) atomicAdd(a.tflag, r1 - r0);
+            double acc[3] = {0.0, 0.0, 0.0};
+            if (i < r1) assemble_row_win<MODE>(a, ki, i, r, lx, ly, lk, lx0, qw, acc);
+            if constexpr (MODE != ASM_INIT_NONE) store_partials<3>(acc, partials, prow, t);
+        }
         __syncthreads();   // the next entry's window replaces this one
     }
 }
