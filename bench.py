@@ -233,6 +233,7 @@ def _ens_form(insts, groups, precision, tol, args, timing=True, inner=None):
     torch.cuda.synchronize()
     picard = kry = 0
     conv = True
+    lc0 = _launch_counts(ens)
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = ens.step(DT)
@@ -241,11 +242,14 @@ def _ens_form(insts, groups, precision, tol, args, timing=True, inner=None):
         conv &= st["converged"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    lc1 = _launch_counts(ens)
     h, th, _ = ens.state()
+    members = ens.members if groups else [ens]
     out = {"rate": ens.n_inst * args.steps / el, "elapsed": el, "picard": picard / args.steps,
            "krylov": kry / args.steps, "converged": bool(conv), "h": ens.split(h), "theta": ens.split(th),
            "h_warm": h_warm, "n": ens.n, "n_elems": ens.n_elems, "nnz": ens.nnz, "format": ens.spmv_format,
-           "shape": ens.shape}
+           "shape": ens.shape, "launches": _launch_summary(lc0, lc1, args.steps),
+           "windows": all(m.assembly_windows for m in members)}
     assert all(bool(torch.isfinite(x).all()) for x in (h, th)), "non-finite state"
     log(f"{precision}{' (' + inner + ' inner)' if precision == 'mixed' else ''} tol {tol:g}: {out['rate']:.0f} "
         f"instance-steps/s, {out['krylov']:.1f} summed Krylov its/step")
@@ -255,6 +259,28 @@ def _ens_form(insts, groups, precision, tol, args, timing=True, inner=None):
         out.update(spmv=sp, asm=asm, el_ev=el_ev, spmv_big=big)
     ens.close()
     return out
+
+
+def _asm_kernel(windows):
+    """The time-step assembly kernel's name."""
+    return "k_assemble_win" if windows else "k_assemble_reg"
+
+
+def _launch_counts(ens):
+    """(launches, Picard rounds) per context of an Ensemble or EnsembleGroups."""
+    lc = ens.launch_count()
+    return lc if isinstance(lc, list) else [lc]
+
+
+def _launch_summary(lc0, lc1, steps):
+    """Kernel launches of the timed steps: per step over all contexts, and per
+    Picard round of each context (its launches over the rounds it ran)."""
+    dl = [b[0] - a[0] for a, b in zip(lc0, lc1)]
+    dr = [b[1] - a[1] for a, b in zip(lc0, lc1)]
+    return {"per_step": round(sum(dl) / steps, 1),
+            "per_picard_round_by_context": [round(l / r, 1) if r else None for l, r in zip(dl, dr)],
+            "picard_rounds_per_step_by_context": [round(r / steps, 2) for r in dr],
+            "what": "kernel launches enqueued (device copies and graph replays one each), from wspm_launch_count"}
 
 
 def _plain_rate(insts, tol, steps):
@@ -383,7 +409,7 @@ def run_headline(args):
                   "instance_steps_per_s": round(mixed["rate"], 1),
                   "max_picard_iters_per_step": round(mixed["picard"], 3),
                   "fp32_inner_iters_per_step_summed": round(mixed["krylov"], 1),
-                  "all_converged": mixed["converged"]},
+                  "all_converged": mixed["converged"], "launches": mixed["launches"]},
         f"mixed_{other}_inner": {"inner": _inner_desc(args, other),
                                  "instance_steps_per_s": round(mixed_o["rate"], 1),
                                  "fp32_inner_iters_per_step_summed": round(mixed_o["krylov"], 1),
@@ -393,7 +419,8 @@ def run_headline(args):
         "batched_fp64": {"krylov_tol_1e-10_instance_steps_per_s": round(f64r["rate"], 1),
                          f"krylov_tol_{TIGHT_TOL:g}_instance_steps_per_s": round(f64t["rate"], 1),
                          "krylov_iters_per_step_summed_tol_1e-10": round(f64r["krylov"], 1),
-                         f"krylov_iters_per_step_summed_tol_{TIGHT_TOL:g}": round(f64t["krylov"], 1)},
+                         f"krylov_iters_per_step_summed_tol_{TIGHT_TOL:g}": round(f64t["krylov"], 1),
+                         "launches_tol_1e-10": f64r["launches"]},
         "errors_after_steps": args.warmup + args.steps,
         "mixed_vs_fp64_same_tol": _err_summary(mixed["h"], f64t["h"], mixed["theta"], f64t["theta"]),
         "mixed_vs_fp64_tol_1e-10": _err_summary(mixed["h"], f64r["h"]),
@@ -422,9 +449,9 @@ def run_headline(args):
            "config": cfg}
     if "asm" in mixed:
         la, ma, ra = mixed["asm"]
-        out["roofline"] = roofline("k_assemble_reg (fused P1 assembly + Dirichlet rows + Jacobi + refinement "
-                                   "residual, one thread per row)", assembly_bytes(n, ne, nnz, 2, False), n,
-                                   la, ma, ra, "assembly_c3_mixed")
+        out["roofline"] = roofline(_asm_kernel(mixed["windows"]) + " (fused P1 assembly + Dirichlet rows + Jacobi "
+                                   "+ refinement residual, one thread per row)", assembly_bytes(n, ne, nnz, 2, False),
+                                   n, la, ma, ra, "assembly_c3_mixed")
         ls, ms, rs = mixed["spmv"]
         epi = "Jacobi-step epilogues" if args.inner == "jacobi" else "BiCGSTAB epilogues"
         out["roofline_spmv"] = roofline(f"k_spmv_{mixed['format']} (FP32 SpMV of the inner solves, fused {epi})",
@@ -461,9 +488,10 @@ def run_ensemble_form(args):
            "config": {"workload": WORKLOADS[args.workload][0], "krylov_tol": tol, "instances": len(insts),
                       "nodes": n, "elements": ne, "nnz": nnz, "krylov_iters_per_step_summed": round(r["krylov"], 1),
                       "max_picard_iters_per_step": round(r["picard"], 3), "launch_shape": r["shape"],
-                      "launch": launch_note()}}
+                      "launch": launch_note(), "launches": r["launches"]}}
     if "asm" in r:
-        out["roofline"] = roofline("k_assemble_reg (fused assembly)", assembly_bytes(n, ne, nnz, 2, False), n,
+        out["roofline"] = roofline(_asm_kernel(r["windows"]) + " (fused assembly)",
+                                   assembly_bytes(n, ne, nnz, 2, False), n,
                                    *r["asm"], "assembly_c3_mixed" if precision == "mixed" else "assembly_c3")
         out["roofline_spmv"] = roofline(f"k_spmv_{r['format']}", spmv_bytes(n, nnz, 4 if precision == "mixed" else 8),
                                         n, *r["spmv"], None if precision == "mixed" else "spmv_c3_fp64")
@@ -531,6 +559,7 @@ def run_single(args):
     h_warm = ctx.richards_state()[0].cpu().numpy()
     torch.cuda.synchronize()
     picard = kry = 0
+    lc0 = _launch_counts(ctx)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = ctx.richards_step(dt)
@@ -538,17 +567,19 @@ def run_single(args):
         kry += st["krylov_iters"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    launches = _launch_summary(lc0, _launch_counts(ctx), args.steps)
     h, th, _ = ctx.richards_state()
     h, th = h.cpu().numpy(), th.cpu().numpy()
     assert np.all(np.isfinite(h)), "non-finite pressure head"
     sp, asm, el_ev, _ = timed_region(ctx, ctx.richards_step, dt, 0 if args.no_timing else args.steps, mixed)
     n, ne, nnz = ctx.n, ctx.n_elems, ctx.nnz
     fmt = ctx.spmv_format
+    asm_name = _asm_kernel(ctx.assembly_windows) if dim == 2 else "k_assemble"
     ctx.close()
     value = args.steps / el
     cfgd = {"workload": desc, "nodes": n, "elements": ne, "nnz": nnz, "dt": dt, "solver": f"{solver}+jacobi",
             "precision": precision, "krylov_tol": tol, "picard_iters_per_step": round(picard / args.steps, 3),
-            "krylov_iters_per_step": round(kry / args.steps, 3), "launch": launch_note(),
+            "krylov_iters_per_step": round(kry / args.steps, 3), "launch": launch_note(), "launches": launches,
             "timing_region": (f"{args.steps} further steps of the same run with HIP events around every "
                               f"assembly and {'FP32 ' if mixed else ''}SpMV launch "
                               f"({el_ev / max(args.steps, 1) * 1e3:.4f} ms/step there)")}
@@ -570,8 +601,7 @@ def run_single(args):
            "data": "synthetic (structured mesh" + (" + seeded correlated Ks field" if ksf else "") +
                    " generated in-process)",
            "config": cfgd,
-           "roofline": roofline(f"k_assemble{'_reg' if dim == 2 else ''} (fused P1 assembly + Dirichlet rows + "
-                                "Jacobi + solver start)", assembly_bytes(n, ne, nnz, dim, ksf is not None), n,
+           "roofline": roofline(f"{asm_name} (fused P1 assembly + Dirichlet rows + Jacobi + solver start)", assembly_bytes(n, ne, nnz, dim, ksf is not None), n,
                                 *asm, "assembly_c1" if args.workload == "c1" else None),
            "roofline_spmv": roofline(f"k_spmv_{fmt} ({'FP32' if mixed else 'FP64'} SpMV, fused Krylov epilogues)",
                                      spmv_bytes(n, nnz, 4 if mixed else 8), n, *sp,

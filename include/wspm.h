@@ -250,6 +250,12 @@ int32_t wspm_timing_read_over(wspm_ctx* ctx, int64_t min_rows, int64_t* launches
  * too; this reads them as wspm_timing_read does (launches that assembled no
  * row, e.g. a Picard iteration enqueued ahead and not needed, are left out). */
 int32_t wspm_timing_read_asm(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows);
+/* Kernel launches (device-to-device copies and HIP graph replays counted as
+ * one each) the context has enqueued on its stream since its creation, and
+ * the Picard rounds its time steps have run: an ensemble step's rounds over
+ * the instances still iterating, or one system's Picard iterations.  No
+ * counterpart in the reference (the benchmark reports launches per round). */
+int32_t wspm_launch_count(const wspm_ctx* ctx, int64_t* launches, int64_t* picard_rounds);
 
 /* Transport (2D) -------------------------------------------------------------
  * Boundary node lists for edge terms, in the reference's list order: an edge

@@ -95,6 +95,7 @@ PROTOTYPES = {
     "wspm_timing_enable": (_I, [_P, _I]),
     "wspm_timing_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "wspm_timing_read_asm": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "wspm_launch_count": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "wspm_timing_read_over": (_I, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                    C.POINTER(C.c_int64)]),
     "wspm_edge_set": (_I, [_P, _I, _I, _P]),

@@ -53,6 +53,13 @@ struct ApiFail {
         }                                                                               \
     } while (0)
 
+// A kernel launch on the context's stream, counted (wspm_launch_count).
+#define LAUNCH(...)                      \
+    do {                                 \
+        ++ctx->n_launch;                 \
+        hipLaunchKernelGGL(__VA_ARGS__); \
+    } while (0)
+
 bool env_is(const char* name, const char* value) {
     const char* e = std::getenv(name);
     return e && std::string(e) == value;
@@ -800,6 +807,8 @@ struct wspm_ctx {
     bool timing = false;
     bool timing_fp32 = false;       // time the FP32 SpMV launches instead of the FP64 ones
     bool timing_asm = false;        // also time the assembly launches (separate events and slots)
+    int64_t n_launch = 0;           // kernel launches and device copies enqueued (wspm_launch_count)
+    int64_t n_picard = 0;           // Picard rounds of the time steps
     int* asm_tflags = nullptr;      // [TIMING_SLOTS] rows each timed assembly launch worked on
     std::vector<hipEvent_t> asm_ev;
     size_t asm_ev_used = 0;
@@ -932,7 +941,7 @@ EnsWalk walk_of(wspm_ctx* ctx, int list) {
 // mode as ens_on (default: 0 for LIST_K, 1 for LIST_P).
 void ens_compact(wspm_ctx* ctx, int list, int mode = -1) {
     if (mode < 0) mode = list == LIST_K ? 0 : 1;
-    hipLaunchKernelGGL(k_ens_compact, dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, (const KState*)ctx->states,
+    LAUNCH(k_ens_compact, dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, (const KState*)ctx->states,
                        ctx->n_inst, (const int*)ctx->iblk, list == LIST_K ? ctx->kact : ctx->pact, mode,
                        list == LIST_K ? ctx->kact_alt : nullptr);
     HIPCHK(hipGetLastError());
@@ -950,15 +959,15 @@ template <class Op, bool RESET_PEV = false>
 void launch_finish(wspm_ctx* ctx, const Op& op, int nblk, int which, bool compact_k = false, int cmode = 0) {
     if constexpr (Op::ND > 0) {
         if (ctx->tab && compact_k) {
-            hipLaunchKernelGGL((k_finish_ens_k<Op::ND, Op>), dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, op,
+            LAUNCH((k_finish_ens_k<Op::ND, Op>), dim3(ctx->n_inst), dim3(BLOCK), 0, ctx->stream, op,
                                ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which), ctx->kact_alt,
                                ctx->kact, cmode);
             std::swap(ctx->kact, ctx->kact_alt);
         } else if (ctx->tab) {
-            hipLaunchKernelGGL((k_finish_ens<Op::ND, Op, RESET_PEV>), dim3(ctx->n_inst), dim3(BLOCK), 0,
+            LAUNCH((k_finish_ens<Op::ND, Op, RESET_PEV>), dim3(ctx->n_inst), dim3(BLOCK), 0,
                                ctx->stream, op, ctx->partials, ctx->prow, ctx->iblk, states_of(ctx, which));
         } else {
-            hipLaunchKernelGGL((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials,
+            LAUNCH((k_finish<Op::ND, Op>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials,
                                ctx->prow, nblk, states_of(ctx, which));
         }
         HIPCHK(hipGetLastError());
@@ -970,7 +979,7 @@ template <class Op>
 void launch_vec(wspm_ctx* ctx, const Op& op, int which, int list = LIST_ALL, bool compact_k = false,
                 int cmode = 0) {
     const int grid = ctx->tab ? ctx->ens_grid : ctx->vec_grid;
-    hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
+    LAUNCH(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, op, red_of(ctx, which),
                        walk_of(ctx, list));
     HIPCHK(hipGetLastError());
     launch_finish(ctx, op, grid, which, compact_k, cmode);
@@ -1008,6 +1017,7 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
     if (part) red.partials = part;
     const EnsWalk tab = walk_of(ctx, list);
     auto go = [&](auto kernel, dim3 g, auto... args) {
+        ++ctx->n_launch;
         if (t) hipExtLaunchKernelGGL(kernel, g, dim3(BLOCK), 0, ctx->stream, e0, e1, 0, args...);
         else hipLaunchKernelGGL(kernel, g, dim3(BLOCK), 0, ctx->stream, args...);
         HIPCHK(hipGetLastError());
@@ -1073,7 +1083,7 @@ void wait_published(wspm_ctx* ctx) {
 
 // Enqueue the publication of state `which` into ctx->hstate[which].
 void publish_state(wspm_ctx* ctx, int which) {
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(BLOCK), 0, ctx->stream,
+    LAUNCH(k_publish, dim3(1), dim3(BLOCK), 0, ctx->stream,
                        reinterpret_cast<const uint32_t*>(states_of(ctx, which)), (int)(sizeof(KState) / 4),
                        reinterpret_cast<uint32_t*>(ctx->hstate + which), ctx->dseq, ctx->hseq);
     HIPCHK(hipGetLastError());
@@ -1093,7 +1103,7 @@ void poll(wspm_ctx* ctx, int which) {
 // Ensemble: counts of running instances (k_ens_status), published straight
 // into ctx->hstatus (fast_poll) or copied there.
 void ens_status_publish(wspm_ctx* ctx, int which) {
-    hipLaunchKernelGGL(k_ens_status, dim3(1), dim3(BLOCK), 0, ctx->stream, states_of(ctx, which), ctx->n_inst,
+    LAUNCH(k_ens_status, dim3(1), dim3(BLOCK), 0, ctx->stream, states_of(ctx, which), ctx->n_inst,
                        ctx->fast_poll ? ctx->hstatus : ctx->ens_status, ctx->dseq,
                        ctx->fast_poll ? ctx->hseq : nullptr);
     HIPCHK(hipGetLastError());
@@ -1122,10 +1132,10 @@ void picard_enqueue(wspm_ctx* ctx, const PicardOp& pop, KState* hdst = nullptr, 
         return;
     }
     const int grid = ctx->vec_grid;
-    hipLaunchKernelGGL(k_vec<PicardOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, pop, red_of(ctx, 0),
+    LAUNCH(k_vec<PicardOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->n, pop, red_of(ctx, 0),
                        walk_of(ctx, LIST_ALL));
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_finish_pub<PicardOp::ND, PicardOp>), dim3(1), dim3(BLOCK), 0, ctx->stream, pop,
+    LAUNCH((k_finish_pub<PicardOp::ND, PicardOp>), dim3(1), dim3(BLOCK), 0, ctx->stream, pop,
                        ctx->partials, ctx->prow, grid, states_of(ctx, 0),
                        reinterpret_cast<uint32_t*>(hdst ? hdst : ctx->hstate), ctx->dseq, ctx->hseq, pop.gate,
                        gate_out, gtol, gallow);
@@ -1307,7 +1317,7 @@ struct KrylovRun {
     void vec_nf(const Op& op, double* part, int rows, int grid) {
         Red red = red_of(ctx, which);
         red.partials = part ? part : ctx->partials;
-        hipLaunchKernelGGL(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, rows, op, red,
+        LAUNCH(k_vec<Op>, dim3(grid), dim3(BLOCK), 0, ctx->stream, rows, op, red,
                            EnsWalk{nullptr, nullptr, 0});
         HIPCHK(hipGetLastError());
     }
@@ -1457,7 +1467,7 @@ struct KrylovRun {
     }
 
     void gmres_cycle() {
-        hipLaunchKernelGGL(k_gm_cycle_begin, dim3(1), dim3(256), 0, ctx->stream, ds, H, R);
+        LAUNCH(k_gm_cycle_begin, dim3(1), dim3(256), 0, ctx->stream, ds, H, R);
         HIPCHK(hipGetLastError());
         for (int j = 0; j < R; ++j) {
             launch_spmv<T>(ctx, A, GmArnoldi<T>{v, j}, which, true);
@@ -1465,10 +1475,10 @@ struct KrylovRun {
                 apply(v.w, (T*)ctx->pc.t1, 2);
                 launch_vec(ctx, GmPcNorm<T>{v, (T*)ctx->pc.t1}, which);
             }
-            hipLaunchKernelGGL(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
+            LAUNCH(k_gm_dots<T>, dim3(nbx, (j + 1 + 7) / 8), dim3(BLOCK), 0, ctx->stream, v, j,
                                red_of(ctx, which));
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials,
+            LAUNCH(k_gm_dots_finish, dim3(1), dim3(BLOCK), 0, ctx->stream, ds, ctx->partials,
                                ctx->prow, j, nbx);
             HIPCHK(hipGetLastError());
             GmUpdate<T> up{v, j, H};
@@ -1476,7 +1486,7 @@ struct KrylovRun {
             launch_vec(ctx, up, which);
             launch_vec(ctx, GmNormalize<T>{v, j, 0.0}, which);
         }
-        hipLaunchKernelGGL(k_gm_lstsq, dim3(1), dim3(64), 0, ctx->stream, ds, H, R);
+        LAUNCH(k_gm_lstsq, dim3(1), dim3(64), 0, ctx->stream, ds, H, R);
         HIPCHK(hipGetLastError());
         launch_vec(ctx, GmUpdX<T>{v, R, state_field<double>(ds, offsetof(KState, y))}, which);
         launch_spmv<T>(ctx, A, GmResid<T>{v}, which, true);
@@ -1831,7 +1841,7 @@ int32_t wspm_soil(wspm_ctx* ctx, int32_t model, const double* params, int32_t n,
     if (n > 0) {
         const SoilOp op{make_soil(model, p), h, C, K, theta, nullptr, 0};
         const int grid = std::min((n + BLOCK - 1) / BLOCK, 1024);
-        hipLaunchKernelGGL(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0),
+        LAUNCH(k_vec<SoilOp>, dim3(grid), dim3(BLOCK), 0, ctx->stream, n, op, red_of(ctx, 0),
                            EnsWalk{nullptr, nullptr, 0});
         HIPCHK(hipGetLastError());
     }
@@ -1945,6 +1955,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
         ctx->asm_ev_used += 2;
         HIPCHK(hipEventRecord(ae0, ctx->stream));
     }
+    ++ctx->n_launch;
     HIPCHK(launch_assemble(ctx->dim, a, mode, ki, ctx->partials, ctx->prow, ctx->asm_lds, &grid, ctx->stream,
                            kernel));
     if (timed) HIPCHK(hipEventRecord(ae1, ctx->stream));
@@ -2099,7 +2110,7 @@ int32_t wspm_krylov(wspm_ctx* ctx, int32_t method, const double* A_vals, const d
     wspm_krylov_stats st{};
     krylov_solve<double>(ctx, method, A_vals, dinv, b, x0, x, tol, atol, maxiter, restart, 0, &st);
     if (method == WSPM_GMRES) {
-        hipLaunchKernelGGL(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
+        LAUNCH(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
         HIPCHK(hipGetLastError());
         poll(ctx, 0);
         st.code = ctx->hstate[0].nan_flag ? -1 : 0;
@@ -2223,7 +2234,7 @@ int32_t wspm_krylov_pc(wspm_ctx* ctx, int32_t method, int32_t precond, const dou
     wspm_krylov_stats st{};
     krylov_solve<double>(ctx, method, A_vals, dinv, b, x0, x, tol, atol, maxiter, restart, 0, &st, false, precond);
     if (method == WSPM_GMRES) {
-        hipLaunchKernelGGL(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
+        LAUNCH(k_nan_check, dim3(1), dim3(256), 0, ctx->stream, ctx->states, x, ctx->n);
         HIPCHK(hipGetLastError());
         poll(ctx, 0);
         st.code = ctx->hstate[0].nan_flag ? -1 : 0;
@@ -2371,7 +2382,7 @@ int32_t wspm_krylov_mixed(wspm_ctx* ctx, int32_t method, const double* A_vals, c
     {
         ToF32 op{A_vals, ctx->A32};
         const int g = (int)std::min<int64_t>((ctx->nnz + BLOCK - 1) / BLOCK, 4096);
-        hipLaunchKernelGGL(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2),
+        LAUNCH(k_vec<ToF32>, dim3(g), dim3(BLOCK), 0, ctx->stream, (int)ctx->nnz, op, red_of(ctx, 2),
                            EnsWalk{nullptr, nullptr, 0});  // ND = 0
         HIPCHK(hipGetLastError());
         if (dinv) launch_vec(ctx, ToF32{dinv, ctx->dinv32}, 2);
@@ -2706,6 +2717,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         ctx->nrec_dirty = false;
     }
     // theta_n = theta(h_n): the soil state at the start of the step is current
+    ++ctx->n_launch;
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     int it = 0, kry = 0;
     double err = INFINITY;
@@ -2752,6 +2764,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                 picard_enqueue(ctx, pop);
             });
             run.enq = K;
+            ++ctx->n_launch;
             HIPCHK(hipGraphLaunch(ex, ctx->stream));
             wait_published(ctx);
             if (run.more()) {
@@ -2877,6 +2890,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         } else if (mixed) {
             const float* d32 = jac ? ctx->dinv32 : nullptr;
             if (own_start)   // the refinement starts from h_m
+                ++ctx->n_launch;
                 HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice,
                                       ctx->stream));
             bool started = !own_start, refined = false;
@@ -2928,6 +2942,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         ++it;
         if (std::isnan(err)) break;
     }
+    ctx->n_picard += it;
     if (stats) {
         stats->picard_iters = it;
         stats->krylov_iters = kry;
@@ -3024,6 +3039,13 @@ int32_t wspm_timing_read_over(wspm_ctx* ctx, int64_t min_rows, int64_t* launches
     timing_sum(ctx, min_rows, launches, total_ms, rows);
     return WSPM_OK;
     API_END
+}
+
+int32_t wspm_launch_count(const wspm_ctx* ctx, int64_t* launches, int64_t* picard_rounds) {
+    if (!ctx || !launches || !picard_rounds) return WSPM_EINVAL;
+    *launches = ctx->n_launch;
+    *picard_rounds = ctx->n_picard;
+    return WSPM_OK;
 }
 
 int32_t wspm_timing_read(wspm_ctx* ctx, int64_t* launches, double* total_ms, int64_t* rows) {
@@ -3178,10 +3200,10 @@ static void ensure_tpos(wspm_ctx* ctx) {
 static void adj_reduce(wspm_ctx* ctx, const double* a, int len, double* out) {
     const AdjSum op{a, len, out};
     const int g = std::max(1, std::min((len + BLOCK - 1) / BLOCK, 1024));
-    hipLaunchKernelGGL(k_vec<AdjSum>, dim3(g), dim3(BLOCK), 0, ctx->stream, len, op, red_of(ctx, 0),
+    LAUNCH(k_vec<AdjSum>, dim3(g), dim3(BLOCK), 0, ctx->stream, len, op, red_of(ctx, 0),
                        EnsWalk{nullptr, nullptr, 0});
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_finish<2, AdjSum>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials, ctx->prow, g,
+    LAUNCH((k_finish<2, AdjSum>), dim3(1), dim3(BLOCK), 0, ctx->stream, op, ctx->partials, ctx->prow, g,
                        ctx->states);
     HIPCHK(hipGetLastError());
 }
@@ -3671,9 +3693,10 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
     const double* Ks = ctx->has_ks ? ctx->Ks : nullptr;
     KState* st0 = states_of(ctx, 0);
     const int ni = ctx->n_inst;
-    hipLaunchKernelGGL(k_ens_step_begin, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni);
+    LAUNCH(k_ens_step_begin, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni);
     HIPCHK(hipGetLastError());
     ens_compact(ctx, LIST_P);
+    ++ctx->n_launch;
     HIPCHK(hipMemcpyAsync(ctx->thn, ctx->thm, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     const bool mixed = P.precision == WSPM_MIXED;
     const int mode = mixed ? ASM_INIT_RESID : P.method == WSPM_BICGSTAB ? ASM_INIT_BICG : ASM_INIT_CG;
@@ -3718,6 +3741,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             }
             ens_compact(ctx, LIST_P);
             ++it;
+            ++ctx->n_picard;
             if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
             continue;
         }
@@ -3739,6 +3763,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         if (graphs) {
             const hipGraphExec_t ex = step_graph(ctx, GRAPH_ENS_PICARD, dt, K, first);
             run.enq = K;
+            ++ctx->n_launch;
             HIPCHK(hipGraphLaunch(ex, ctx->stream));
             wait_published(ctx);
         } else {
@@ -3762,7 +3787,8 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                     ens_status_publish(ctx, 0);
                 });
                 run.enq = enq0 + 2;
-                HIPCHK(hipGraphLaunch(ex, ctx->stream));
+                ++ctx->n_launch;
+            HIPCHK(hipGraphLaunch(ex, ctx->stream));
                 wait_published(ctx);
                 continue;
             }
@@ -3773,6 +3799,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         ens_compact(ctx, LIST_P);
         ctx->last_iters[P.method] = std::max(1, ctx->hstatus[2]);
         ++it;
+        ++ctx->n_picard;
         if (ctx->hstatus[1] == 0 || it >= P.picard_maxit) break;
     }
     if (stats) {
@@ -3780,7 +3807,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
         int* ik = ip + ni;
         int* ic = ik + ni;
         double* er = reinterpret_cast<double*>(ctx->ens_out + 3 * (size_t)ni + (ni & 1));
-        hipLaunchKernelGGL(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni,
+        LAUNCH(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, st0, ni,
                            ip, ik, ic, er);
         HIPCHK(hipGetLastError());
         std::vector<int> hi(3 * (size_t)ni);
@@ -3815,7 +3842,7 @@ int32_t wspm_ens_step_stats(wspm_ctx* ctx, int32_t* picard_iters, int32_t* krylo
     int* ik = ip + ni;
     int* ic = ik + ni;
     double* er = reinterpret_cast<double*>(ctx->ens_out + 3 * (size_t)ni + (ni & 1));
-    hipLaunchKernelGGL(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
+    LAUNCH(k_ens_collect, dim3((ni + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
                        states_of(ctx, 0), ni, ip, ik, ic, er);
     HIPCHK(hipGetLastError());
     if (picard_iters) HIPCHK(hipMemcpyAsync(picard_iters, ip, ni * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -322,6 +322,13 @@ class Context:
         self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
         return int(n.value), float(ms.value), int(rows.value)
 
+    def launch_count(self):
+        """(kernel launches enqueued since creation, Picard rounds run)."""
+        n = C.c_int64()
+        r = C.c_int64()
+        self._ck(self.lib.wspm_launch_count(self.h, C.byref(n), C.byref(r)))
+        return int(n.value), int(r.value)
+
     def timing_read_over(self, min_rows: int):
         """timing_read's figures over the launches of at least min_rows rows,
         without resetting the timing slots (read these first)."""

@@ -222,6 +222,13 @@ class Ensemble:
         self._ck(self.lib.wspm_timing_read_asm(self.h, C.byref(n), C.byref(ms), C.byref(rows)))
         return int(n.value), float(ms.value), int(rows.value)
 
+    def launch_count(self):
+        """(kernel launches enqueued since creation, Picard rounds run)."""
+        n = C.c_int64()
+        r = C.c_int64()
+        self._ck(self.lib.wspm_launch_count(self.h, C.byref(n), C.byref(r)))
+        return int(n.value), int(r.value)
+
     def timing_read_over(self, min_rows: int):
         """timing_read's figures over the launches of at least min_rows rows,
         without resetting the timing slots (read these first)."""
@@ -360,6 +367,10 @@ class EnsembleGroups:
     def timing_read_asm(self):
         res = self._each(lambda m: m.timing_read_asm())
         return sum(r[0] for r in res), sum(r[1] for r in res), sum(r[2] for r in res)
+
+    def launch_count(self):
+        """launch_count of every member context, in member order."""
+        return self._each(lambda m: m.launch_count())
 
     @property
     def shape(self) -> dict:

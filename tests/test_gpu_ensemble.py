@@ -276,3 +276,31 @@ def test_timing_hooks_count_working_launches():
     assert ens.timing_read() == (0, 0.0, 0)
     ens.timing(False)
     ens.close()
+
+
+def test_launch_count_follows_steps():
+    """wspm_launch_count: a step enqueues at least an assembly, an SpMV and a
+    Picard update per Picard round, and the rounds an ensemble step runs are
+    the most Picard iterations any of its instances took; one system counts
+    its Picard iterations."""
+    from jax_wspm_amd.device import Context
+    from jax_wspm_amd.ensemble import Ensemble, vg_ensemble
+    insts = vg_ensemble(n_inst=4, sides=(8, 16, 24, 32), seed=2)
+    ens = Ensemble(insts)
+    ens.richards_init("vg", "bicgstab", "jacobi")
+    l0, r0 = ens.launch_count()
+    st = ens.step(1e-4)
+    l1, r1 = ens.launch_count()
+    assert r1 - r0 == st["picard_iters"] >= 1
+    assert l1 - l0 >= 3 * (r1 - r0)
+    ens.close()
+    inst = insts[1]
+    ctx = Context(inst.points, inst.elements)
+    ctx.set_dirichlet(inst.bc_lists)
+    ctx.richards_init("vg", inst.phi, "bicgstab", "jacobi", inst.h0)
+    l0, r0 = ctx.launch_count()
+    st = ctx.richards_step(1e-4)
+    l1, r1 = ctx.launch_count()
+    assert r1 - r0 == st["picard_iters"] >= 1
+    assert l1 - l0 >= 3 * (r1 - r0)
+    ctx.close()
