@@ -204,10 +204,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __res
         double acc[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += BLOCK) {
-#pragma unroll
-            for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
-        }
+        strided_sum<ND>(partials, prow, 0, nblk, acc);
         __shared__ double sh[ND * NWAVE];
         block_sum<ND>(acc, sh);
         if (threadIdx.x == 0) {

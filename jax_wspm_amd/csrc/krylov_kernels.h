@@ -439,10 +439,7 @@ __device__ inline void reduce_all(const FoldIn& in, double (&tot)[ND]) {
     double acc[ND];
 #pragma unroll
     for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-    for (int b = threadIdx.x; b < in.nblk; b += BLOCK) {
-#pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += in.p[(size_t)j * in.prow + b];
-    }
+    strided_sum<ND>(in.p, in.prow, 0, in.nblk, acc);
     __shared__ double sh[ND * NWAVE];
     __shared__ double bc[ND];
     block_sum<ND>(acc, sh);

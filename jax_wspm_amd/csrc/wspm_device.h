@@ -188,6 +188,32 @@ __device__ inline void store_partials(double (&v)[ND], double* partials, int pro
     }
 }
 
+// acc[j] += p[j * prow + b] for this thread's columns b = b0 + threadIdx.x,
+// + BLOCK, ... below b1, in that order.  The loads of SUM_U columns are
+// issued before their adds, so a thread waits for one memory round trip per
+// SUM_U columns instead of per column; each acc[j] still adds its columns in
+// column order, so the sums are bitwise those of the one-column loop.
+constexpr int SUM_U = 8;
+template <int ND>
+__device__ inline void strided_sum(const double* __restrict__ p, int prow, int b0, int b1, double (&acc)[ND]) {
+    int b = b0 + (int)threadIdx.x;
+    for (; b + (SUM_U - 1) * BLOCK < b1; b += SUM_U * BLOCK) {
+        double v[SUM_U][ND];
+#pragma unroll
+        for (int u = 0; u < SUM_U; ++u)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) v[u][j] = p[(size_t)j * prow + b + u * BLOCK];
+#pragma unroll
+        for (int u = 0; u < SUM_U; ++u)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) acc[j] += v[u][j];
+    }
+    for (; b < b1; b += BLOCK) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] += p[(size_t)j * prow + b];
+    }
+}
+
 // Sum nblk partials per dot in workgroup order and run op.fin(totals, state).
 // One workgroup; the reads are coalesced.
 template <int ND, class Op>
@@ -197,10 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Op op, const double* __restric
     double acc[ND];
 #pragma unroll
     for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += BLOCK) {
-#pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
-    }
+    strided_sum<ND>(partials, prow, 0, nblk, acc);
     __shared__ double sh[ND * NWAVE];
     block_sum<ND>(acc, sh);
     if (threadIdx.x == 0) op.fin(acc, st);
@@ -213,10 +236,7 @@ template <int ND>
 __device__ inline void ens_reduce(const double* __restrict__ partials, int prow, int b0, int b1, double (&acc)[ND]) {
 #pragma unroll
     for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += BLOCK) {
-#pragma unroll
-        for (int j = 0; j < ND; ++j) acc[j] += partials[(size_t)j * prow + b];
-    }
+    strided_sum<ND>(partials, prow, b0, b1, acc);
     __shared__ double sh[ND * NWAVE];
     block_sum<ND>(acc, sh);
 }
