@@ -201,8 +201,85 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     double* con = lds + 2 * a.win_cap;                    // [inc_cap][NC] incidence contributions
     uint32_t* slw = reinterpret_cast<uint32_t*>(con + (size_t)a.inc_cap * NC);  // [inc_cap] slot words
 
-    // 0. row-level metadata and the x0 gathers (all global reads happen in 0-1)
-    for (int j = threadIdx.x; j <= nrows; j += ASM_THREADS) {
+    // 0-1. all global reads: the row metadata, the x0 gathers of the window
+    //      and the element contributions of every incidence of the workgroup.
+    //      Each thread's first pass of the three (its row, its window entry,
+    //      its first pair of incidences) is loaded at once with indices
+    //      clamped into range, so that no load waits behind a branch or
+    //      behind an independent one: the incidences, metadata and columns,
+    //      then the elements and x0, then the node records -- three round
+    //      trips instead of six.  Longer blocks finish in the loops below.
+    auto elem_pair = [&](int t0, const int2 ic0, const int2 ic1, const double4 (&r0c)[K], const double4 (&r1c)[K],
+                         double ks0, double ks1) {
+        const int t1 = t0 + ASM_THREADS;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int t = h == 0 ? t0 : t1;
+            if (t >= nq) break;
+            const double4* r = h == 0 ? r0c : r1c;
+            double px[K], py[K], pz[K], kn[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                px[c] = r[c].x;
+                py[c] = r[c].y;
+                pz[c] = (D == 3) ? r[c].z : 0.0;
+                kn[c] = (D == 3) ? r[c].w : r[c].z;
+            }
+            double out[D + 3];
+            const int2 ic = h == 0 ? ic0 : ic1;
+            element_compute<D>(a, px, py, pz, kn, h == 0 ? ks0 : ks1, ic.x & 3, out);
+            slw[t] = (uint32_t)ic.y;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
+        }
+    };
+    {
+        const int tid = threadIdx.x;
+        // a block always has rows, window entries and (on a mesh) incidences
+        const int qa = q0 + (nq > 0 ? min(tid, nq - 1) : 0);
+        const int qb = q0 + (nq > 0 ? min(tid + ASM_THREADS, nq - 1) : 0);
+        const int2 ic0 = a.inc[qa];
+        const int2 ic1 = a.inc[qb];
+        const int jm = min(tid, nrows), jr = min(tid, nrows - 1);
+        const int rpv = a.row_ptr[r0 + jm];
+        const int ipv = a.inc_ptr[r0 + jm];
+        const int dgv = a.diag_pos[r0 + jr];
+        const uint8_t* bf = a.bc_flag != nullptr ? a.bc_flag : reinterpret_cast<const uint8_t*>(a.row_ptr);
+        const int fl = bf[r0 + jr];
+        const int bcv = a.bc_flag != nullptr ? fl : 0;
+        const double ci = a.C[r0 + jr];
+        const double rsv = a.theta_m[r0 + jr] - a.theta_n[r0 + jr] - ci * a.h_m[r0 + jr];
+        const int colk = MODE != ASM_INIT_NONE ? a.cols[base + min(tid, wlen - 1)] : 0;
+        const int4 cn0 = a.conn4[ic0.x >> 2];
+        const int4 cn1 = a.conn4[ic1.x >> 2];
+        double x0k = 0.0;
+        if constexpr (MODE != ASM_INIT_NONE) x0k = ki.x0[colk];
+        const int nd0[4] = {cn0.x, cn0.y, cn0.z, cn0.w};
+        const int nd1[4] = {cn1.x, cn1.y, cn1.z, cn1.w};
+        double4 rc0[K], rc1[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            rc0[c] = a.nrec.template rec<D>(nd0[c]);
+            rc1[c] = a.nrec.template rec<D>(nd1[c]);
+        }
+        const double* ksp = a.Ks_elem != nullptr ? a.Ks_elem : nullptr;
+        const double ks0 = ksp != nullptr ? ksp[ic0.x >> 2] : 1.0;
+        const double ks1 = ksp != nullptr ? ksp[ic1.x >> 2] : 1.0;
+        if (tid <= nrows) {
+            rp_s[tid] = rpv - base;
+            ip_s[tid] = ipv - q0;
+        }
+        if (tid < nrows) {
+            dg_s[tid] = dgv - base;
+            bc_s[tid] = (uint8_t)bcv;
+            cs_s[tid] = ci;
+            rs_s[tid] = rsv;
+        }
+        if constexpr (MODE != ASM_INIT_NONE)
+            if (tid < wlen) xw[tid] = x0k;
+        elem_pair(tid, ic0, ic1, rc0, rc1, ks0, ks1);
+    }
+    for (int j = threadIdx.x + ASM_THREADS; j <= nrows; j += ASM_THREADS) {
         rp_s[j] = a.row_ptr[r0 + j] - base;
         ip_s[j] = a.inc_ptr[r0 + j] - q0;
         if (j < nrows) {
@@ -215,12 +292,8 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
         }
     }
     if constexpr (MODE != ASM_INIT_NONE)
-        for (int k = threadIdx.x; k < wlen; k += ASM_THREADS) xw[k] = ki.x0[a.cols[base + k]];
-
-    // 1. element row contributions of every incidence of the workgroup, two
-    //    per thread and pass: the incidence, element and record gathers of
-    //    both are issued together (three dependent round trips per pair)
-    for (int t0 = threadIdx.x; t0 < nq; t0 += 2 * ASM_THREADS) {
+        for (int k = threadIdx.x + ASM_THREADS; k < wlen; k += ASM_THREADS) xw[k] = ki.x0[a.cols[base + k]];
+    for (int t0 = threadIdx.x + 2 * ASM_THREADS; t0 < nq; t0 += 2 * ASM_THREADS) {
         const int t1 = t0 + ASM_THREADS;
         const bool two = t1 < nq;
         const int2 ic0 = a.inc[q0 + t0];
@@ -229,34 +302,15 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
         const int4 cn1 = a.conn4[ic1.x >> 2];
         const int nd0[4] = {cn0.x, cn0.y, cn0.z, cn0.w};
         const int nd1[4] = {cn1.x, cn1.y, cn1.z, cn1.w};
-        double4 r0[K], r1[K];
+        double4 rc0[K], rc1[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            r0[c] = a.nrec.template rec<D>(nd0[c]);
-            r1[c] = a.nrec.template rec<D>(nd1[c]);
+            rc0[c] = a.nrec.template rec<D>(nd0[c]);
+            rc1[c] = a.nrec.template rec<D>(nd1[c]);
         }
         const double ks0 = a.Ks_elem != nullptr ? a.Ks_elem[ic0.x >> 2] : 1.0;
         const double ks1 = a.Ks_elem != nullptr ? a.Ks_elem[ic1.x >> 2] : 1.0;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (h == 1 && !two) break;
-            const double4* r = h == 0 ? r0 : r1;
-            double px[K], py[K], pz[K], kn[K];
-#pragma unroll
-            for (int c = 0; c < K; ++c) {
-                px[c] = r[c].x;
-                py[c] = r[c].y;
-                pz[c] = (D == 3) ? r[c].z : 0.0;
-                kn[c] = (D == 3) ? r[c].w : r[c].z;
-            }
-            double out[D + 3];
-            const int2 ic = h == 0 ? ic0 : ic1;
-            element_compute<D>(a, px, py, pz, kn, h == 0 ? ks0 : ks1, ic.x & 3, out);
-            const int t = h == 0 ? t0 : t1;
-            slw[t] = (uint32_t)ic.y;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) con[t * NC + c] = out[c];
-        }
+        elem_pair(t0, ic0, ic1, rc0, rc1, ks0, ks1);
     }
     __syncthreads();
 
