@@ -1024,7 +1024,10 @@ void launch_spmv(wspm_ctx* ctx, const T* av, const Epi& epi, int which, bool tim
         if (ctx->ell_col16) {
             const int16_t* c16 = ctx->ell_col16;
             if (ctx->ell_w == 7) go(k_spmv_ell<7, T, Epi, int16_t>, dim3(eg), ctx->n, c16, av, epi, red, tflag, tab);
-            else go(k_spmv_ell<8, T, Epi, int16_t>, dim3(eg), ctx->n, c16, av, epi, red, tflag, tab);
+            else if (ctx->ell_w == 8) go(k_spmv_ell<8, T, Epi, int16_t>, dim3(eg), ctx->n, c16, av, epi, red, tflag, tab);
+            else if (ctx->ell_w == ELL_W3D - 1)
+                go(k_spmv_ell<ELL_W3D - 1, T, Epi, int16_t>, dim3(eg), ctx->n, c16, av, epi, red, tflag, tab);
+            else go(k_spmv_ell<ELL_W3D, T, Epi, int16_t>, dim3(eg), ctx->n, c16, av, epi, red, tflag, tab);
         } else if (ctx->ell_w == 7) {
             go(k_spmv_ell<7, T, Epi>, dim3(eg), ctx->n, (const int32_t*)ctx->ell_col, av, epi, red, tflag, tab);
         } else {
@@ -1896,7 +1899,7 @@ static void do_assemble(wspm_ctx* ctx, const double* Ks_elem, const double* h_m,
     a.bc_flag = ctx->n_bc > 0 ? ctx->bc_flag : nullptr;
     a.bc_val = ctx->bc_val;
     a.A = A; a.A32 = A32; a.b = b; a.dinv = dinv; a.dinv32 = dinv32;
-    a.ell_n = ell ? ctx->n : 0;    // callers pass ell only with the register kernels
+    a.ell_n = ell ? ctx->n : 0;    // the register kernels, or the 3D LDS kernel
     int kernel = ctx->asm_kernel;
     if (ell && ctx->fan) {
         a.ell_col = ctx->ell_col;       // 32-bit columns: the 16-bit offsets measured slower here
@@ -2421,19 +2424,27 @@ static int check_richards_params(wspm_ctx* ctx, const wspm_richards_params* prm)
 
 // Values held for the time-step operator: CSR, or ELL of width max_row.
 static size_t op_cap(const wspm_ctx* ctx) {
-    const size_t ell = (ctx->dim == 2 && ctx->max_row <= 8) ? (size_t)ctx->max_row * ctx->n : 0;
+    const size_t ell = ((ctx->dim == 2 && ctx->max_row <= 8) || (ctx->dim == 3 && ctx->max_row <= ELL_W3D))
+                           ? (size_t)ctx->max_row * ctx->n : 0;
     return std::max((size_t)ctx->nnz, ell);
 }
 
-// ELL for the time steps: 2D register assembly, rows of 7 or 8 entries, at
-// most 25 % padding, and a Krylov solve with no preconditioner or Jacobi
-// (the general preconditioners and the direct solver read CSR).
-// WSPM_SPMV=csr keeps CSR.
+// ELL for the time steps: 2D register assembly with rows of 7 or 8 entries,
+// or 3D (the LDS assembly) with rows of 15 or 16 (a tetrahedral grid's
+// interior rows have 15), at most 25 % padding, and a Krylov solve with no
+// preconditioner or Jacobi (the general preconditioners and the direct
+// solver read CSR).  3D also needs every column within 16-bit offset range
+// of its row (checked when the table is built).  WSPM_SPMV=csr keeps CSR.
 static bool ell_shape_ok(const wspm_ctx* ctx) {
     const char* env = std::getenv("WSPM_SPMV");
     if (env && std::string(env) == "csr") return false;
-    if (ctx->dim != 2 || !ctx->asm_reg || ctx->asm_kernel == ASM_K_LDS) return false;
-    if (ctx->max_row != 7 && ctx->max_row != 8) return false;
+    if (ctx->dim == 2) {
+        if (!ctx->asm_reg || ctx->asm_kernel == ASM_K_LDS) return false;
+        if (ctx->max_row != 7 && ctx->max_row != 8) return false;
+    } else {
+        if (ctx->asm_kernel != ASM_K_LDS || ctx->max_row < ELL_W3D - 1 || ctx->max_row > ELL_W3D) return false;
+        if (env_is("WSPM_ELL_COLS", "32")) return false;
+    }
     return (double)ctx->max_row * ctx->n <= 1.25 * (double)ctx->nnz;
 }
 
@@ -2638,8 +2649,9 @@ static void richards_alloc(wspm_ctx* ctx, const wspm_richards_params* prm, const
     }
     // the time-step solves read the operator in ELL order when the rows allow
     ctx->ell = ell_ok(ctx, prm);
+    if (ctx->ell) ensure_ell(ctx);
+    if (ctx->ell && ctx->dim == 3 && !ctx->ell_col16) ctx->ell = false;   // columns beyond 16-bit offsets
     if (ctx->ell) {
-        ensure_ell(ctx);
         // padding slots stay zero: the assembly writes only the real entries
         HIPCHK(hipMemsetAsync(ctx->Av, 0, op_cap(ctx) * sizeof(double), ctx->stream));
         if (ctx->A32) HIPCHK(hipMemsetAsync(ctx->A32, 0, op_cap(ctx) * sizeof(float), ctx->stream));

@@ -374,8 +374,10 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
             }
         }
         win[k] = v;
-        a.A[base + k] = v;
-        if (a.A32 != nullptr) a.A32[base + k] = (float)v;
+        // CSR order, or ELL (entry sl of row r0 + j at sl * ell_n + r0 + j)
+        const size_t o = a.ell_n ? (size_t)sl * a.ell_n + (size_t)(r0 + j) : (size_t)base + k;
+        a.A[o] = v;
+        if (a.A32 != nullptr) a.A32[o] = (float)v;
         if (diag) {
             const int i = r0 + j;
             a.b[i] = bi;

@@ -10,6 +10,7 @@ namespace wspm {
 constexpr int BLOCK = 256;          // threads per workgroup (4 wavefronts of 64)
 constexpr int NWAVE = BLOCK / 64;
 constexpr int MAX_BLOCKS = 2048;    // cap on the grids of SpMV / vector reductions
+constexpr int ELL_W3D = 16;         // widest 3D ELL row (a tetrahedral grid's interior rows have 15)
 constexpr int MAX_DOTS = 8;         // dot products per reduction kernel
 constexpr int MAX_RESTART = 64;
 
