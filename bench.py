@@ -586,10 +586,18 @@ def run_single(args):
     if mixed:
         # the FP64 form on the same problem, same steps, same tolerance
         _, c64 = make("fp64", tol)
-        for _ in range(args.warmup + args.steps):
+        for _ in range(args.warmup):
             c64.richards_step(dt)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter()
+        for _ in range(args.steps):
+            c64.richards_step(dt)
+        torch.cuda.synchronize()
+        rate64 = args.steps / (time.perf_counter() - t64)
         h64, th64, _ = (x.cpu().numpy() for x in c64.richards_state())
         c64.close()
+        cfgd["fp64_same_tol_steps_per_s"] = round(rate64, 3)
+        cfgd["speedup_vs_fp64_same_tol"] = round(value / rate64, 3)
         cfgd["inner"] = _inner_desc(args)
         cfgd["rel_l2_vs_fp64_h"] = float(np.linalg.norm(h - h64) / np.linalg.norm(h64))
         cfgd["rel_l2_vs_fp64_theta"] = float(np.linalg.norm(th - th64) / np.linalg.norm(th64))
