@@ -114,7 +114,10 @@ struct PicardOp {
     // (rdone, rstall, kry), so that one read-back tells the host all three
     const KState* ref = nullptr;
     __device__ bool skip(const KState* s) const {
-        return (gate != nullptr && *gate == 0) || (ens && (s->pdone || !s->done || s->pev));
+        if (!ens) return gate != nullptr && *gate == 0;   // uniform
+        // the three flags loaded together (no load behind another's test)
+        const int pd = s->pdone, dn = s->done, pv = s->pev;
+        return (pd != 0) | (dn == 0) | (pv != 0);
     }
     __device__ void load(const KState* s) {
         if (ens) sp = s->sp;
@@ -359,7 +362,7 @@ struct RefResidE {
     // iteration count.
     double stall = 0.0;
     int steps = 0;
-    __device__ bool skip(const KState* s) const { return !first && s->rdone; }
+    __device__ bool skip(const KState* s) const { return (!first) & (s->rdone != 0); }
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return x[j]; }
     __device__ void row(int i, double ax, double* acc) const {
@@ -405,7 +408,7 @@ struct JacFirst {     // d = M r~ + M (r~ - A M r~); x += |r| d (x set) or d, r~
     double* x;
     int gated;
     double inv, scale;
-    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
     __device__ void load(const KState* s) {
         scale = s->rnorm;
         inv = scale > 0.0 ? 1.0 / scale : 0.0;
@@ -433,7 +436,7 @@ struct JacOnly {      // one step: x += |r| M r~
     double* x;
     int gated;
     double inv, scale;
-    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
     __device__ void load(const KState* s) {
         scale = s->rnorm;
         inv = scale > 0.0 ? 1.0 / scale : 0.0;
@@ -451,7 +454,7 @@ struct JacStep {      // d' = d + M (r~ - A d); x += |r| d' (x set) or d' stored
     double* x;
     int gated;
     double scale;
-    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
+    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
     __device__ void load(const KState* s) { scale = s->rnorm; }
     __device__ double xv(int j) const { return (double)din[j]; }
     __device__ void row(int i, double ax, double*) const {

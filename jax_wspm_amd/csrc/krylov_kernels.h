@@ -127,10 +127,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int n, const int32_t* __restrict
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;
-        if (epi.skip(st)) continue;
-        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
+        // the state's skip flags and the epilogue's scalars in one round trip
         Epi e = epi;
+        const bool sk = epi.skip(st);
         e.load(st);
+        if (sk) continue;
+        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
@@ -196,10 +198,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(int n, const CI* __restrict_
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;
-        if (epi.skip(st)) continue;
-        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
+        // the state's skip flags and the epilogue's scalars in one round trip
         Epi e = epi;
+        const bool sk = epi.skip(st);
         e.load(st);
+        if (sk) continue;
+        if (tflag != nullptr && threadIdx.x == 0 && tb.w > 0) atomicAdd(tflag, tb.w);
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
@@ -234,9 +238,11 @@ __global__ __launch_bounds__(BLOCK) void k_vec(int n, Op op, Red red, EnsWalk ew
         const int t = ew.entry(v);
         const RowTab tb = ew.tab[t];
         KState* st = red.st + tb.z;
-        if (op.skip(st)) continue;
+        // the state's skip flags and the functor's scalars in one round trip
         Op o = op;
+        const bool sk = op.skip(st);
         o.load(st);
+        if (sk) continue;
         double acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.0;
@@ -359,7 +365,7 @@ template <class T>
 struct BicgT {  // t = A s_hat; omega = (t.s)/(t.t)
     static constexpr int ND = 2;
     KVec<T> v;
-    __device__ bool skip(const KState* s) const { return s->done || s->exit_early; }
+    __device__ bool skip(const KState* s) const { return (s->done != 0) | (s->exit_early != 0); }
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return (double)v.sh[j]; }
     __device__ void row(int i, double ax, double* acc) const {
@@ -880,7 +886,7 @@ struct GmArnoldi {  // w = M (A V[:, j]); v0 = |w|
     static constexpr int ND = 1;
     KVec<T> v;
     int j;
-    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ bool skip(const KState* s) const { return (s->done != 0) | (s->active == 0); }
     __device__ void load(const KState*) {}
     __device__ double xv(int c) const { return (double)v.V[(long)j * v.ldv + c]; }
     __device__ void row(int i, double ax, double* acc) const {
@@ -899,7 +905,7 @@ struct GmUpdate {  // w -= V[:, :j+1] h; v1 = |w| (threshold eps * v0); H row j
     KVec<T> v;
     int j;
     double* H;   // [restart][restart+1], row-major
-    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ bool skip(const KState* s) const { return (s->done != 0) | (s->active == 0); }
     __device__ void load(const KState*) {}
     __device__ void row(int i, double* acc) const {
         double wi = (double)v.w[i];
@@ -929,7 +935,7 @@ struct GmNormalize {  // V[:, j+1] = w / v1 (0 when v1 == 0)
     KVec<T> v;
     int j;
     double v1;
-    __device__ bool skip(const KState* s) const { return s->done || s->jdone != j; }
+    __device__ bool skip(const KState* s) const { return (s->done != 0) | (s->jdone != j); }
     __device__ void load(const KState* s) { v1 = s->v1; }
     __device__ void row(int i, double*) const {
         v.V[(long)(j + 1) * v.ldv + i] = v1 > 0.0 ? (T)((double)v.w[i] / v1) : (T)0;
@@ -1011,7 +1017,7 @@ struct GmPcNorm {  // Arnoldi: w = M (A v) (in t); v0 = |w|
     static constexpr int ND = 1;
     KVec<T> v;
     const T* mw;
-    __device__ bool skip(const KState* s) const { return s->done || !s->active; }
+    __device__ bool skip(const KState* s) const { return (s->done != 0) | (s->active == 0); }
     __device__ void load(const KState*) {}
     __device__ void row(int i, double* acc) const {
         const T wi = mw[i];
