@@ -362,7 +362,7 @@ struct RefResidE {
     // iteration count.
     double stall = 0.0;
     int steps = 0;
-    __device__ bool skip(const KState* s) const { return (!first) & (s->rdone != 0); }
+    __device__ bool skip(const KState* s) const { return !first && s->rdone; }
     __device__ void load(const KState*) {}
     __device__ double xv(int j) const { return x[j]; }
     __device__ void row(int i, double ax, double* acc) const {
@@ -408,7 +408,7 @@ struct JacFirst {     // d = M r~ + M (r~ - A M r~); x += |r| d (x set) or d, r~
     double* x;
     int gated;
     double inv, scale;
-    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
     __device__ void load(const KState* s) {
         scale = s->rnorm;
         inv = scale > 0.0 ? 1.0 / scale : 0.0;
@@ -436,7 +436,7 @@ struct JacOnly {      // one step: x += |r| M r~
     double* x;
     int gated;
     double inv, scale;
-    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
     __device__ void load(const KState* s) {
         scale = s->rnorm;
         inv = scale > 0.0 ? 1.0 / scale : 0.0;
@@ -454,7 +454,7 @@ struct JacStep {      // d' = d + M (r~ - A d); x += |r| d' (x set) or d' stored
     double* x;
     int gated;
     double scale;
-    __device__ bool skip(const KState* s) const { return (gated != 0) & (s->rdone != 0); }
+    __device__ bool skip(const KState* s) const { return gated && s->rdone; }
     __device__ void load(const KState* s) { scale = s->rnorm; }
     __device__ double xv(int j) const { return (double)din[j]; }
     __device__ void row(int i, double ax, double*) const {
@@ -2901,10 +2901,11 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
             if (direct_info(ctx)) throw ApiFail{WSPM_EINVAL, "direct solve: singular matrix"};
         } else if (mixed) {
             const float* d32 = jac ? ctx->dinv32 : nullptr;
-            if (own_start)   // the refinement starts from h_m
+            if (own_start) {   // the refinement starts from h_m
                 ++ctx->n_launch;
                 HIPCHK(hipMemcpyAsync(ctx->hB, ctx->hA, ctx->n * sizeof(double), hipMemcpyDeviceToDevice,
                                       ctx->stream));
+            }
             bool started = !own_start, refined = false;
             if (jacobi_on(ctx, d32)) {
                 // Jacobi sweeps and the Picard update enqueued behind the
@@ -3800,7 +3801,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                 });
                 run.enq = enq0 + 2;
                 ++ctx->n_launch;
-            HIPCHK(hipGraphLaunch(ex, ctx->stream));
+                HIPCHK(hipGraphLaunch(ex, ctx->stream));
                 wait_published(ctx);
                 continue;
             }
