@@ -230,31 +230,45 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pub(Op op, const double* __res
 // the counts are published as by k_publish.
 __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__ st, int n_inst, int* out,
                                                       int* dseq, int* hseq) {
-    __shared__ int cnt[3];
-    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    __shared__ int stalled;
-    if (threadIdx.x == 0) stalled = 0;
+    // every flag of an instance loaded at once (no load behind another's
+    // test), counts combined by wave butterflies and the 4 wave totals
     int kact = 0, pact = 0, ms = 0, sta = 0;
     for (int i = threadIdx.x; i < n_inst; i += BLOCK) {
-        if (!st[i].pdone) {
-            ++pact;
-            if (!st[i].done) ++kact;
-            if (st[i].rstall) ++sta;
-        }
-        ms = max(ms, st[i].steps);
+        const int pd = st[i].pdone, dn = st[i].done, rs = st[i].rstall, sp = st[i].steps;
+        pact += pd == 0;
+        kact += (pd == 0) & (dn == 0);
+        sta += (pd == 0) & (rs != 0);
+        ms = max(ms, sp);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        kact += __shfl_xor(kact, o, 64);
+        pact += __shfl_xor(pact, o, 64);
+        sta += __shfl_xor(sta, o, 64);
+        ms = max(ms, __shfl_xor(ms, o, 64));
+    }
+    __shared__ int wv[4][NWAVE];
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        wv[0][w] = kact;
+        wv[1][w] = pact;
+        wv[2][w] = ms;
+        wv[3][w] = sta;
     }
     __syncthreads();
-    atomicAdd(&cnt[0], kact);
-    atomicAdd(&cnt[1], pact);
-    atomicMax(&cnt[2], ms);
-    atomicAdd(&stalled, sta);
-    __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = cnt[0];
-        out[1] = cnt[1];
-        out[2] = cnt[2];
-        out[3] = stalled;
+        int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            c0 += wv[0][w];
+            c1 += wv[1][w];
+            c2 = max(c2, wv[2][w]);
+            c3 += wv[3][w];
+        }
+        out[0] = c0;
+        out[1] = c1;
+        out[2] = c2;
+        out[3] = c3;
     }
     if (hseq != nullptr) publish_seq(dseq, hseq);
 }
