@@ -282,7 +282,8 @@ __global__ __launch_bounds__(BLOCK) void k_ens_status(const KState* __restrict__
 // mode 0: Krylov solve running; 1: Picard iterating; 2: mixed-precision
 // refinement still running (rdone clear).
 __device__ inline bool ens_on(const KState& s, int mode) {
-    return !s.pdone && (mode == 1 || (mode == 2 ? !s.rdone : !s.done));
+    const int pd = s.pdone, rd = s.rdone, dn = s.done;   // loaded together
+    return (pd == 0) & ((mode == 1) | (mode == 2 ? rd == 0 : dn == 0));
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict__ st, int n_inst,
@@ -290,8 +291,10 @@ __global__ __launch_bounds__(BLOCK) void k_ens_compact(const KState* __restrict_
                                                        int* act_zero) {
     const int i = blockIdx.x;
     int c = 0;
-    for (int j = threadIdx.x; j < i; j += BLOCK)
-        if (ens_on(st[j], mode)) c += iblk[j + 1] - iblk[j];
+    for (int j = threadIdx.x; j < i; j += BLOCK) {
+        const int nb = iblk[j + 1] - iblk[j];
+        c += ens_on(st[j], mode) ? nb : 0;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     __shared__ int ws[NWAVE];
@@ -323,7 +326,8 @@ __global__ __launch_bounds__(BLOCK) void k_finish_ens_k(Op op, const double* __r
                                                         const int* __restrict__ iblk, KState* st0, int* act_out,
                                                         int* act_zero, int mode) {
     KState* st = st0 + blockIdx.x;
-    if (!(st->pdone || op.skip(st))) {   // uniform over the workgroup
+    const bool pd = st->pdone != 0, sk = op.skip(st);   // loaded together
+    if (!(pd | sk)) {   // uniform over the workgroup
         double acc[ND];
         ens_reduce<ND>(partials, prow, iblk[blockIdx.x], iblk[blockIdx.x + 1], acc);
         if (threadIdx.x == 0) op.fin(acc, st);

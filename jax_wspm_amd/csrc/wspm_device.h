@@ -250,7 +250,8 @@ template <int ND, class Op, bool RESET_PEV = false>
 __global__ __launch_bounds__(BLOCK) void k_finish_ens(Op op, const double* __restrict__ partials, int prow,
                                                        const int* __restrict__ iblk, KState* st0) {
     KState* st = st0 + blockIdx.x;
-    if (st->pdone || op.skip(st)) return;
+    const bool pd = st->pdone != 0, sk = op.skip(st);   // loaded together
+    if (pd | sk) return;
     double acc[ND];
     ens_reduce<ND>(partials, prow, iblk[blockIdx.x], iblk[blockIdx.x + 1], acc);
     if (threadIdx.x == 0) {
