@@ -1684,6 +1684,16 @@ int32_t wspm_mesh_rcm(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32
     return WSPM_OK;
 }
 
+// Workgroups of a one-system CSR SpMV: each takes a chunk of rows that fills
+// one pass of its lane groups (BLOCK / lanes groups, SPMV_UNROLL rows each),
+// at most MAX_BLOCKS workgroups.  (Chunks of one row per group left three
+// quarters of each pass idle and multiplied the partial rows the next
+// kernel reduces: the 66k-row species SpMVs of the nitrogen runs.)
+static int csr_spmv_grid(long n, int lanes) {
+    const long rows = (long)(BLOCK / lanes) * SPMV_UNROLL;
+    return (int)std::min<long>(std::max<long>((n + rows - 1) / rows, 1), MAX_BLOCKS);
+}
+
 int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int32_t* conn,
                         const double* coords, void* stream, wspm_ctx** out) {
     if (!out || !conn || !coords) {
@@ -1735,8 +1745,7 @@ int32_t wspm_ctx_create(int32_t dim, int32_t n_nodes, int32_t n_elems, const int
         int lanes = 2;
         while (lanes < 64 && lanes < avg) lanes *= 2;
         ctx->lanes = lanes;
-        const long rows_per_block = BLOCK / lanes;
-        ctx->spmv_grid = (int)std::min<long>((n_nodes + rows_per_block - 1) / rows_per_block, MAX_BLOCKS);
+        ctx->spmv_grid = csr_spmv_grid(n_nodes, lanes);
         ctx->vec_grid = std::min((n_nodes + BLOCK - 1) / BLOCK, 4096);
         ctx->prow = std::max(ctx->prow, ctx->vec_grid);
         ctx->row_ptr = dupload(ctx, P.row_ptr.data(), P.row_ptr.size());
@@ -2194,7 +2203,7 @@ int32_t wspm_krylov_noalloc(int32_t method, int32_t n, int64_t nnz, const int32_
     int lanes = 2;
     while (lanes < 64 && lanes < avg) lanes *= 2;
     c.lanes = lanes;
-    c.spmv_grid = (int)std::min<long>((n + BLOCK / lanes - 1) / (BLOCK / lanes), MAX_BLOCKS);
+    c.spmv_grid = csr_spmv_grid(n, lanes);
     c.vec_grid = std::min((n + BLOCK - 1) / BLOCK, 4096);
     c.prow = (int)NOALLOC_PROW;
     c.row_ptr = const_cast<int32_t*>(row_ptr);
