@@ -2297,11 +2297,11 @@ static bool jacobi_on(const wspm_ctx* ctx, const float* dinv32) {
 }
 
 // Refinement sweeps enqueued per read-back with Jacobi inner steps (each is a
-// few launches; the ones after convergence return at once).  On the
-// 256-instance ensemble 2, 3, 4, 6, 8 and 12 sweeps of 3 steps per read-back
-// measured 60.9k-65.7k instance-steps/s, within one box's spread from 3 on;
-// WSPM_JAC_SWEEPS overrides it.
-static int jac_sweeps() { return env_int("WSPM_JAC_SWEEPS", 3); }
+// few launches; the ones after convergence return at once).  Measured, 2 to
+// 5 sweeps of 3 steps, same box: the 256-instance ensemble is fastest at 4
+// (+3 % over 3), one 512^2 or 64^3 system at 3 (4: -15 % and -2 %).
+// WSPM_JAC_SWEEPS overrides both.
+static int jac_sweeps(bool ens) { return env_int("WSPM_JAC_SWEEPS", ens ? 4 : 3); }
 
 // One refinement sweep with Jacobi inner steps: ctx->prm.inner_iters steps on
 // the correction, x += |r| d, then the FP64 residual r = b - A x with its
@@ -2349,7 +2349,7 @@ static void mixed_solve(wspm_ctx* ctx, int method, const double* A, const double
         if (!(hs.rnorm > thresh) || sweeps >= max_refine) break;
         if (hs.rstall) ctx->jac_inner_off = true;
         if (jac && jacobi_on(ctx, dinv32)) {
-            const int ns = std::min(jac_sweeps(), max_refine - sweeps);
+            const int ns = std::min(jac_sweeps(false), max_refine - sweeps);
             for (int k = 0; k < ns; ++k) jacobi_sweep(ctx, A, A32, dinv32, b, x, tol, atol, 2, false);
             sweeps += ns;
             continue;
@@ -2942,7 +2942,7 @@ int32_t wspm_richards_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                     launch_spmv<double>(ctx, ctx->Av, RefResidE{ctx->bv, ctx->hB, ctx->ref_r, P.tol, P.atol, 1}, 2,
                                         true);
                 started = true;
-                for (int k = 0; k < jac_sweeps(); ++k)
+                for (int k = 0; k < jac_sweeps(false); ++k)
                     jacobi_sweep(ctx, ctx->Av, ctx->A32, d32, ctx->bv, ctx->hB, P.tol, P.atol, 2, false);
                 PicardOp popr = pop;
                 popr.ref = states_of(ctx, 2);
@@ -3757,7 +3757,7 @@ static void ens_step(wspm_ctx* ctx, double dt, wspm_step_stats* stats) {
                 // Picard update of those refined (the others skip it), read
                 // back once; more sweeps and updates while any instance refines
                 const bool fc = fused_compact(ctx);
-                const int ns = jac_sweeps();
+                const int ns = jac_sweeps(true);
                 while (true) {
                     for (int k = 0; k < ns; ++k) {
                         jacobi_sweep(ctx, ctx->Av, ctx->A32, ctx->dinv32, ctx->bv, ctx->hB, P.tol, P.atol, 0, fc);
