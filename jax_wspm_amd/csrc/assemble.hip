@@ -206,9 +206,10 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AsmArgs a, KrylovInit 
     //      Each thread's first pass of the three (its row, its window entry,
     //      its first pair of incidences) is loaded at once with indices
     //      clamped into range, so that no load waits behind a branch or
-    //      behind an independent one: the incidences, metadata and columns,
-    //      then the elements and x0, then the node records -- three round
-    //      trips instead of six.  Longer blocks finish in the loops below.
+    //      behind an independent one: after the block's row pointers, the
+    //      incidences, metadata and columns, then the elements and x0, then
+    //      the node records -- three round trips instead of six.  Longer
+    //      blocks finish in the loops below.
     auto elem_pair = [&](int t0, const int2 ic0, const int2 ic1, const double4 (&r0c)[K], const double4 (&r1c)[K],
                          double ks0, double ks1) {
         const int t1 = t0 + ASM_THREADS;
