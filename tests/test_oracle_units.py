@@ -1,6 +1,11 @@
 """Pin the oracle's pieces by mathematics (closed forms, identities, brute force)
 and by the reference's mesh fixtures.  Each check is chosen so a plausible
-mistake (dropped term, wrong sign or index, transposed operand) fails it."""
+mistake (dropped term, wrong sign or index, transposed operand) fails it.
+
+The van Genuchten model rests on these checks alone: the reference commits
+no numeric output of its van Genuchten path (its published runs use the
+Gardner model), so C, K and theta are pinned here by closed forms and
+high-precision arithmetic, not by reference vectors."""
 import hashlib
 import json
 
