@@ -4,8 +4,10 @@ mistake (dropped term, wrong sign or index, transposed operand) fails it.
 
 The van Genuchten model rests on these checks alone: the reference commits
 no numeric output of its van Genuchten path (its published runs use the
-Gardner model), so C, K and theta are pinned here by closed forms and
-high-precision arithmetic, not by reference vectors."""
+Gardner model), so C, K and theta are pinned here by identities (C = dtheta/dh
+by central differences, the saturated limits, the Mualem form at a point
+computed by hand) and, for the GPU kernel, by 40-digit arithmetic in
+test_gpu_kernels.py -- not by reference vectors."""
 import hashlib
 import json
 
