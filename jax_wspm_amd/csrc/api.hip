@@ -2299,9 +2299,10 @@ static bool jacobi_on(const wspm_ctx* ctx, const float* dinv32) {
 // Refinement sweeps enqueued per read-back with Jacobi inner steps (each is a
 // few launches; the ones after convergence return at once).  Measured, 2 to
 // 5 sweeps of 3 steps, same box: the 256-instance ensemble is fastest at 4
-// (+3 % over 3), one 512^2 or 64^3 system at 3 (4: -15 % and -2 %).
+// (+3 % over 3); one 512^2 system at 2 (5,690-5,780 steps/s against
+// 4,960-5,650 at 3 and about 4,150 at 4), one 64^3 system alike at 2 and 3.
 // WSPM_JAC_SWEEPS overrides both.
-static int jac_sweeps(bool ens) { return env_int("WSPM_JAC_SWEEPS", ens ? 4 : 3); }
+static int jac_sweeps(bool ens) { return env_int("WSPM_JAC_SWEEPS", ens ? 4 : 2); }
 
 // One refinement sweep with Jacobi inner steps: ctx->prm.inner_iters steps on
 // the correction, x += |r| d, then the FP64 residual r = b - A x with its
