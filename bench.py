@@ -80,6 +80,7 @@ WORKLOADS = {
 DEFAULT_STEPS = {"c2": 100, "c2-fp64": 100, "c3": 100, "c3-fp64": 100, "c3-mixed": 100, "c4": 200}
 # oracle steps of the CPU baseline: about 10-20 s of one host core each
 DEFAULT_CPU_STEPS = {"c1": 10, "c1-mixed": 10, "c2": 3, "c2-fp64": 3, "c3": 15}
+DEFAULT_INNER_ITERS = {"c1-mixed": 2}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
@@ -101,7 +102,8 @@ def parse():
     ap.add_argument("--inner", choices=["krylov", "jacobi"], default="jacobi",
                     help="mixed forms: FP32 inner solves of each refinement sweep (jacobi: Jacobi-preconditioned "
                          "Richardson steps; krylov: the method's BiCGSTAB / CG to the inner tolerance)")
-    ap.add_argument("--inner-iters", type=int, default=3, help="--inner jacobi: steps per refinement sweep")
+    ap.add_argument("--inner-iters", type=int, default=None,
+                    help="--inner jacobi: steps per refinement sweep (default 2 for c1-mixed, else 3: measured)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="c3: timed steps per instance of the plain single-instance path (0 = skip)")
     ap.add_argument("--fd-dirs", type=int, default=16, help="c4: random directions of the finite-difference check")
@@ -114,6 +116,12 @@ def parse():
         args.steps = DEFAULT_STEPS.get(args.workload, 200)
     if args.cpu_steps is None:
         args.cpu_steps = DEFAULT_CPU_STEPS.get(args.workload, 0)
+    if args.inner_iters is None:
+        # Jacobi steps per sweep, measured per workload on the final kernels:
+        # 512^2 one system 2 (5,923-6,214 steps/s against 5,692-5,784 at 3),
+        # 64^3 and the ensemble 3 (2: 1,262-1,389 against 1,437-1,455 steps/s,
+        # and 67.1k against 70.9k instance-steps/s)
+        args.inner_iters = DEFAULT_INNER_ITERS.get(args.workload, 3)
     return args
 
 
